@@ -1,0 +1,736 @@
+// Host runtime behind include/dext_hip.h: plans (device buffers for a set of
+// frame shapes), the per-thread workspace of the drop-in find(), window and
+// profiling policy, graphs, errors.
+//
+// Reference counterparts: src/dext_gpu.cc (window policy, announce, stage
+// times), src/dext_cuda.cu:442-728 (workspace, stream guard, find). The
+// structure here is this build's own: a plan owns every buffer a launch
+// needs, a frame is thresholded by one fused kernel and one ordering kernel,
+// and the list comes back already ascending, so there is no host sort.
+#include "sf_device.hh"
+
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace sf {
+void upload_sqrt_tables();
+hipError_t launch_zero(const FrameDesc *, int, hipStream_t);
+hipError_t launch_fused(int, bool, bool, const FrameDesc *, int, uint32_t, int,
+                        hipStream_t);
+hipError_t launch_order(const FrameDesc *, int, hipStream_t);
+hipError_t launch_plain(int, const FrameDesc *, int, uint32_t, uint32_t,
+                        uint8_t *, uint8_t *, hipStream_t, hipEvent_t,
+                        hipEvent_t);
+} // namespace sf
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(const char *what, hipError_t err) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(err);
+  return SF_DEVICE_ERROR;
+}
+
+int bad(const char *what) {
+  g_last_error = what;
+  return SF_BAD_ARGUMENT;
+}
+
+#define SF_CHECK(call, what)                                                   \
+  do {                                                                         \
+    const hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess)                                                      \
+      return fail(what, e_);                                                   \
+  } while (0)
+
+// The square-root table, once per device.
+std::once_flag g_tables_once;
+void ensure_tables() { std::call_once(g_tables_once, sf::upload_sqrt_tables); }
+
+// ---------------------------------------------------------------------------
+// Window policy (src/dext_gpu.cc:39-135): environment default, setters,
+// announce on change.
+// ---------------------------------------------------------------------------
+std::atomic<int> g_window0{1}, g_window2{1};
+std::once_flag g_windows_once;
+std::atomic<bool> g_profiling{false};
+std::atomic<bool> g_reporting{true};
+std::atomic<int> g_announced{-1};
+
+void resolve_one(const char *suffix, std::atomic<int> &into) {
+  char name[64];
+  std::snprintf(name, sizeof(name), "SPOTFINDER_GPU_%s", suffix);
+  const char *asked = std::getenv(name);
+  if (!asked)
+    return;
+  if (std::strcmp(asked, "direct") == 0)
+    into = 0;
+  else if (std::strcmp(asked, "tile") == 0)
+    into = 1;
+  else
+    std::fprintf(stderr,
+                 "warning: %s=%s is not 'tile' or 'direct'; leaving it alone\n",
+                 name, asked);
+}
+
+void resolve_windows() {
+  std::call_once(g_windows_once, [] {
+    resolve_one("STAGE0", g_window0);
+    resolve_one("STAGE2", g_window2);
+  });
+}
+
+const char *window_name(int w) { return w ? "tile" : "direct"; }
+
+void announce(int w0, int w2) {
+  if (!g_reporting.load(std::memory_order_relaxed))
+    return;
+  const int now = w0 * 2 + w2;
+  if (g_announced.exchange(now, std::memory_order_relaxed) == now)
+    return;
+  std::fprintf(stderr, "HIP: stage0 %s window, stage2 %s window (%s form)\n",
+               window_name(w0), window_name(w2),
+               (w0 && w2) ? "fused" : "plain");
+}
+
+thread_local double g_stage_times[6] = {0, 0, 0, 0, 0, 0};
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Plans.
+// ---------------------------------------------------------------------------
+struct sf_plan {
+  int pixel_bytes = 2;
+  int n = 0;
+  std::vector<uint32_t> heights, widths;
+  std::vector<uint32_t> capacity;
+  int band_rows = 236;
+  int form = 0;   // 0 fused, 1 plain
+  int blocks = 0; // 0: one block per item
+  hipStream_t stream = nullptr;
+  bool timing = false;
+
+  std::vector<sf::FrameDesc> desc; // host copy
+  sf::FrameDesc *h_desc = nullptr; // pinned staging for the upload
+  sf::FrameDesc *d_desc = nullptr;
+  sf_signal_pixel *d_staging = nullptr, *d_out = nullptr;
+  uint2 *d_cells = nullptr;
+  uint32_t *d_counters = nullptr, *d_found = nullptr;
+  uint8_t *d_mask0 = nullptr, *d_mask1 = nullptr;
+  size_t max_pixels = 0;
+  uint32_t *h_found = nullptr;
+  uint32_t items = 0;
+  bool any_vec = false, any_scalar = false;
+  hipEvent_t ev[6] = {};
+  hipEvent_t uploaded = nullptr;
+  bool upload_pending = false;
+  std::vector<const void *> bound;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_form = -1;
+};
+
+namespace {
+
+uint32_t strips_for(const sf_plan *p, uint32_t width, int form) {
+  if (form == 1)
+    return (width + 63) / 64;
+  const uint32_t ow = p->pixel_bytes == 2 ? sf::strip_width<uint16_t>()
+                                          : sf::strip_width<uint32_t>();
+  return (width + ow - 1) / ow;
+}
+
+void plan_free(sf_plan *p) {
+  if (p->stream)
+    (void)hipStreamSynchronize(p->stream);
+  if (p->exec)
+    (void)hipGraphExecDestroy(p->exec);
+  if (p->graph)
+    (void)hipGraphDestroy(p->graph);
+  for (hipEvent_t &e : p->ev)
+    if (e)
+      (void)hipEventDestroy(e);
+  if (p->uploaded)
+    (void)hipEventDestroy(p->uploaded);
+  (void)hipFree(p->d_desc);
+  (void)hipFree(p->d_staging);
+  (void)hipFree(p->d_out);
+  (void)hipFree(p->d_cells);
+  (void)hipFree(p->d_counters);
+  (void)hipFree(p->d_found);
+  (void)hipFree(p->d_mask0);
+  (void)hipFree(p->d_mask1);
+  if (p->h_desc)
+    (void)hipHostFree(p->h_desc);
+  if (p->h_found)
+    (void)hipHostFree(p->h_found);
+  if (p->stream)
+    (void)hipStreamDestroy(p->stream);
+}
+
+// Lay out descriptors for the current form (strip width differs by form).
+void layout(sf_plan *p) {
+  uint32_t item = 0;
+  size_t staging = 0, cells = 0;
+  p->any_vec = p->any_scalar = false;
+  for (int f = 0; f < p->n; f++) {
+    sf::FrameDesc &d = p->desc[f];
+    d.height = p->heights[f];
+    d.width = p->widths[f];
+    d.n_strips = strips_for(p, d.width, p->form);
+    d.band_rows = (uint32_t)p->band_rows;
+    d.n_bands = (d.height + d.band_rows - 1) / d.band_rows;
+    d.capacity = p->capacity[f];
+    d.item_begin = item;
+    item += d.n_strips * d.n_bands;
+    d.staging = p->d_staging + staging;
+    d.out = p->d_out + staging;
+    staging += d.capacity;
+    d.cells = p->d_cells + cells;
+    cells += (size_t)d.height * ((d.width + 63) / 64);
+    d.counter = p->d_counters + f;
+    d.out_count = p->d_found + f;
+  }
+  p->items = item;
+}
+
+int upload_desc(sf_plan *p, const void *const *frames) {
+  const int k = p->pixel_bytes == 2 ? 4 : 2;
+  bool changed = p->bound.size() != (size_t)p->n;
+  if (!changed)
+    for (int f = 0; f < p->n; f++)
+      changed |= p->bound[f] != frames[f];
+  p->any_vec = p->any_scalar = false;
+  for (int f = 0; f < p->n; f++) {
+    sf::FrameDesc &d = p->desc[f];
+    d.pixels = frames[f];
+    const bool aligned = (reinterpret_cast<uintptr_t>(frames[f]) % 8) == 0;
+    d.vec = (aligned && d.width % (uint32_t)k == 0) ? 1u : 0u;
+    p->any_vec |= d.vec != 0;
+    p->any_scalar |= d.vec == 0;
+  }
+  if (!changed)
+    return SF_OK;
+  // The pinned staging may still be read by an earlier upload.
+  if (p->upload_pending)
+    SF_CHECK(hipEventSynchronize(p->uploaded), "waiting for an upload");
+  std::memcpy(p->h_desc, p->desc.data(), sizeof(sf::FrameDesc) * p->n);
+  SF_CHECK(hipMemcpyAsync(p->d_desc, p->h_desc, sizeof(sf::FrameDesc) * p->n,
+                          hipMemcpyHostToDevice, p->stream),
+           "uploading frame descriptors");
+  SF_CHECK(hipEventRecord(p->uploaded, p->stream), "hipEventRecord");
+  p->upload_pending = true;
+  p->bound.assign(frames, frames + p->n);
+  return SF_OK;
+}
+
+int enqueue(sf_plan *p, bool timing) {
+  SF_CHECK(sf::launch_zero(p->d_desc, p->n, p->stream), "zeroing counters");
+  if (timing)
+    SF_CHECK(hipEventRecord(p->ev[0], p->stream), "hipEventRecord");
+  if (p->form == 0) {
+    SF_CHECK(sf::launch_fused(p->pixel_bytes, p->any_vec, p->any_scalar,
+                              p->d_desc, p->n, p->items, p->blocks,
+                              p->stream),
+             "launching the fused threshold");
+    if (timing) {
+      SF_CHECK(hipEventRecord(p->ev[1], p->stream), "hipEventRecord");
+      SF_CHECK(hipEventRecord(p->ev[2], p->stream), "hipEventRecord");
+    }
+  } else {
+    for (int f = 0; f < p->n; f++) {
+      const bool first = f == 0 && timing;
+      SF_CHECK(sf::launch_plain(p->pixel_bytes, p->d_desc, f, p->heights[f],
+                                p->widths[f], p->d_mask0, p->d_mask1,
+                                p->stream, first ? p->ev[1] : nullptr,
+                                first ? p->ev[2] : nullptr),
+               "launching the plain threshold");
+    }
+  }
+  if (timing)
+    SF_CHECK(hipEventRecord(p->ev[3], p->stream), "hipEventRecord");
+  SF_CHECK(sf::launch_order(p->d_desc, p->n, p->stream),
+           "launching the ordering kernel");
+  if (timing)
+    SF_CHECK(hipEventRecord(p->ev[4], p->stream), "hipEventRecord");
+  return SF_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
+                            const uint32_t *heights, const uint32_t *widths,
+                            int band_rows) {
+  if ((pixel_bytes != 2 && pixel_bytes != 4) || n_frames <= 0 || !heights ||
+      !widths) {
+    bad("sf_hip_plan_create: bad argument");
+    return nullptr;
+  }
+  for (int f = 0; f < n_frames; f++) {
+    if (!heights[f] || !widths[f] || heights[f] > 0x7fffffffu / widths[f]) {
+      bad("sf_hip_plan_create: a frame has no pixels or too many");
+      return nullptr;
+    }
+  }
+  ensure_tables();
+  sf_plan *p = new sf_plan;
+  p->pixel_bytes = pixel_bytes;
+  p->n = n_frames;
+  p->heights.assign(heights, heights + n_frames);
+  p->widths.assign(widths, widths + n_frames);
+  if (band_rows > 0)
+    p->band_rows = ((band_rows + 20 + 15) & ~15) - 20;
+  size_t staging = 0, cells = 0;
+  for (int f = 0; f < n_frames; f++) {
+    const size_t px = (size_t)heights[f] * widths[f];
+    // The list holds one entry per 16 bytes of frame, as the reference's
+    // device buffer does (src/dext_cuda.cu:614-616).
+    p->capacity.push_back((uint32_t)(px * (size_t)pixel_bytes / 16));
+    staging += p->capacity.back();
+    cells += (size_t)heights[f] * ((widths[f] + 63) / 64);
+    p->max_pixels = px > p->max_pixels ? px : p->max_pixels;
+  }
+  p->desc.resize(n_frames);
+  bool ok = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) ==
+            hipSuccess;
+  ok = ok && hipMalloc(&p->d_desc, sizeof(sf::FrameDesc) * n_frames) ==
+                 hipSuccess;
+  ok = ok && hipHostMalloc(&p->h_desc, sizeof(sf::FrameDesc) * n_frames) ==
+                 hipSuccess;
+  ok = ok && hipMalloc(&p->d_staging,
+                       sizeof(sf_signal_pixel) * (staging ? staging : 1)) ==
+                 hipSuccess;
+  ok = ok && hipMalloc(&p->d_out, sizeof(sf_signal_pixel) *
+                                      (staging ? staging : 1)) == hipSuccess;
+  ok = ok && hipMalloc(&p->d_cells, sizeof(uint2) * cells) == hipSuccess;
+  ok = ok && hipMalloc(&p->d_counters, sizeof(uint32_t) * n_frames) ==
+                 hipSuccess;
+  ok = ok && hipMalloc(&p->d_found, sizeof(uint32_t) * n_frames) ==
+                 hipSuccess;
+  ok = ok && hipHostMalloc(&p->h_found, sizeof(uint32_t) * n_frames) ==
+                 hipSuccess;
+  for (hipEvent_t &e : p->ev)
+    ok = ok && hipEventCreate(&e) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming) ==
+                 hipSuccess;
+  if (!ok) {
+    g_last_error = "sf_hip_plan_create: device allocation failed";
+    plan_free(p);
+    delete p;
+    return nullptr;
+  }
+  layout(p);
+  return p;
+}
+
+void sf_hip_plan_destroy(sf_plan *plan) {
+  if (!plan)
+    return;
+  plan_free(plan);
+  delete plan;
+}
+
+int sf_hip_plan_set_form(sf_plan *p, int form) {
+  if (!p || (form != 0 && form != 1))
+    return bad("sf_hip_plan_set_form: bad argument");
+  if (form == p->form)
+    return SF_OK;
+  if (form == 1 && !p->d_mask0) {
+    SF_CHECK(hipMalloc(&p->d_mask0, p->max_pixels), "allocating mask0");
+    SF_CHECK(hipMalloc(&p->d_mask1, p->max_pixels), "allocating mask1");
+  }
+  SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  p->form = form;
+  layout(p);
+  p->bound.clear(); // force a descriptor upload
+  return SF_OK;
+}
+
+int sf_hip_plan_set_grid(sf_plan *p, int blocks) {
+  if (!p || blocks < 0)
+    return bad("sf_hip_plan_set_grid: bad argument");
+  p->blocks = blocks;
+  p->graph_form = -1;
+  return SF_OK;
+}
+
+int sf_hip_plan_set_timing(sf_plan *p, int on) {
+  if (!p)
+    return bad("sf_hip_plan_set_timing: no plan");
+  p->timing = on != 0;
+  return SF_OK;
+}
+
+int sf_hip_plan_run(sf_plan *p, const void *const *frames) {
+  if (!p || !frames)
+    return bad("sf_hip_plan_run: bad argument");
+  for (int f = 0; f < p->n; f++)
+    if (!frames[f])
+      return bad("sf_hip_plan_run: a frame pointer is null");
+  const int up = upload_desc(p, frames);
+  if (up != SF_OK)
+    return up;
+  return enqueue(p, p->timing);
+}
+
+int sf_hip_plan_bind(sf_plan *p, const void *const *frames) {
+  if (!p || !frames)
+    return bad("sf_hip_plan_bind: bad argument");
+  for (int f = 0; f < p->n; f++)
+    if (!frames[f])
+      return bad("sf_hip_plan_bind: a frame pointer is null");
+  return upload_desc(p, frames);
+}
+
+int sf_hip_plan_capture(sf_plan *p, const void *const *frames) {
+  if (!p || !frames)
+    return bad("sf_hip_plan_capture: bad argument");
+  const int up = upload_desc(p, frames);
+  if (up != SF_OK)
+    return up;
+  SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  if (p->exec) {
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  SF_CHECK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal),
+           "hipStreamBeginCapture");
+  const int status = enqueue(p, false);
+  hipGraph_t graph = nullptr;
+  const hipError_t end = hipStreamEndCapture(p->stream, &graph);
+  if (status != SF_OK)
+    return status;
+  SF_CHECK(end, "hipStreamEndCapture");
+  p->graph = graph;
+  SF_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0),
+           "hipGraphInstantiate");
+  p->graph_form = p->form;
+  return SF_OK;
+}
+
+int sf_hip_plan_replay(sf_plan *p) {
+  if (!p || !p->exec)
+    return bad("sf_hip_plan_replay: nothing captured");
+  SF_CHECK(hipGraphLaunch(p->exec, p->stream), "hipGraphLaunch");
+  return SF_OK;
+}
+
+int sf_hip_plan_wait(sf_plan *p) {
+  if (!p)
+    return bad("sf_hip_plan_wait: no plan");
+  SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  p->upload_pending = false;
+  return SF_OK;
+}
+
+int sf_hip_plan_counts(sf_plan *p, uint32_t *counts) {
+  if (!p || !counts)
+    return bad("sf_hip_plan_counts: bad argument");
+  SF_CHECK(hipMemcpyAsync(p->h_found, p->d_found, sizeof(uint32_t) * p->n,
+                          hipMemcpyDeviceToHost, p->stream),
+           "reading counts");
+  SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  std::memcpy(counts, p->h_found, sizeof(uint32_t) * p->n);
+  return SF_OK;
+}
+
+const sf_signal_pixel *sf_hip_plan_output(sf_plan *p, int frame) {
+  if (!p || frame < 0 || frame >= p->n)
+    return nullptr;
+  return p->desc[frame].out;
+}
+
+uint32_t sf_hip_plan_capacity(sf_plan *p, int frame) {
+  if (!p || frame < 0 || frame >= p->n)
+    return 0;
+  return p->capacity[frame];
+}
+
+int sf_hip_plan_last_times(sf_plan *p, double *threshold_ms,
+                           double *order_ms) {
+  if (!p || !p->timing)
+    return bad("sf_hip_plan_last_times: timing is off");
+  float a = 0.0f, b = 0.0f;
+  SF_CHECK(hipEventElapsedTime(&a, p->ev[0], p->ev[3]), "event time");
+  SF_CHECK(hipEventElapsedTime(&b, p->ev[3], p->ev[4]), "event time");
+  if (threshold_ms)
+    *threshold_ms = a;
+  if (order_ms)
+    *order_ms = b;
+  return SF_OK;
+}
+
+void *sf_hip_plan_stream(sf_plan *p) { return p ? (void *)p->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+// Device, memory, policy.
+// ---------------------------------------------------------------------------
+const char *sf_hip_backend(void) { return "HIP"; }
+
+int sf_hip_available(void) {
+  int devices = 0;
+  return hipGetDeviceCount(&devices) == hipSuccess && devices > 0;
+}
+
+size_t sf_hip_memory_free(void) {
+  size_t free_bytes = 0, total = 0;
+  if (hipMemGetInfo(&free_bytes, &total) != hipSuccess)
+    return 0;
+  return free_bytes;
+}
+
+void *sf_hip_host_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes) != hipSuccess)
+    return nullptr;
+  return p;
+}
+
+void sf_hip_host_free(void *p) {
+  if (p)
+    (void)hipHostFree(p);
+}
+
+const char *sf_hip_last_error(void) { return g_last_error.c_str(); }
+
+void *sf_hip_device_alloc(size_t bytes) {
+  void *p = nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    fail("hipMalloc", e);
+    return nullptr;
+  }
+  return p;
+}
+
+void sf_hip_device_free(void *p) {
+  if (p)
+    (void)hipFree(p);
+}
+
+int sf_hip_copy_to_device(void *device, const void *host, size_t bytes) {
+  SF_CHECK(hipMemcpy(device, host, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+  return SF_OK;
+}
+
+int sf_hip_copy_to_host(void *host, const void *device, size_t bytes) {
+  SF_CHECK(hipMemcpy(host, device, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+  return SF_OK;
+}
+
+int sf_hip_synchronize(void) {
+  SF_CHECK(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return SF_OK;
+}
+
+int sf_hip_stage0_window(void) {
+  resolve_windows();
+  return g_window0.load();
+}
+void sf_hip_set_stage0_window(int w) {
+  resolve_windows();
+  g_window0 = w ? 1 : 0;
+}
+int sf_hip_stage2_window(void) {
+  resolve_windows();
+  return g_window2.load();
+}
+void sf_hip_set_stage2_window(int w) {
+  resolve_windows();
+  g_window2 = w ? 1 : 0;
+}
+void sf_hip_profile_stages(int on) { g_profiling = on != 0; }
+int sf_hip_profiling(void) { return g_profiling.load() ? 1 : 0; }
+void sf_hip_last_stage_times(double times[6]) {
+  for (int i = 0; i < 6; i++)
+    times[i] = g_stage_times[i];
+}
+void sf_hip_report_windows(int on) { g_reporting = on != 0; }
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// The drop-in find(): per-thread workspace and stream, synchronous.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Workspace {
+  sf_plan *plan = nullptr;
+  int pixel_bytes = 0;
+  uint32_t height = 0, width = 0;
+  void *frame = nullptr;
+  size_t frame_bytes = 0;
+  hipEvent_t before_upload = nullptr;
+  ~Workspace() {
+    if (plan)
+      sf_hip_plan_destroy(plan);
+    if (frame)
+      (void)hipFree(frame);
+    if (before_upload)
+      (void)hipEventDestroy(before_upload);
+  }
+};
+
+Workspace &workspace() {
+  static thread_local Workspace w;
+  return w;
+}
+
+// Leaves the stream idle on every exit path (the reference's StreamGuard,
+// src/dext_cuda.cu:442-464).
+struct Idle {
+  hipStream_t s;
+  ~Idle() {
+    if (s)
+      (void)hipStreamSynchronize(s);
+  }
+};
+
+template <typename T>
+int find(const T *image, size_t height, size_t width, sf_signal_pixel *out,
+         size_t capacity, size_t *found) {
+  if (found)
+    *found = 0;
+  if (!image || height == 0 || width == 0 || !found)
+    return bad("find: null image or zero size");
+  if (height > 0x7fffffffu / width)
+    return bad("find: frame larger than INT32_MAX pixels");
+  Workspace &w = workspace();
+  const int pb = (int)sizeof(T);
+  const int w0 = sf_hip_stage0_window(), w2 = sf_hip_stage2_window();
+  announce(w0, w2);
+  const int form = (w0 && w2) ? 0 : 1;
+  if (!w.plan || w.pixel_bytes != pb || w.height != height ||
+      w.width != width) {
+    if (w.plan)
+      sf_hip_plan_destroy(w.plan);
+    const uint32_t h32 = (uint32_t)height, w32 = (uint32_t)width;
+    w.plan = sf_hip_plan_create(pb, 1, &h32, &w32, 0);
+    if (!w.plan)
+      return SF_DEVICE_ERROR;
+    w.pixel_bytes = pb;
+    w.height = (uint32_t)height;
+    w.width = (uint32_t)width;
+  }
+  int status = sf_hip_plan_set_form(w.plan, form);
+  if (status != SF_OK)
+    return status;
+  const size_t bytes = height * width * sizeof(T);
+  if (bytes > w.frame_bytes) {
+    (void)hipStreamSynchronize(w.plan->stream);
+    if (w.frame)
+      (void)hipFree(w.frame);
+    w.frame = nullptr;
+    w.frame_bytes = 0;
+    SF_CHECK(hipMalloc(&w.frame, bytes), "allocating the frame");
+    w.frame_bytes = bytes;
+  }
+  if (!w.before_upload)
+    SF_CHECK(hipEventCreate(&w.before_upload), "hipEventCreate");
+  sf_plan *p = w.plan;
+  const Idle idle{p->stream};
+  const bool timing = sf_hip_profiling() != 0;
+  p->timing = timing;
+  if (timing)
+    SF_CHECK(hipEventRecord(w.before_upload, p->stream), "hipEventRecord");
+  SF_CHECK(hipMemcpyAsync(w.frame, image, bytes, hipMemcpyHostToDevice,
+                          p->stream),
+           "copying a frame to the device");
+  if (timing)
+    SF_CHECK(hipEventRecord(p->ev[5], p->stream), "hipEventRecord");
+  const void *frames[1] = {w.frame};
+  status = sf_hip_plan_run(p, frames);
+  if (status != SF_OK)
+    return status;
+  uint32_t n = 0;
+  status = sf_hip_plan_counts(p, &n);
+  if (status != SF_OK)
+    return status;
+  p->upload_pending = false;
+  double times[6] = {0, 0, 0, 0, 0, 0};
+  if (timing) {
+    float ms = 0.0f;
+    SF_CHECK(hipEventElapsedTime(&ms, w.before_upload, p->ev[5]), "time");
+    times[0] = ms;
+    if (form == 0) {
+      SF_CHECK(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]), "time");
+      times[1] = ms;
+      SF_CHECK(hipEventElapsedTime(&ms, p->ev[3], p->ev[4]), "time");
+      times[3] = ms;
+    } else {
+      SF_CHECK(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]), "time");
+      times[1] = ms;
+      SF_CHECK(hipEventElapsedTime(&ms, p->ev[1], p->ev[2]), "time");
+      times[2] = ms;
+      SF_CHECK(hipEventElapsedTime(&ms, p->ev[2], p->ev[4]), "time");
+      times[3] = ms;
+    }
+  }
+  *found = n;
+  if (n > p->capacity[0]) {
+    if (timing)
+      std::memcpy(g_stage_times, times, sizeof(times));
+    g_last_error = "more signal pixels than the device list holds";
+    return SF_OVERFLOW;
+  }
+  const auto copy_started = std::chrono::steady_clock::now();
+  const size_t take = n < capacity ? n : capacity;
+  if (take && out) {
+    SF_CHECK(hipMemcpyAsync(out, p->desc[0].out, take * sizeof(sf_signal_pixel),
+                            hipMemcpyDeviceToHost, p->stream),
+             "copying the signal pixels back");
+    SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  }
+  if (timing) {
+    times[4] = std::chrono::duration<double, std::milli>(
+                   std::chrono::steady_clock::now() - copy_started)
+                   .count();
+    times[5] = 0.0; // the list arrives ordered
+    std::memcpy(g_stage_times, times, sizeof(times));
+  }
+  return SF_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int sf_hip_find_u16(const uint16_t *image, size_t height, size_t width,
+                    sf_signal_pixel *out, size_t capacity, size_t *found) {
+  return find<uint16_t>(image, height, width, out, capacity, found);
+}
+
+int sf_hip_find_u32(const uint32_t *image, size_t height, size_t width,
+                    sf_signal_pixel *out, size_t capacity, size_t *found) {
+  return find<uint32_t>(image, height, width, out, capacity, found);
+}
+
+int sf_hip_take_found(sf_signal_pixel *out, size_t n) {
+  Workspace &w = workspace();
+  if (!w.plan || !out)
+    return bad("sf_hip_take_found: nothing found yet");
+  if (n == 0)
+    return SF_OK;
+  SF_CHECK(hipMemcpyAsync(out, w.plan->desc[0].out,
+                          n * sizeof(sf_signal_pixel), hipMemcpyDeviceToHost,
+                          w.plan->stream),
+           "copying the signal pixels back");
+  SF_CHECK(hipStreamSynchronize(w.plan->stream), "hipStreamSynchronize");
+  return SF_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,352 @@
+"""Python host side of the MI355X spot-finder hot path (ctypes over the C ABI).
+
+Mirrors the reference's device interface (src/dext_gpu.hh, namespace gpu):
+`find` is gpu::find<T> -- same arguments, same ascending output, same -1 / -2
+return codes -- plus `backend`, `available`, `memory_free`, the window and
+profiling knobs, and device-resident batches (`Plan`) and the device
+labeller (`Labeller`, mirroring dials_spots::Labeller) that the reference
+does not have.
+
+Nothing here computes: every call goes to libdext_hip.so, and importing this
+module on a machine without the library raises rather than falling back.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libdext_hip.so")
+
+SF_OK = 0
+SF_BAD_ARGUMENT = -1
+SF_OVERFLOW = -2
+SF_DEVICE_ERROR = -3
+SF_ORDER_ERROR = -4
+
+DIRECT, TILE = 0, 1
+
+SIGNAL = np.dtype(
+    [("index", "<u4"), ("value", "<u4"), ("background", "<f4"),
+     ("population", "<u2"), ("reserved", "<u2")]
+)
+PIXEL = np.dtype([("frame", "<i4"), ("index", "<u4"), ("value", "<u4")])
+SPOT = np.dtype(
+    [("bbox", "<i4", (6,)), ("position", "<f8", (3,)),
+     ("variance", "<f8", (3,)), ("intensity", "<f8"),
+     ("intensity_variance", "<f8"), ("n_signal", "<u4"), ("_pad", "<u4"),
+     ("first", "<u8")]
+)
+COUNTS = ("signal_pixels", "groups", "too_small", "too_large", "separated",
+          "accepted")
+
+_sz = ctypes.c_size_t
+_p = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_i = ctypes.c_int
+
+
+class DeviceError(RuntimeError):
+    """The device failed (gpu::find throws std::runtime_error here)."""
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    def fn(name, restype, *argtypes):
+        f = getattr(lib, name)
+        f.restype = restype
+        f.argtypes = list(argtypes)
+
+    fn("sf_hip_backend", ctypes.c_char_p)
+    fn("sf_hip_available", _i)
+    fn("sf_hip_memory_free", _sz)
+    fn("sf_hip_host_alloc", _p, _sz)
+    fn("sf_hip_host_free", None, _p)
+    fn("sf_hip_last_error", ctypes.c_char_p)
+    fn("sf_hip_device_alloc", _p, _sz)
+    fn("sf_hip_device_free", None, _p)
+    fn("sf_hip_copy_to_device", _i, _p, _p, _sz)
+    fn("sf_hip_copy_to_host", _i, _p, _p, _sz)
+    fn("sf_hip_synchronize", _i)
+    fn("sf_hip_stage0_window", _i)
+    fn("sf_hip_set_stage0_window", None, _i)
+    fn("sf_hip_stage2_window", _i)
+    fn("sf_hip_set_stage2_window", None, _i)
+    fn("sf_hip_profile_stages", None, _i)
+    fn("sf_hip_profiling", _i)
+    fn("sf_hip_last_stage_times", None, ctypes.POINTER(ctypes.c_double))
+    fn("sf_hip_report_windows", None, _i)
+    for name in ("sf_hip_find_u16", "sf_hip_find_u32"):
+        fn(name, _i, _p, _sz, _sz, _p, _sz, ctypes.POINTER(_sz))
+    fn("sf_hip_take_found", _i, _p, _sz)
+    fn("sf_hip_plan_create", _p, _i, _i, ctypes.POINTER(_u32),
+       ctypes.POINTER(_u32), _i)
+    fn("sf_hip_plan_destroy", None, _p)
+    fn("sf_hip_plan_set_form", _i, _p, _i)
+    fn("sf_hip_plan_set_grid", _i, _p, _i)
+    fn("sf_hip_plan_run", _i, _p, ctypes.POINTER(_p))
+    fn("sf_hip_plan_bind", _i, _p, ctypes.POINTER(_p))
+    fn("sf_hip_plan_capture", _i, _p, ctypes.POINTER(_p))
+    fn("sf_hip_plan_replay", _i, _p)
+    fn("sf_hip_plan_wait", _i, _p)
+    fn("sf_hip_plan_counts", _i, _p, ctypes.POINTER(_u32))
+    fn("sf_hip_plan_output", _p, _p, _i)
+    fn("sf_hip_plan_capacity", _u32, _p, _i)
+    fn("sf_hip_plan_set_timing", _i, _p, _i)
+    fn("sf_hip_plan_last_times", _i, _p, ctypes.POINTER(ctypes.c_double),
+       ctypes.POINTER(ctypes.c_double))
+    fn("sf_hip_plan_stream", _p, _p)
+    fn("sf_hip_synth_frames", _i, _p, _i, _u32, _u32, _u32, ctypes.c_uint64,
+       ctypes.c_double, _i, ctypes.c_double, _p, _u32, ctypes.c_double)
+
+
+_lib: ctypes.CDLL | None = None
+
+
+def lib() -> ctypes.CDLL:
+    """The product library. Raises if it is not built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise RuntimeError(
+                f"{LIBRARY} is missing; build it with "
+                "dials-metal-spotfinder_amd/build.py")
+        handle = ctypes.CDLL(LIBRARY)
+        _declare(handle)
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    return lib().sf_hip_last_error().decode()
+
+
+def _check(status: int, what: str) -> None:
+    if status == SF_DEVICE_ERROR:
+        raise DeviceError(f"{what}: {last_error()}")
+    if status != SF_OK:
+        raise ValueError(f"{what}: status {status}: {last_error()}")
+
+
+# --- src/dext_gpu.hh:28-139 -------------------------------------------------
+
+def backend() -> str:
+    return lib().sf_hip_backend().decode()
+
+
+def available() -> bool:
+    return bool(lib().sf_hip_available())
+
+
+def memory_free() -> int:
+    return int(lib().sf_hip_memory_free())
+
+
+def stage0_window(window: int | None = None) -> int:
+    if window is not None:
+        lib().sf_hip_set_stage0_window(window)
+    return int(lib().sf_hip_stage0_window())
+
+
+def stage2_window(window: int | None = None) -> int:
+    if window is not None:
+        lib().sf_hip_set_stage2_window(window)
+    return int(lib().sf_hip_stage2_window())
+
+
+def profile_stages(on: bool | None = None) -> bool:
+    if on is not None:
+        lib().sf_hip_profile_stages(1 if on else 0)
+    return bool(lib().sf_hip_profiling())
+
+
+def last_stage_times() -> dict:
+    times = (ctypes.c_double * 6)()
+    lib().sf_hip_last_stage_times(times)
+    return dict(zip(("upload", "stage0", "stage1", "stage2", "copy", "sort"),
+                    list(times)))
+
+
+def report_windows(on: bool) -> None:
+    lib().sf_hip_report_windows(1 if on else 0)
+
+
+def find(image: np.ndarray):
+    """gpu::find<T> (src/dext_gpu.hh:156-158): (status, ascending list).
+
+    status is 0, -1 (bad argument) or -2 (more signal pixels than
+    height*width*sizeof(T)/16); a device failure raises DeviceError.
+    """
+    if image is None or image.ndim != 2 or image.dtype not in (np.uint16,
+                                                               np.uint32):
+        return SF_BAD_ARGUMENT, np.zeros(0, dtype=SIGNAL)
+    image = np.ascontiguousarray(image)
+    height, width = image.shape
+    fn = lib().sf_hip_find_u16 if image.dtype == np.uint16 else lib().sf_hip_find_u32
+    capacity = 4096
+    out = np.zeros(capacity, dtype=SIGNAL)
+    found = _sz(0)
+    status = fn(image.ctypes.data if image.size else None, height, width,
+                out.ctypes.data, capacity, ctypes.byref(found))
+    if status == SF_DEVICE_ERROR:
+        raise DeviceError(last_error())
+    if status != SF_OK:
+        return status, np.zeros(0, dtype=SIGNAL)
+    n = found.value
+    if n > capacity:
+        out = np.zeros(n, dtype=SIGNAL)
+        _check(lib().sf_hip_take_found(out.ctypes.data, n), "take_found")
+        return SF_OK, out
+    return SF_OK, out[:n].copy()
+
+
+# --- device memory ----------------------------------------------------------
+
+class DeviceBuffer:
+    """A device allocation (hipMalloc), freed on close()/garbage collection."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().sf_hip_device_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise DeviceError(f"device allocation of {nbytes} bytes: "
+                              f"{last_error()}")
+
+    def upload(self, array: np.ndarray, offset: int = 0) -> None:
+        array = np.ascontiguousarray(array)
+        assert offset + array.nbytes <= self.nbytes
+        _check(lib().sf_hip_copy_to_device(self.ptr + offset,
+                                           array.ctypes.data, array.nbytes),
+               "copy to device")
+
+    def download(self, array: np.ndarray, offset: int = 0) -> np.ndarray:
+        assert array.flags.c_contiguous
+        assert offset + array.nbytes <= self.nbytes
+        _check(lib().sf_hip_copy_to_host(array.ctypes.data, self.ptr + offset,
+                                         array.nbytes), "copy to host")
+        return array
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().sf_hip_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def download_signal(device_ptr: int, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=SIGNAL)
+    if n:
+        _check(lib().sf_hip_copy_to_host(out.ctypes.data, device_ptr,
+                                         out.nbytes), "copy to host")
+    return out
+
+
+def synth_frames(buffer: DeviceBuffer, pixel_bytes: int, height: int,
+                 width: int, frames: int, seed: int, background: float,
+                 gaps: bool, hot_fraction: float, spots: np.ndarray,
+                 sigma_z: float = 0.7, offset: int = 0) -> None:
+    """Fill device frames with the synthetic value model (see synth.hip)."""
+    spots = np.ascontiguousarray(spots, dtype=np.float64).reshape(-1, 6)
+    need = pixel_bytes * height * width * frames
+    assert offset + need <= buffer.nbytes
+    _check(lib().sf_hip_synth_frames(
+        buffer.ptr + offset, pixel_bytes, height, width, frames, seed,
+        background, 1 if gaps else 0, hot_fraction,
+        spots.ctypes.data if len(spots) else None, len(spots), sigma_z),
+        "synth_frames")
+
+
+# --- device-resident batches -------------------------------------------------
+
+class Plan:
+    """Fixed frame shapes thresholded together on the device.
+
+    form 0 is the fused single-pass kernel, form 1 the plain three-kernel
+    per-frame form. Frames are device pointers (ints).
+    """
+
+    def __init__(self, pixel_bytes: int, shapes, band_rows: int = 0,
+                 form: int = 0):
+        self.shapes = [(int(h), int(w)) for h, w in shapes]
+        self.pixel_bytes = pixel_bytes
+        n = len(self.shapes)
+        heights = (_u32 * n)(*[h for h, _ in self.shapes])
+        widths = (_u32 * n)(*[w for _, w in self.shapes])
+        self.handle = lib().sf_hip_plan_create(pixel_bytes, n, heights,
+                                               widths, band_rows)
+        if not self.handle:
+            raise DeviceError(f"plan: {last_error()}")
+        if form:
+            self.set_form(form)
+
+    def _frames(self, frames):
+        assert len(frames) == len(self.shapes)
+        return (_p * len(frames))(*[int(f) for f in frames])
+
+    def set_form(self, form: int) -> None:
+        _check(lib().sf_hip_plan_set_form(self.handle, form), "set_form")
+
+    def set_grid(self, blocks: int) -> None:
+        _check(lib().sf_hip_plan_set_grid(self.handle, blocks), "set_grid")
+
+    def set_timing(self, on: bool) -> None:
+        _check(lib().sf_hip_plan_set_timing(self.handle, 1 if on else 0),
+               "set_timing")
+
+    def run(self, frames) -> None:
+        _check(lib().sf_hip_plan_run(self.handle, self._frames(frames)), "run")
+
+    def bind(self, frames) -> None:
+        _check(lib().sf_hip_plan_bind(self.handle, self._frames(frames)),
+               "bind")
+
+    def capture(self, frames) -> None:
+        _check(lib().sf_hip_plan_capture(self.handle, self._frames(frames)),
+               "capture")
+
+    def replay(self) -> None:
+        _check(lib().sf_hip_plan_replay(self.handle), "replay")
+
+    def wait(self) -> None:
+        _check(lib().sf_hip_plan_wait(self.handle), "wait")
+
+    def counts(self) -> np.ndarray:
+        out = (_u32 * len(self.shapes))()
+        _check(lib().sf_hip_plan_counts(self.handle, out), "counts")
+        return np.array(list(out), dtype=np.int64)
+
+    def capacity(self, frame: int) -> int:
+        return int(lib().sf_hip_plan_capacity(self.handle, frame))
+
+    def output_ptr(self, frame: int) -> int:
+        return int(lib().sf_hip_plan_output(self.handle, frame))
+
+    def output(self, frame: int, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = int(self.counts()[frame])
+        count = min(count, self.capacity(frame))
+        return download_signal(self.output_ptr(frame), count)
+
+    def last_times(self):
+        a, b = ctypes.c_double(0), ctypes.c_double(0)
+        _check(lib().sf_hip_plan_last_times(self.handle, ctypes.byref(a),
+                                            ctypes.byref(b)), "last_times")
+        return a.value, b.value
+
+    def close(self) -> None:
+        if self.handle:
+            lib().sf_hip_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,183 @@
+/*
+ * dext_hip -- the C ABI of the MI355X (gfx950) spot-finder hot path.
+ *
+ * Plain C: pointers, sizes and status codes; no exception crosses it and no
+ * HIP or torch type appears in a signature. Each entry point names the
+ * reference interface it stands in for (paths relative to the reference
+ * repository graeme-winter/dials-metal-spotfinder).
+ *
+ * Status codes mirror the reference's return codes and add two:
+ *   SF_OK (0), SF_BAD_ARGUMENT (-1, dext_gpu.hh:141 / dext_cuda.cu:588-593),
+ *   SF_OVERFLOW (-2, more signal pixels than the list holds,
+ *   dext_gpu.hh:148-151), SF_DEVICE_ERROR (-3, where gpu::find throws
+ *   std::runtime_error, dext_gpu.hh:153-155), SF_ORDER_ERROR (-4, where
+ *   dials_spots::Labeller::add throws, dials_spots.cc:58-91).
+ * sf_hip_last_error() returns the calling thread's last message.
+ */
+#ifndef DEXT_HIP_H
+#define DEXT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF_OK 0
+#define SF_BAD_ARGUMENT (-1)
+#define SF_OVERFLOW (-2)
+#define SF_DEVICE_ERROR (-3)
+#define SF_ORDER_ERROR (-4)
+
+/* src/signal_pixel.hh:20-28: 16 bytes, identical layout to SignalPixel. */
+typedef struct sf_signal_pixel {
+  uint32_t index;
+  uint32_t value;
+  float background;
+  uint16_t population;
+  uint16_t reserved;
+} sf_signal_pixel;
+
+/* src/dials_spots.hh:48-52: 12 bytes, identical layout to Pixel. */
+typedef struct sf_pixel {
+  int32_t frame;
+  uint32_t index;
+  uint32_t value;
+} sf_pixel;
+
+/* src/dials_spots.hh:55-79: 104 bytes, identical layout to Spot. */
+typedef struct sf_spot {
+  int32_t bbox[6];
+  double position[3];
+  double variance[3];
+  double intensity;
+  double intensity_variance;
+  uint32_t n_signal;
+  uint64_t first;
+} sf_spot;
+
+/* src/dials_spots.hh:99-106 */
+typedef struct sf_counts {
+  uint64_t signal_pixels, groups, too_small, too_large, separated, accepted;
+} sf_counts;
+
+/* src/dials_spots.hh:81-97 */
+typedef struct sf_options {
+  uint64_t min_spot_size;  /* 3 */
+  uint64_t max_spot_size;  /* 1000 */
+  double max_separation;   /* 2.0; <= 0 turns the filter off */
+  int32_t two_d;           /* 0 */
+} sf_options;
+
+/* ---- device and host memory (dext_gpu.hh:125-139) ---------------------- */
+
+/* dext_gpu.hh:28 backend(): "HIP". */
+const char *sf_hip_backend(void);
+/* dext_gpu.hh:126 available(). */
+int sf_hip_available(void);
+/* dext_gpu.hh:131 memory_free(). */
+size_t sf_hip_memory_free(void);
+/* dext_gpu.hh:138-139 host_alloc / host_free: page-locked (hipHostMalloc). */
+void *sf_hip_host_alloc(size_t bytes);
+void sf_hip_host_free(void *pointer);
+/* The calling thread's last error message ("" if none). */
+const char *sf_hip_last_error(void);
+
+/* Device buffers and copies, for callers that keep frames resident in HBM
+ * (no reference counterpart: the reference always uploads per frame). */
+void *sf_hip_device_alloc(size_t bytes);
+void sf_hip_device_free(void *pointer);
+int sf_hip_copy_to_device(void *device, const void *host, size_t bytes);
+int sf_hip_copy_to_host(void *host, const void *device, size_t bytes);
+int sf_hip_synchronize(void);
+
+/* ---- window choice and profiling (dext_gpu.hh:49-123) -------------------
+ * Window 0 = Direct, 1 = Tile. On this backend the pair selects the form:
+ * (Tile, Tile) runs the fused single-pass kernel; any Direct runs the plain
+ * three-kernel form whose stages sum their windows directly. Defaults are
+ * Tile/Tile, overridable by SPOTFINDER_GPU_STAGE0 / SPOTFINDER_GPU_STAGE2. */
+int sf_hip_stage0_window(void);
+void sf_hip_set_stage0_window(int window);
+int sf_hip_stage2_window(void);
+void sf_hip_set_stage2_window(int window);
+/* dext_gpu.hh:112-116: upload, stage0, stage1, stage2, copy, sort (ms).
+ * For the fused form stage0 holds the fused kernel, stage1 zero and stage2
+ * the ordering kernel. */
+void sf_hip_profile_stages(int on);
+int sf_hip_profiling(void);
+void sf_hip_last_stage_times(double times[6]);
+/* dext_gpu.hh:123 report_windows(). */
+void sf_hip_report_windows(int on);
+
+/* ---- the drop-in threshold (dext_gpu.hh:156-158 gpu::find<T>) -----------
+ * Host frame in, ascending list out, synchronous. Writes at most `capacity`
+ * entries to `out`, stores the number found in *found. Returns SF_OK,
+ * SF_BAD_ARGUMENT, SF_OVERFLOW when found exceeds height*width*sizeof(T)/16
+ * (the device list's size, dext_cuda.cu:614-616), or SF_DEVICE_ERROR.
+ * Uses a per-thread stream and workspace (dext_cuda.cu:468-542). */
+int sf_hip_find_u16(const uint16_t *image, size_t height, size_t width,
+                    sf_signal_pixel *out, size_t capacity, size_t *found);
+int sf_hip_find_u32(const uint32_t *image, size_t height, size_t width,
+                    sf_signal_pixel *out, size_t capacity, size_t *found);
+/* When the last find() on this thread found more than `capacity`, copy its
+ * first n entries (n <= found) into out. */
+int sf_hip_take_found(sf_signal_pixel *out, size_t n);
+
+/* ---- device-resident batches (no reference counterpart: the reference
+ * launches one frame at a time, dext_cuda.cu:603-656) ----------------------
+ * A plan fixes a list of frame shapes of one pixel type (2 or 4 bytes) and
+ * owns the staging, cell tables and ordered output lists for them. run()
+ * thresholds one device-resident frame per shape in one fused launch plus one
+ * ordering launch, asynchronously on the plan's stream. form 0 = fused
+ * (default), 1 = plain three-kernel, one frame per launch sequence. */
+typedef struct sf_plan sf_plan;
+sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
+                            const uint32_t *heights, const uint32_t *widths,
+                            int band_rows);
+void sf_hip_plan_destroy(sf_plan *plan);
+int sf_hip_plan_set_form(sf_plan *plan, int form);
+/* Persistent grid: 0 = one block per work item, >0 = that many blocks
+ * striding over the items. */
+int sf_hip_plan_set_grid(sf_plan *plan, int blocks);
+int sf_hip_plan_run(sf_plan *plan, const void *const *device_frames);
+/* Point the plan at these device frames without launching (the descriptor
+ * upload that run() does first). A captured graph reads the descriptors at
+ * execution, so bind() then replay() thresholds new frames. */
+int sf_hip_plan_bind(sf_plan *plan, const void *const *device_frames);
+/* Capture run()'s launches for the bound frames into a hipGraph; replay()
+ * launches it. The graph clears its counters with a kernel, not a memset. */
+int sf_hip_plan_capture(sf_plan *plan, const void *const *device_frames);
+int sf_hip_plan_replay(sf_plan *plan);
+int sf_hip_plan_wait(sf_plan *plan);
+/* After wait(): per-frame counts (may exceed capacity -> that frame is -2). */
+int sf_hip_plan_counts(sf_plan *plan, uint32_t *counts);
+/* Device pointer to frame f's ordered list, and its capacity. */
+const sf_signal_pixel *sf_hip_plan_output(sf_plan *plan, int frame);
+uint32_t sf_hip_plan_capacity(sf_plan *plan, int frame);
+/* Kernel time of the last run (ms): fused/plain kernels, ordering. Needs
+ * sf_hip_plan_set_timing(plan, 1) before the run. */
+int sf_hip_plan_set_timing(sf_plan *plan, int on);
+int sf_hip_plan_last_times(sf_plan *plan, double *threshold_ms,
+                           double *order_ms);
+/* The plan's stream, as an opaque handle (hipStream_t). */
+void *sf_hip_plan_stream(sf_plan *plan);
+
+/* ---- synthetic frames on the device (benchmark and test input) ----------
+ * Fills `frames` device frames of height x width (pixel_bytes 2 or 4) at
+ * `device` with a Poisson background of mean `background`, masked module
+ * gaps (0xFFFF.. at the module pitch when gaps != 0), hot pixels at
+ * max-1 with probability hot_fraction, and the given spots: each spot is
+ * (x, y, sigma, total counts, first frame, frame span) as doubles, six per
+ * spot, drawn with a Gaussian rocking curve of sigma_z frames. */
+int sf_hip_synth_frames(void *device, int pixel_bytes, uint32_t height,
+                        uint32_t width, uint32_t frames, uint64_t seed,
+                        double background, int gaps, double hot_fraction,
+                        const double *spots, uint32_t n_spots,
+                        double sigma_z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEXT_HIP_H */
