@@ -1,0 +1,248 @@
+"""Benchmark: the spot-finder hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--chunk C]
+
+Workload: a series of 1800 device-resident
+4148 x 4362 (W x H) uint16 frames, Poisson background 0.3, ~1500 Gaussian
+spots a frame (sigma 0.8-2 px, 1-5 frames, peaks up to 20,000), masked
+module gaps at the 514 x 1030 pitch and 0.05% hot pixels -- generated on the
+device from a seed. A step is one chunk of C frames through the hot path:
+the fused threshold with ordered compaction (one graph replay), then the
+device 3-D labelling and centroid pass for the chunk. `value` is frames/s
+over the timed steps with the inputs already in HBM.
+
+Also reported: the fused kernel's roofline (algorithmic bytes per launch /
+its HIP-event time on the plan's stream, against 8 TB/s), the plain
+one-frame-per-launch form on the same frames, and the reference's own CPU
+path (oracle/_ref, compiled from the reference sources) timed on this
+host's cores on a bounded sample, whose lists are also compared with the
+device's (mask_diffs).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "dials-metal-spotfinder_amd"),
+           os.path.join(ROOT, "tests")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HEIGHT, WIDTH = 4362, 4148
+PEAK_HBM = 8.0e12
+METRIC = ("frames/s + HBM GB/s (% peak), 4148×4362 uint16, 0 mask diffs; "
+          "p99 latency")
+
+
+def log(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sf, host_frames, device_lists, threads):
+    """The reference CPU path (or the oracle port) on a bounded sample."""
+    from oracle import cpu
+
+    kind = "reference" if cpu.have_ref() else "port"
+    t0 = time.perf_counter()
+    if kind == "reference":
+        total = cpu.ref_dext_many(host_frames, threads)
+    else:
+        total = cpu.oracle_dext_many(host_frames, threads)
+    elapsed = time.perf_counter() - t0
+    # parity on part of the same sample: every SignalPixel field
+    diffs = 0
+    for f, frame in enumerate(host_frames[:8]):
+        expected = (cpu.ref_dext if kind == "reference" else cpu.dext)(
+            frame, capacity=frame.size // 8)[1]
+        got = device_lists[f]
+        if len(got) != len(expected) or got.tobytes() != expected.tobytes():
+            diffs += 1
+    n = len(host_frames)
+    return {
+        "value": round(n / elapsed, 3),
+        "unit": "frames/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"{n} frames of the same series, threshold (dext) on "
+                  f"{threads} host threads, one frame per thread",
+        "signal_pixels": int(total),
+        "seconds": round(elapsed, 3),
+        "parity_frames": min(n, 8),
+    }, diffs
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--chunk", type=int, default=60)
+    ap.add_argument("--frames", type=int, default=1800)
+    ap.add_argument("--band", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-label", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the hot path has no multi-GPU form "
+              "in this build (frames shard trivially; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import spotfinder as sf
+    import frames as fr
+
+    if not sf.available():
+        print("no HIP device", file=sys.stderr)
+        return 1
+    sf.report_windows(False)
+    frame_bytes = HEIGHT * WIDTH * 2
+    n_frames = args.frames
+    chunk = args.chunk
+    series = sf.DeviceBuffer(frame_bytes * n_frames)
+    spots = fr.diffraction_spots(1, HEIGHT, WIDTH, n_frames, 1500.0)
+    t0 = time.perf_counter()
+    sf.synth_frames(series, 2, HEIGHT, WIDTH, n_frames, 1, 0.3, True,
+                    0.0005, spots)
+    log(f"generated {n_frames} frames in {time.perf_counter() - t0:.2f} s")
+    ptr = [series.ptr + frame_bytes * f for f in range(n_frames)]
+    n_chunks = n_frames // chunk
+
+    plan = sf.Plan(2, [(HEIGHT, WIDTH)] * chunk, band_rows=args.band)
+    labeller = None
+    if not args.no_label and hasattr(sf, "Labeller"):
+        labeller = sf.Labeller(HEIGHT, WIDTH)
+
+    def step(s, timing=False):
+        c = s % n_chunks
+        frames = ptr[c * chunk:(c + 1) * chunk]
+        plan.set_timing(timing)
+        plan.run(frames)
+        if labeller is not None:
+            labeller.add_plan(plan, first_frame=c * chunk,
+                              restart=(c == 0))
+
+    for s in range(args.warmup):
+        step(s)
+    plan.wait()
+    sf.lib().sf_hip_synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(args.warmup + s)
+    plan.wait()
+    if labeller is not None:
+        labeller.wait()
+    sf.lib().sf_hip_synchronize()
+    elapsed = time.perf_counter() - t0
+    fps = args.steps * chunk / elapsed
+
+    # The fused kernel alone, from HIP events on the plan's stream.
+    kernel_ms, order_ms, counts = [], [], []
+    for s in range(5):
+        c = s % n_chunks
+        plan.set_timing(True)
+        plan.run(ptr[c * chunk:(c + 1) * chunk])
+        plan.wait()
+        a, b = plan.last_times()
+        kernel_ms.append(a)
+        order_ms.append(b)
+        counts.append(plan.counts())
+    plan.set_timing(False)
+    kernel_s = float(np.median(kernel_ms)) / 1e3
+    mean_signal = float(np.mean(np.concatenate(counts)))
+    algorithmic = chunk * (frame_bytes + 16.0 * mean_signal)
+    achieved = algorithmic / kernel_s
+    traffic = None
+    traffic_file = os.path.join(ROOT, "profiles", "fused_threshold_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            t = json.load(fh)
+        if t.get("chunk") == chunk:
+            traffic = t.get("bytes_per_launch")
+
+    # The plain one-frame-per-launch form on the same frames.
+    plain = sf.Plan(2, [(HEIGHT, WIDTH)] * chunk, band_rows=args.band,
+                    form=1)
+    plain.run(ptr[:chunk])
+    plain.wait()
+    t1 = time.perf_counter()
+    for s in range(args.plain_steps):
+        c = s % n_chunks
+        plain.run(ptr[c * chunk:(c + 1) * chunk])
+    plain.wait()
+    plain_fps = args.plain_steps * chunk / (time.perf_counter() - t1)
+    plain.close()
+
+    # Device lists for the CPU sample, and the CPU baseline itself.
+    sample = min(args.cpu_sample, chunk)
+    plan.run(ptr[:chunk])
+    plan.wait()
+    counts0 = plan.counts()
+    device_lists = [plan.output(f, int(counts0[f])) for f in range(sample)]
+    host = np.zeros((sample, HEIGHT, WIDTH), dtype=np.uint16)
+    series.download(host)
+    threads = min(os.cpu_count() or 1, 16)
+    baseline, diffs = cpu_baseline(sf, host, device_lists, threads)
+
+    latency = None
+    if hasattr(sf, "serve_latency"):
+        latency = sf.serve_latency()
+
+    record = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic (device-generated, seeded)",
+        "config": {
+            "workload": "config 1: 1800-frame series of 4148x4362 uint16, "
+                        "Poisson 0.3, ~1500 spots/frame, gaps, 0.05% hot",
+            "frames_per_step": chunk,
+            "series_frames": n_frames,
+            "path": "fused threshold + ordered compaction"
+                    + (" + 3-D labelling and centroids" if labeller else ""),
+        },
+        "gb_per_s": round(fps * (frame_bytes + 16 * mean_signal) / 1e9, 2),
+        "hbm_frac_whole_path": round(
+            fps * (frame_bytes + 16 * mean_signal) / PEAK_HBM, 4),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved / 1e9, 2),
+            "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM, 4),
+            "traffic": traffic,
+            "kernel": "fused_kernel<uint16_t, true>",
+            "kernel_ms_per_launch": round(kernel_s * 1e3, 4),
+            "order_ms_per_launch": round(float(np.median(order_ms)), 4),
+            "algorithmic_bytes_per_launch": int(algorithmic),
+        },
+        "plain_form_fps": round(plain_fps, 2),
+        "speedup_vs_plain": round(fps / plain_fps, 3),
+        "mean_signal_pixels_per_frame": round(mean_signal, 1),
+        "mask_diffs": diffs,
+        "cpu_baseline": baseline,
+        "p99_latency_ms": latency,
+    }
+    print(json.dumps(record))
+    plan.close()
+    series.close()
+    return 0 if diffs == 0 else 3
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -23,7 +23,7 @@ void upload_sqrt_tables();
 hipError_t launch_zero(const FrameDesc *, int, hipStream_t);
 hipError_t launch_fused(int, bool, bool, const FrameDesc *, int, uint32_t, int,
                         hipStream_t);
-hipError_t launch_order(const FrameDesc *, int, hipStream_t);
+hipError_t launch_order(const FrameDesc *, int, uint32_t, hipStream_t);
 hipError_t launch_plain(int, const FrameDesc *, int, uint32_t, uint32_t,
                         uint8_t *, uint8_t *, hipStream_t, hipEvent_t,
                         hipEvent_t);
@@ -122,7 +122,9 @@ struct sf_plan {
   sf::FrameDesc *h_desc = nullptr; // pinned staging for the upload
   sf::FrameDesc *d_desc = nullptr;
   sf_signal_pixel *d_staging = nullptr, *d_out = nullptr;
-  uint2 *d_cells = nullptr;
+  uint8_t *d_cell_count = nullptr;
+  uint32_t *d_cell_base = nullptr, *d_band_total = nullptr;
+  uint32_t max_bands = 0;
   uint32_t *d_counters = nullptr, *d_found = nullptr;
   uint8_t *d_mask0 = nullptr, *d_mask1 = nullptr;
   size_t max_pixels = 0;
@@ -163,7 +165,9 @@ void plan_free(sf_plan *p) {
   (void)hipFree(p->d_desc);
   (void)hipFree(p->d_staging);
   (void)hipFree(p->d_out);
-  (void)hipFree(p->d_cells);
+  (void)hipFree(p->d_cell_count);
+  (void)hipFree(p->d_cell_base);
+  (void)hipFree(p->d_band_total);
   (void)hipFree(p->d_counters);
   (void)hipFree(p->d_found);
   (void)hipFree(p->d_mask0);
@@ -179,8 +183,9 @@ void plan_free(sf_plan *p) {
 // Lay out descriptors for the current form (strip width differs by form).
 void layout(sf_plan *p) {
   uint32_t item = 0;
-  size_t staging = 0, cells = 0;
+  size_t staging = 0, cells = 0, bands = 0;
   p->any_vec = p->any_scalar = false;
+  p->max_bands = 0;
   for (int f = 0; f < p->n; f++) {
     sf::FrameDesc &d = p->desc[f];
     d.height = p->heights[f];
@@ -194,8 +199,12 @@ void layout(sf_plan *p) {
     d.staging = p->d_staging + staging;
     d.out = p->d_out + staging;
     staging += d.capacity;
-    d.cells = p->d_cells + cells;
+    d.cell_count = p->d_cell_count + cells;
+    d.cell_base = p->d_cell_base + cells;
     cells += (size_t)d.height * ((d.width + 63) / 64);
+    d.band_total = p->d_band_total + bands;
+    bands += d.n_bands;
+    p->max_bands = d.n_bands > p->max_bands ? d.n_bands : p->max_bands;
     d.counter = p->d_counters + f;
     d.out_count = p->d_found + f;
   }
@@ -257,7 +266,7 @@ int enqueue(sf_plan *p, bool timing) {
   }
   if (timing)
     SF_CHECK(hipEventRecord(p->ev[3], p->stream), "hipEventRecord");
-  SF_CHECK(sf::launch_order(p->d_desc, p->n, p->stream),
+  SF_CHECK(sf::launch_order(p->d_desc, p->n, p->max_bands, p->stream),
            "launching the ordering kernel");
   if (timing)
     SF_CHECK(hipEventRecord(p->ev[4], p->stream), "hipEventRecord");
@@ -290,7 +299,7 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
   p->widths.assign(widths, widths + n_frames);
   if (band_rows > 0)
     p->band_rows = ((band_rows + 20 + 15) & ~15) - 20;
-  size_t staging = 0, cells = 0;
+  size_t staging = 0, cells = 0, bands = 0;
   for (int f = 0; f < n_frames; f++) {
     const size_t px = (size_t)heights[f] * widths[f];
     // The list holds one entry per 16 bytes of frame, as the reference's
@@ -298,6 +307,7 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
     p->capacity.push_back((uint32_t)(px * (size_t)pixel_bytes / 16));
     staging += p->capacity.back();
     cells += (size_t)heights[f] * ((widths[f] + 63) / 64);
+    bands += (heights[f] + (uint32_t)p->band_rows - 1) / (uint32_t)p->band_rows;
     p->max_pixels = px > p->max_pixels ? px : p->max_pixels;
   }
   p->desc.resize(n_frames);
@@ -312,7 +322,11 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
                  hipSuccess;
   ok = ok && hipMalloc(&p->d_out, sizeof(sf_signal_pixel) *
                                       (staging ? staging : 1)) == hipSuccess;
-  ok = ok && hipMalloc(&p->d_cells, sizeof(uint2) * cells) == hipSuccess;
+  ok = ok && hipMalloc(&p->d_cell_count, cells) == hipSuccess;
+  ok = ok && hipMalloc(&p->d_cell_base, sizeof(uint32_t) * cells) ==
+                 hipSuccess;
+  ok = ok && hipMalloc(&p->d_band_total, sizeof(uint32_t) * bands) ==
+                 hipSuccess;
   ok = ok && hipMalloc(&p->d_counters, sizeof(uint32_t) * n_frames) ==
                  hipSuccess;
   ok = ok && hipMalloc(&p->d_found, sizeof(uint32_t) * n_frames) ==

@@ -206,9 +206,9 @@ __device__ __forceinline__ void load_row(const T *pix, int y, int height,
 }
 
 template <typename T, bool VEC>
-__device__ __forceinline__ void threshold_item(const FrameDesc &fd,
-                                               uint32_t strip,
-                                               uint32_t band) {
+__device__ __forceinline__ void threshold_item(
+    const FrameDesc &fd, uint32_t strip, uint32_t band,
+    const typename Traits<T>::R *lut) {
   using Tr = Traits<T>;
   using W = typename Tr::W;
   using R = typename Tr::R;
@@ -228,7 +228,7 @@ __device__ __forceinline__ void threshold_item(const FrameDesc &fd,
   const int r0 = (int)(band * fd.band_rows);
   const int r1 = min(r0 + (int)fd.band_rows, height);
   const int steps = ((r1 - r0) + 20 + 15) & ~15;
-  const R *lut = sqrt_lut<R>();
+  uint32_t band_sum = 0;
 
   // Output columns of this lane: inside the strip's output span and the
   // frame.
@@ -350,11 +350,18 @@ __device__ __forceinline__ void threshold_item(const FrameDesc &fd,
             p10[i] = raw[(j + 6) & 15][i];
           cell = poisson_and_emit<T>(fd, cand, (uint32_t)row, c_lane, h2, p10);
         }
-        if (lane == 0)
-          fd.cells[(size_t)row * fd.n_strips + strip] = cell;
+        band_sum += cell.x;
+        if (lane == 0) {
+          const size_t c = (size_t)row * fd.n_strips + strip;
+          fd.cell_count[c] = (uint8_t)cell.x;
+          if (cell.x)
+            fd.cell_base[c] = cell.y;
+        }
       }
     }
   }
+  if (lane == 0 && band_sum)
+    atomicAdd(&fd.band_total[band], band_sum);
 }
 
 // One wave per block. Items are (frame, band, strip), strip fastest; with
@@ -363,6 +370,12 @@ template <typename T, bool VEC>
 __global__ __launch_bounds__(64) void fused_kernel(const FrameDesc *frames,
                                                    int n_frames,
                                                    uint32_t items) {
+  using R = typename Traits<T>::R;
+  // The square-root table in LDS: a per-lane lookup there costs one LDS
+  // read instead of a dependent load through the vector memory path.
+  __shared__ R lut[64];
+  lut[threadIdx.x] = sqrt_lut<R>()[threadIdx.x];
+  __syncthreads();
   const uint32_t blocks = gridDim.x;
   for (uint32_t b = blockIdx.x; b < ((items + 7) & ~7u); b += blocks) {
     const uint32_t item =
@@ -374,11 +387,12 @@ __global__ __launch_bounds__(64) void fused_kernel(const FrameDesc *frames,
     if (VEC != (fd.vec != 0))
       continue;
     const uint32_t local = item - fd.item_begin;
-    threshold_item<T, VEC>(fd, local % fd.n_strips, local / fd.n_strips);
+    threshold_item<T, VEC>(fd, local % fd.n_strips, local / fd.n_strips, lut);
   }
 }
 
-// Exclusive prefix over a block of 1024 threads (16 waves).
+// Exclusive prefix over a block of 256 threads (4 waves); *total gets the
+// block's sum.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v,
                                                          uint32_t *total,
                                                          uint32_t *lds) {
@@ -394,61 +408,68 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v,
   if (lane == 63)
     lds[wave] = x;
   __syncthreads();
-  if (threadIdx.x < 16) {
-    uint32_t w = lds[threadIdx.x];
-    uint32_t acc = w;
+  uint32_t before = 0, all = 0;
 #pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-      const uint32_t y = __shfl_up(acc, d, 16);
-      if (threadIdx.x >= (uint32_t)d)
-        acc += y;
-    }
-    lds[16 + threadIdx.x] = acc - w; // exclusive per wave
-    if (threadIdx.x == 15)
-      lds[32] = acc;
+  for (int w = 0; w < 4; w++) {
+    before += (uint32_t)w < wave ? lds[w] : 0u;
+    all += lds[w];
   }
+  *total = all;
   __syncthreads();
-  const uint32_t result = lds[16 + wave] + x - v;
-  *total = lds[32];
-  __syncthreads();
-  return result;
+  return before + x - v;
 }
 
-// Lays each frame's cells out in row-major order: the cells' exclusive
-// prefix of counts is where each cell's run goes. One block per frame.
-__global__ __launch_bounds__(1024) void order_kernel(const FrameDesc *frames) {
-  __shared__ uint32_t lds[40];
-  const FrameDesc &fd = frames[blockIdx.x];
-  const uint32_t found = *fd.counter;
-  if (found > fd.capacity) {
-    if (threadIdx.x == 0)
-      *fd.out_count = found;
+// Lays one band of one frame out in row-major cell order, which is ascending
+// index order: the band starts after every earlier band's survivors, and
+// each cell's run after the cells before it in the band. Grid (bands,
+// frames).
+__global__ __launch_bounds__(256) void order_kernel(const FrameDesc *frames) {
+  __shared__ uint32_t lds[8];
+  const FrameDesc &fd = frames[blockIdx.y];
+  const uint32_t band = blockIdx.x;
+  if (band >= fd.n_bands)
     return;
-  }
-  const size_t n_cells = (size_t)fd.height * fd.n_strips;
-  uint32_t dest = 0;
-  for (size_t c0 = 0; c0 < n_cells; c0 += 1024) {
-    const size_t c = c0 + threadIdx.x;
-    uint2 cell = make_uint2(0u, 0u);
-    if (c < n_cells)
-      cell = fd.cells[c];
-    uint32_t tile = 0;
-    const uint32_t at = dest + block_exclusive_scan(cell.x, &tile, lds);
-    // Every cell's run lies inside the reserved staging and the counts sum
-    // to `found`, so neither bound binds; they are kept as guards.
-    for (uint32_t e = 0; e < cell.x; e++)
-      if (at + e < fd.capacity && cell.y + e < fd.capacity)
-        fd.out[at + e] = fd.staging[cell.y + e];
-    dest += tile;
-  }
-  if (threadIdx.x == 0)
+  const uint32_t found = *fd.counter;
+  if (band == 0 && threadIdx.x == 0)
     *fd.out_count = found;
+  if (found > fd.capacity)
+    return;
+  uint32_t mine = 0;
+  for (uint32_t b = threadIdx.x; b < band; b += blockDim.x)
+    mine += fd.band_total[b];
+  uint32_t before = 0;
+  (void)block_exclusive_scan(mine, &before, lds);
+  const size_t c0 = (size_t)band * fd.band_rows * fd.n_strips;
+  const size_t r1 = min((size_t)(band + 1) * fd.band_rows, (size_t)fd.height);
+  const size_t c1 = r1 * fd.n_strips;
+  const size_t per = (c1 - c0 + blockDim.x - 1) / blockDim.x;
+  const size_t my0 = min(c0 + threadIdx.x * per, c1);
+  const size_t my1 = min(my0 + per, c1);
+  uint32_t sum = 0;
+  for (size_t c = my0; c < my1; c++)
+    sum += fd.cell_count[c];
+  uint32_t band_sum = 0;
+  uint32_t dest = before + block_exclusive_scan(sum, &band_sum, lds);
+  for (size_t c = my0; c < my1; c++) {
+    const uint32_t k = fd.cell_count[c];
+    if (!k)
+      continue;
+    const uint32_t base = fd.cell_base[c];
+    // The runs lie inside the reserved staging and the counts sum to
+    // `found`, so neither bound binds; they are kept as guards.
+    for (uint32_t e = 0; e < k; e++)
+      if (dest + e < fd.capacity && base + e < fd.capacity)
+        fd.out[dest + e] = fd.staging[base + e];
+    dest += k;
+  }
 }
 
-__global__ void zero_counters(const FrameDesc *frames, int n_frames) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f < n_frames)
-    *frames[f].counter = 0;
+__global__ void zero_counters(const FrameDesc *frames) {
+  const FrameDesc &fd = frames[blockIdx.x];
+  if (threadIdx.x == 0)
+    *fd.counter = 0;
+  for (uint32_t b = threadIdx.x; b < fd.n_bands; b += blockDim.x)
+    fd.band_total[b] = 0;
 }
 
 } // namespace sf
@@ -476,7 +497,7 @@ void upload_sqrt_tables() {
 
 hipError_t launch_zero(const FrameDesc *frames, int n_frames,
                        hipStream_t stream) {
-  zero_counters<<<(n_frames + 63) / 64, 64, 0, stream>>>(frames, n_frames);
+  zero_counters<<<n_frames, 64, 0, stream>>>(frames);
   return hipGetLastError();
 }
 
@@ -506,8 +527,8 @@ hipError_t launch_fused(int pixel_bytes, bool any_vec, bool any_scalar,
 }
 
 hipError_t launch_order(const FrameDesc *frames, int n_frames,
-                        hipStream_t stream) {
-  order_kernel<<<n_frames, 1024, 0, stream>>>(frames);
+                        uint32_t max_bands, hipStream_t stream) {
+  order_kernel<<<dim3(max_bands, n_frames), 256, 0, stream>>>(frames);
   return hipGetLastError();
 }
 

@@ -132,8 +132,14 @@ __global__ __launch_bounds__(256) void plain_stage2(const FrameDesc *frames,
       }
     }
   }
-  if (lane_id() == 0)
-    fd.cells[(size_t)i * fd.n_strips + blockIdx.x] = make_uint2(total, base);
+  if (lane_id() == 0) {
+    const size_t c = (size_t)i * fd.n_strips + blockIdx.x;
+    fd.cell_count[c] = (uint8_t)total;
+    if (total) {
+      fd.cell_base[c] = base;
+      atomicAdd(&fd.band_total[i / fd.band_rows], total);
+    }
+  }
 }
 
 hipError_t launch_plain(int pixel_bytes, const FrameDesc *frames,

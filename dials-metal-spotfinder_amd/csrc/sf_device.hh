@@ -62,7 +62,9 @@ template <typename T> constexpr int strip_width() {
 struct FrameDesc {
   const void *pixels;          // T[height * width], row-major
   sf_signal_pixel *staging;    // capacity entries, unordered cell runs
-  uint2 *cells;                // height * n_strips: {count, staging offset}
+  uint8_t *cell_count;         // height * n_strips survivors per cell
+  uint32_t *cell_base;         // their staging offset (where count > 0)
+  uint32_t *band_total;        // n_bands: survivors per band of rows
   uint32_t *counter;           // staging allocation counter
   sf_signal_pixel *out;        // ordered list, capacity entries
   uint32_t *out_count;         // found (may exceed capacity: -2)
