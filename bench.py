@@ -126,8 +126,8 @@ def main() -> int:
         plan.set_timing(timing)
         plan.run(frames)
         if labeller is not None:
-            labeller.add_plan(plan, first_frame=c * chunk,
-                              restart=(c == 0))
+            # frame numbers keep ascending as the series is replayed
+            labeller.add_plan(plan, first_frame=s * chunk)
 
     for s in range(args.warmup):
         step(s)

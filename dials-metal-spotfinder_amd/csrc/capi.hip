@@ -748,3 +748,22 @@ int sf_hip_take_found(sf_signal_pixel *out, size_t n) {
 }
 
 } // extern "C"
+
+namespace sf {
+int plan_lists(sf_plan *p, std::vector<const sf_signal_pixel *> &lists,
+               std::vector<const uint32_t *> &counts,
+               std::vector<uint32_t> &capacity, hipStream_t *stream) {
+  if (!p)
+    return SF_BAD_ARGUMENT;
+  lists.resize(p->n);
+  counts.resize(p->n);
+  capacity.resize(p->n);
+  for (int f = 0; f < p->n; f++) {
+    lists[f] = p->desc[f].out;
+    counts[f] = p->desc[f].out_count;
+    capacity[f] = p->capacity[f];
+  }
+  *stream = p->stream;
+  return SF_OK;
+}
+} // namespace sf

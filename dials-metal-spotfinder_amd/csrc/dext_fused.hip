@@ -335,15 +335,17 @@ __device__ __forceinline__ void threshold_item(
       }
       cand_h = (cand_h << K) | (eroded & positive & ~masked5);
 
-      W h2[K];
-      window_across<K, 5>(v2, h2);
-
-      // (5): row y-10 is an output row of this band once t >= 20.
+      // (5): row y-10 is an output row of this band once t >= 20. Its
+      // 11x11 sums are formed across lanes only when some lane of the wave
+      // has a candidate there (about one wave-row in ten on diffraction
+      // frames); the column sums above are kept up to date every row.
       const int row = y - 10;
       if (t >= 20 && row < r1) {
         const uint32_t cand = (cand_h >> (5 * K)) & KMASK & colmask;
         uint2 cell = make_uint2(0u, 0u);
         if (ballot(cand != 0) != 0) {
+          W h2[K];
+          window_across<K, 5>(v2, h2);
           uint32_t p10[K];
 #pragma unroll
           for (int i = 0; i < K; i++)

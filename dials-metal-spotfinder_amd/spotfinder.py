@@ -53,6 +53,15 @@ class DeviceError(RuntimeError):
     """The device failed (gpu::find throws std::runtime_error here)."""
 
 
+class Options(ctypes.Structure):
+    """dials_spots::Options (src/dials_spots.hh:81-97)."""
+
+    _fields_ = [("min_spot_size", ctypes.c_uint64),
+                ("max_spot_size", ctypes.c_uint64),
+                ("max_separation", ctypes.c_double),
+                ("two_d", ctypes.c_int32)]
+
+
 def _declare(lib: ctypes.CDLL) -> None:
     def fn(name, restype, *argtypes):
         f = getattr(lib, name)
@@ -100,6 +109,20 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_plan_stream", _p, _p)
     fn("sf_hip_synth_frames", _i, _p, _i, _u32, _u32, _u32, ctypes.c_uint64,
        ctypes.c_double, _i, ctypes.c_double, _p, _u32, ctypes.c_double)
+    fn("sf_hip_labeller_create", _p, _sz, _sz, ctypes.POINTER(Options))
+    fn("sf_hip_labeller_destroy", None, _p)
+    fn("sf_hip_labeller_add", _i, _p, ctypes.c_int64, _p, _sz)
+    fn("sf_hip_labeller_add_plan", _i, _p, _p, ctypes.c_int64)
+    fn("sf_hip_labeller_flush", _i, _p)
+    fn("sf_hip_labeller_finish", _i, _p)
+    fn("sf_hip_labeller_wait", _i, _p)
+    fn("sf_hip_labeller_live", _sz, _p)
+    fn("sf_hip_labeller_sizes", _i, _p, ctypes.POINTER(_sz),
+       ctypes.POINTER(_sz))
+    fn("sf_hip_labeller_spot_count", _sz, _p)
+    fn("sf_hip_labeller_pixel_count", _sz, _p)
+    fn("sf_hip_labeller_results", _i, _p, _p, _p, _p)
+    fn("sf_hip_labeller_last_error", ctypes.c_char_p)
 
 
 _lib: ctypes.CDLL | None = None
@@ -343,6 +366,84 @@ class Plan:
     def close(self) -> None:
         if self.handle:
             lib().sf_hip_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --- 3-D grouping and centroids (dials_spots::Labeller) ----------------------
+
+class Labeller:
+    """dials_spots::Labeller on the device (src/dials_spots.hh:108-158).
+
+    add() raises ValueError where the reference's add() throws; spots(),
+    pixels() and counts() match the reference's row for row after finish().
+    """
+
+    def __init__(self, height: int, width: int, min_spot_size: int = 3,
+                 max_spot_size: int = 1000, max_separation: float = 2.0,
+                 two_d: bool = False):
+        options = Options(min_spot_size, max_spot_size, float(max_separation),
+                          1 if two_d else 0)
+        self.handle = lib().sf_hip_labeller_create(height, width,
+                                                   ctypes.byref(options))
+        if not self.handle:
+            raise ValueError(lib().sf_hip_labeller_last_error().decode())
+
+    def _status(self, status: int, what: str) -> None:
+        if status == SF_OK:
+            return
+        message = lib().sf_hip_labeller_last_error().decode()
+        if status == SF_DEVICE_ERROR:
+            raise DeviceError(f"{what}: {message}")
+        raise ValueError(f"{what}: {message}")
+
+    def add(self, frame: int, signal: np.ndarray) -> None:
+        signal = np.ascontiguousarray(signal, dtype=SIGNAL)
+        self._status(lib().sf_hip_labeller_add(
+            self.handle, int(frame),
+            signal.ctypes.data if len(signal) else None, len(signal)), "add")
+
+    def add_plan(self, plan: "Plan", first_frame: int) -> None:
+        """Add every frame of the plan's last run as consecutive frames from
+        first_frame, straight from the device lists."""
+        self._status(lib().sf_hip_labeller_add_plan(
+            self.handle, plan.handle, int(first_frame)), "add_plan")
+
+    def flush(self) -> None:
+        self._status(lib().sf_hip_labeller_flush(self.handle), "flush")
+
+    def finish(self) -> None:
+        self._status(lib().sf_hip_labeller_finish(self.handle), "finish")
+
+    def wait(self) -> None:
+        self._status(lib().sf_hip_labeller_wait(self.handle), "wait")
+
+    def live(self) -> int:
+        return int(lib().sf_hip_labeller_live(self.handle))
+
+    def results(self):
+        """(spots, pixels, counts dict), as the reference's spots(),
+        pixels() and counts()."""
+        n_spots, n_pixels = _sz(0), _sz(0)
+        self._status(lib().sf_hip_labeller_sizes(
+            self.handle, ctypes.byref(n_spots), ctypes.byref(n_pixels)),
+            "sizes")
+        spots = np.zeros(n_spots.value, dtype=SPOT)
+        pixels = np.zeros(n_pixels.value, dtype=PIXEL)
+        counts = (ctypes.c_uint64 * 6)()
+        self._status(lib().sf_hip_labeller_results(
+            self.handle, spots.ctypes.data if len(spots) else None,
+            pixels.ctypes.data if len(pixels) else None, counts), "results")
+        return spots, pixels, dict(zip(COUNTS, [int(c) for c in counts]))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().sf_hip_labeller_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -163,6 +163,45 @@ int sf_hip_plan_last_times(sf_plan *plan, double *threshold_ms,
 /* The plan's stream, as an opaque handle (hipStream_t). */
 void *sf_hip_plan_stream(sf_plan *plan);
 
+/* ---- 3-D grouping and centroids on the device
+ * (dials_spots::Labeller, src/dials_spots.hh:108-158) -----------------------
+ * Same options, same output: the Spot and Pixel tables and Counts come out
+ * identical, row for row, to the reference Labeller's after finish(). Frames
+ * are labelled in chunks on the device; a component is emitted once a frame
+ * arrives that it has no pixel on, ordered as the reference orders them. */
+typedef struct sf_labeller sf_labeller;
+/* Labeller(height, width, options); NULL where the constructor throws
+ * (src/dials_spots.cc:22-30). options NULL = the dials.find_spots defaults. */
+sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
+                                    const sf_options *options);
+void sf_hip_labeller_destroy(sf_labeller *labeller);
+/* Labeller::add (src/dials_spots.cc:57-145) with a host list. SF_ORDER_ERROR
+ * where add() throws: a frame not above the last, indices not strictly
+ * ascending, an index off the frame (the frame is then not added). */
+int sf_hip_labeller_add(sf_labeller *labeller, int64_t frame,
+                        const sf_signal_pixel *signal, size_t n);
+/* Add every frame of a plan's last run, as frames first_frame,
+ * first_frame + 1, ... straight from the device lists (no host copy). A
+ * frame whose list overflowed (-2) is added empty. */
+int sf_hip_labeller_add_plan(sf_labeller *labeller, sf_plan *plan,
+                             int64_t first_frame);
+/* Label what has been added so far (queries do this themselves). */
+int sf_hip_labeller_flush(sf_labeller *labeller);
+/* Labeller::finish (src/dials_spots.cc:147-151). */
+int sf_hip_labeller_finish(sf_labeller *labeller);
+int sf_hip_labeller_wait(sf_labeller *labeller);
+/* Labeller::live(): pixels held for components that may still grow. */
+size_t sf_hip_labeller_live(sf_labeller *labeller);
+/* Sizes of spots() and pixels() so far. */
+int sf_hip_labeller_sizes(sf_labeller *labeller, size_t *spots,
+                          size_t *pixels);
+size_t sf_hip_labeller_spot_count(sf_labeller *labeller);
+size_t sf_hip_labeller_pixel_count(sf_labeller *labeller);
+/* spots(), pixels() and counts() (any pointer may be NULL). */
+int sf_hip_labeller_results(sf_labeller *labeller, sf_spot *spots,
+                            sf_pixel *pixels, sf_counts *counts);
+const char *sf_hip_labeller_last_error(void);
+
 /* ---- synthetic frames on the device (benchmark and test input) ----------
  * Fills `frames` device frames of height x width (pixel_bytes 2 or 4) at
  * `device` with a Poisson background of mean `background`, masked module

@@ -1,0 +1,170 @@
+"""Device grouping parity: Spot and Pixel tables and Counts bit-identical to
+the reference Labeller's (tests/test_dials_spots.cc's cases, the golden
+tables written by the compiled reference, random sequences against the
+oracle), through host lists and through device lists carried across chunks.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import cpu
+import frames as fr
+from test_oracle import random_sequence, flat, pixels, PLAIN
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def same_tables(a, b):
+    return (a[0].tobytes() == b[0].tobytes() and
+            a[1].tobytes() == b[1].tobytes() and a[2] == b[2])
+
+
+def label_host(Lab, seq, h, w, **options):
+    lab = Lab(h, w, **options)
+    for z, p in seq:
+        lab.add(z, p)
+    lab.finish()
+    return lab.results()
+
+
+def test_known_answers(sf):
+    """tests/test_dials_spots.cc:71-412 on the device labeller."""
+    H, W = 8, 10
+    close = lambda a, b: abs(a - b) < 1e-9  # noqa: E731
+    spots, kept, counts = label_host(sf.Labeller, [(0, pixels([(0, 1), (1, 2),
+                                                              (2, 1)]))],
+                                     H, W, **PLAIN)
+    s = spots[0]
+    assert len(spots) == 1 and s["n_signal"] == 3
+    assert close(s["position"][0], 1.5) and close(s["variance"][0],
+                                                  0.8 / 4 + 1 / 12)
+    assert list(s["bbox"]) == [0, 3, 0, 1, 0, 1] and len(kept) == 3
+    seq = [(z, flat([0, 1, 2], 1)) for z in range(3)]
+    spots, _, _ = label_host(sf.Labeller, seq, H, W, **PLAIN)
+    assert len(spots) == 1 and spots[0]["n_signal"] == 9
+    assert close(spots[0]["variance"][2], 0.75 / 9 + 1 / 12)
+    spots, _, _ = label_host(sf.Labeller, seq, H, W, two_d=True, **PLAIN)
+    assert len(spots) == 3
+    seq = [(z, flat([0, 1, 2], 1)) for z in (0, 1, 3, 4)]
+    spots, _, _ = label_host(sf.Labeller, seq, H, W, **PLAIN)
+    assert [int(x) for x in spots["n_signal"]] == [6, 6]
+    for idx, expect in (([9, 10], 2), ([0, 10], 1), ([0, 11], 2)):
+        spots, _, _ = label_host(sf.Labeller, [(0, flat(idx, 1))], H, W,
+                                 min_spot_size=1, **PLAIN)
+        assert len(spots) == expect, idx
+    # nothing emitted before it is finished; live() counts what is held
+    lab = sf.Labeller(H, W, **PLAIN)
+    lab.add(0, flat([0, 1, 2], 1))
+    assert len(lab.results()[0]) == 0 and lab.live() == 3
+    lab.add(1, flat([40, 41, 42], 1))
+    assert len(lab.results()[0]) == 1 and lab.live() == 3
+    lab.finish()
+    assert len(lab.results()[0]) == 2 and lab.live() == 0
+    # filters
+    spots, _, counts = label_host(
+        sf.Labeller, [(0, flat([0, 1], 1)), (1, flat([40, 41, 42], 1))], H, W,
+        **PLAIN)
+    assert len(spots) == 1 and counts["too_small"] == 1 and counts["groups"] == 2
+    spots, _, counts = label_host(sf.Labeller, [(0, flat(range(6), 1))], H, W,
+                                  max_spot_size=5, **PLAIN)
+    assert len(spots) == 0 and counts["too_large"] == 1
+    spots, _, counts = label_host(
+        sf.Labeller, [(0, pixels([(0, 100), (1, 100), (2, 1), (3, 1), (4, 1),
+                                  (5, 1), (6, 90), (7, 90)]))], H, W,
+        max_separation=2.0)
+    assert len(spots) == 0 and counts["separated"] == 1
+    # refusals
+    lab = sf.Labeller(H, W, **PLAIN)
+    lab.add(5, flat([0, 1, 2], 1))
+    for frame, p in ((4, flat([0], 1)), (5, flat([0], 1))):
+        with pytest.raises(ValueError):
+            lab.add(frame, p)
+    with pytest.raises(ValueError):
+        sf.Labeller(H, W, **PLAIN).add(0, flat([2, 1], 1))
+    with pytest.raises(ValueError):
+        sf.Labeller(H, W, **PLAIN).add(0, flat([H * W], 1))
+    with pytest.raises(ValueError):
+        sf.Labeller(H, W, min_spot_size=0)
+    # a 200-frame streak, carried frame after frame
+    seq = [(z, flat([44, 45, 46, 70, 71, 72], 3)) for z in range(200)]
+    spots, kept, _ = label_host(sf.Labeller, seq, H, W, **PLAIN)
+    assert len(spots) == 2 and spots[0]["n_signal"] == 600 and len(kept) == 1200
+
+
+def test_random_sequences_match_oracle(sf):
+    rng = np.random.default_rng(21)
+    option_sets = [{}, {"two_d": True}, {"max_separation": 0.0},
+                   {"min_spot_size": 1, "max_spot_size": 20},
+                   {"max_separation": 1.0}]
+    for trial in range(40):
+        h, w = int(rng.integers(3, 60)), int(rng.integers(3, 60))
+        seq = random_sequence(rng, h, w, int(rng.integers(1, 12)),
+                              float(rng.uniform(0.02, 0.5)),
+                              gap_every=int(rng.integers(0, 4)))
+        options = option_sets[trial % len(option_sets)]
+        a = label_host(cpu.Labeller, seq, h, w, **options)
+        b = label_host(sf.Labeller, seq, h, w, **options)
+        assert same_tables(a, b), (trial, options, a[2], b[2])
+
+
+def test_golden_tables_end_to_end(sf):
+    """Device threshold then device grouping against the tables the
+    compiled reference wrote."""
+    data = np.load(os.path.join(GOLDEN, "labeller.npz"))
+    nx = fr.nxmx_fixture()
+    series, _ = fr.planted_series()
+    stacks = {"nxmx": nx, "nxmx_2d": nx, "nxmx_holey": np.delete(nx, 5, axis=0),
+              "config0": series, "config0_nosep": series}
+    options = {"nxmx": {}, "nxmx_2d": {"two_d": True}, "nxmx_holey": {},
+               "config0": {}, "config0_nosep": {"max_separation": 0.0,
+                                                "min_spot_size": 1}}
+    for k in range(int(data["n_labeller"])):
+        name = str(data[f"l{k}_name"])
+        stack = stacks[name]
+        numbers = [int(z) for z in data[f"l{k}_numbers"]]
+        lab = sf.Labeller(stack.shape[1], stack.shape[2], **options[name])
+        for z, frame in zip(numbers, stack):
+            status, signal = sf.find(frame)
+            assert status == 0
+            lab.add(z, signal)
+        lab.finish()
+        spots, kept, counts = lab.results()
+        assert spots.tobytes() == data[f"l{k}_spots"].tobytes(), name
+        assert kept.tobytes() == data[f"l{k}_pixels"].tobytes(), name
+        assert [counts[c] for c in cpu.COUNTS] == list(data[f"l{k}_counts"])
+        if name == "nxmx":
+            assert counts["signal_pixels"] == 8928 and len(spots) == 48
+
+
+def test_device_lists_across_chunks(sf):
+    """Frames thresholded in plans of 4, labelled straight from the device
+    lists with the frontier carried between chunks, against the oracle fed
+    the same frames one at a time."""
+    h, w, n = 300, 412, 12
+    buf = sf.DeviceBuffer(2 * h * w * n)
+    spots = fr.diffraction_spots(5, h, w, n, 1500.0 * 40)
+    sf.synth_frames(buf, 2, h, w, n, 5, 0.3, True, 0.0005, spots)
+    host = np.zeros((n, h, w), dtype=np.uint16)
+    buf.download(host)
+    for two_d in (False, True):
+        oracle = cpu.Labeller(h, w, two_d=two_d)
+        for z in range(n):
+            assert oracle.add(100 + z, cpu.dext(host[z])[1]) == 0
+        oracle.finish()
+        expected = oracle.results()
+        plan = sf.Plan(2, [(h, w)] * 4)
+        lab = sf.Labeller(h, w, two_d=two_d)
+        for c in range(n // 4):
+            plan.run([buf.ptr + 2 * h * w * (4 * c + k) for k in range(4)])
+            lab.add_plan(plan, 100 + 4 * c)
+        lab.finish()
+        got = lab.results()
+        assert got[2] == expected[2], (two_d, got[2], expected[2])
+        assert same_tables(got, expected), two_d
+        assert expected[2]["accepted"] > 20
+        plan.close()
