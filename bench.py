@@ -89,7 +89,20 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-label", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy, three times (the "
+                         "known-size kernel that scales FETCH_SIZE and "
+                         "WRITE_SIZE under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        x = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the hot path has no multi-GPU form "
               "in this build (frames shard trivially; see DESIGN.md)",
