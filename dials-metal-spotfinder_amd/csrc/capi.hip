@@ -9,6 +9,7 @@
 // and the list comes back already ascending, so there is no host sort.
 #include "sf_device.hh"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -27,6 +28,9 @@ hipError_t launch_order(const FrameDesc *, int, uint32_t, hipStream_t);
 hipError_t launch_plain(int, const FrameDesc *, int, uint32_t, uint32_t,
                         uint8_t *, uint8_t *, hipStream_t, hipEvent_t,
                         hipEvent_t);
+hipError_t launch_probe(const FrameDesc *, int, uint32_t, int, hipStream_t);
+int probe_tile_width();
+int probe_tile_height();
 } // namespace sf
 
 namespace {
@@ -113,7 +117,8 @@ struct sf_plan {
   std::vector<uint32_t> heights, widths;
   std::vector<uint32_t> capacity;
   int band_rows = 236;
-  int form = 0;   // 0 fused, 1 plain
+  int form = 0;   // 0 fused (probe at 16 bits, sweep at 32), 1 plain,
+                  // 2 sweep
   int blocks = 0; // 0: one block per item
   hipStream_t stream = nullptr;
   bool timing = false;
@@ -142,9 +147,23 @@ struct sf_plan {
 
 namespace {
 
-uint32_t strips_for(const sf_plan *p, uint32_t width, int form) {
+// The kernel a form runs for this plan's pixel type.
+enum Kernel { kSweep, kPlain, kProbe };
+
+Kernel kernel_for(const sf_plan *p, int form) {
   if (form == 1)
+    return kPlain;
+  if (form == 0 && p->pixel_bytes == 2)
+    return kProbe;
+  return kSweep;
+}
+
+uint32_t strips_for(const sf_plan *p, uint32_t width, int form) {
+  const Kernel k = kernel_for(p, form);
+  if (k == kPlain)
     return (width + 63) / 64;
+  if (k == kProbe)
+    return (width + sf::probe_tile_width() - 1) / sf::probe_tile_width();
   const uint32_t ow = p->pixel_bytes == 2 ? sf::strip_width<uint16_t>()
                                           : sf::strip_width<uint32_t>();
   return (width + ow - 1) / ow;
@@ -191,7 +210,9 @@ void layout(sf_plan *p) {
     d.height = p->heights[f];
     d.width = p->widths[f];
     d.n_strips = strips_for(p, d.width, p->form);
-    d.band_rows = (uint32_t)p->band_rows;
+    d.band_rows = kernel_for(p, p->form) == kProbe
+                      ? (uint32_t)sf::probe_tile_height()
+                      : (uint32_t)p->band_rows;
     d.n_bands = (d.height + d.band_rows - 1) / d.band_rows;
     d.capacity = p->capacity[f];
     d.item_begin = item;
@@ -245,7 +266,16 @@ int enqueue(sf_plan *p, bool timing) {
   SF_CHECK(sf::launch_zero(p->d_desc, p->n, p->stream), "zeroing counters");
   if (timing)
     SF_CHECK(hipEventRecord(p->ev[0], p->stream), "hipEventRecord");
-  if (p->form == 0) {
+  const Kernel kernel = kernel_for(p, p->form);
+  if (kernel == kProbe) {
+    SF_CHECK(sf::launch_probe(p->d_desc, p->n, p->items, p->blocks,
+                              p->stream),
+             "launching the probe threshold");
+    if (timing) {
+      SF_CHECK(hipEventRecord(p->ev[1], p->stream), "hipEventRecord");
+      SF_CHECK(hipEventRecord(p->ev[2], p->stream), "hipEventRecord");
+    }
+  } else if (kernel == kSweep) {
     SF_CHECK(sf::launch_fused(p->pixel_bytes, p->any_vec, p->any_scalar,
                               p->d_desc, p->n, p->items, p->blocks,
                               p->stream),
@@ -307,7 +337,10 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
     p->capacity.push_back((uint32_t)(px * (size_t)pixel_bytes / 16));
     staging += p->capacity.back();
     cells += (size_t)heights[f] * ((widths[f] + 63) / 64);
-    bands += (heights[f] + (uint32_t)p->band_rows - 1) / (uint32_t)p->band_rows;
+    // enough band totals for the shortest band any form uses
+    const uint32_t shortest = (uint32_t)std::min(p->band_rows,
+                                                 sf::probe_tile_height());
+    bands += (heights[f] + shortest - 1) / shortest;
     p->max_pixels = px > p->max_pixels ? px : p->max_pixels;
   }
   p->desc.resize(n_frames);
@@ -355,11 +388,11 @@ void sf_hip_plan_destroy(sf_plan *plan) {
 }
 
 int sf_hip_plan_set_form(sf_plan *p, int form) {
-  if (!p || (form != 0 && form != 1))
+  if (!p || form < 0 || form > 2)
     return bad("sf_hip_plan_set_form: bad argument");
   if (form == p->form)
     return SF_OK;
-  if (form == 1 && !p->d_mask0) {
+  if (kernel_for(p, form) == kPlain && !p->d_mask0) {
     SF_CHECK(hipMalloc(&p->d_mask0, p->max_pixels), "allocating mask0");
     SF_CHECK(hipMalloc(&p->d_mask1, p->max_pixels), "allocating mask1");
   }

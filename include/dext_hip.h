@@ -128,9 +128,11 @@ int sf_hip_take_found(sf_signal_pixel *out, size_t n);
  * launches one frame at a time, dext_cuda.cu:603-656) ----------------------
  * A plan fixes a list of frame shapes of one pixel type (2 or 4 bytes) and
  * owns the staging, cell tables and ordered output lists for them. run()
- * thresholds one device-resident frame per shape in one fused launch plus one
- * ordering launch, asynchronously on the plan's stream. form 0 = fused
- * (default), 1 = plain three-kernel, one frame per launch sequence. */
+ * thresholds one device-resident frame per shape in one threshold launch
+ * plus one ordering launch, asynchronously on the plan's stream.
+ * form 0 = fused (default: the sparse probe-lattice tile kernel for 16-bit
+ * frames, the register-sweep kernel for 32-bit ones), 1 = plain three-kernel
+ * form, one frame per launch sequence, 2 = the register-sweep kernel. */
 typedef struct sf_plan sf_plan;
 sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
                             const uint32_t *heights, const uint32_t *widths,

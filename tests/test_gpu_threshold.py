@@ -105,6 +105,38 @@ def test_random_frames_edges_and_both_paths(sf):
             assert same(got, expected), (shape, dtype, describe(got, expected))
 
 
+@pytest.mark.parametrize("form", [0, 2])
+def test_random_frames_device_resident(sf, form):
+    """The same edge cases through device-resident plans: the probe kernel
+    (form 0 at 16 bits) and the register-sweep kernel (form 2), including
+    widths that are not a multiple of 4 and an unaligned frame start."""
+    rng = np.random.default_rng(40 + form)
+    shapes = [(1, 1), (2, 3), (9, 1), (5, 5), (7, 300), (300, 7), (33, 65),
+              (64, 64), (129, 257), (250, 232), (250, 233), (251, 464),
+              (97, 105), (128, 104), (65, 209)]
+    for shape in shapes:
+        for dtype, mean in ((np.uint16, 0.3), (np.uint16, 30.0),
+                            (np.uint32, 500.0)):
+            frame = random_frame(rng, shape, dtype, mean)
+            expected = cpu.dext(frame)[1]
+            pb = frame.dtype.itemsize
+            for offset in (0, pb):  # aligned, and one pixel in
+                buf = sf.DeviceBuffer(frame.nbytes + 16)
+                buf.upload(frame, offset=offset)
+                plan = sf.Plan(pb, [shape], form=form)
+                plan.run([buf.ptr + offset])
+                plan.wait()
+                n = int(plan.counts()[0])
+                if len(expected) > frame.nbytes // 16:
+                    assert n > plan.capacity(0), shape
+                else:
+                    got = plan.output(0, n)
+                    assert same(got, expected), (shape, dtype, offset,
+                                                 describe(got, expected))
+                plan.close()
+                buf.close()
+
+
 def test_bad_arguments_and_empty(sf):
     assert sf.find(np.zeros((0, 4), np.uint16))[0] == -1
     assert sf.find(np.zeros((4, 0), np.uint32))[0] == -1
@@ -166,7 +198,7 @@ def test_full_size_config1_frames_fused_and_plain(sf):
     buf, host = config1_frames(sf, 2, h, w, seed=101)
     expected = [cpu.dext(host[f], capacity=h * w // 8)[1] for f in range(2)]
     assert all(len(e) > 1000 for e in expected)
-    for form in (0, 1):
+    for form in (0, 1, 2):
         plan = sf.Plan(2, [(h, w)] * 2, form=form)
         plan.run([buf.ptr, buf.ptr + 2 * h * w])
         plan.wait()
@@ -191,6 +223,13 @@ def test_batched_uneven_sizes_graph_and_persistent(sf):
     expected = [cpu.dext(x)[1] for x in hosts]
     for x, e in zip(hosts, expected):
         assert len(e) <= x.nbytes // 16  # every frame fits its list
+    sweep = sf.Plan(2, shapes, form=2)
+    sweep.run([b.ptr for b in bufs])
+    sweep.wait()
+    counts = sweep.counts()
+    for f in range(len(shapes)):
+        assert same(sweep.output(f, int(counts[f])), expected[f]), ("sweep", f)
+    sweep.close()
     plan = sf.Plan(2, shapes)
     ptrs = [b.ptr for b in bufs]
     plan.run(ptrs)
@@ -223,7 +262,7 @@ def test_band_heights(sf):
     buf = sf.DeviceBuffer(frame.nbytes)
     buf.upload(frame)
     for band in (1, 12, 44, 236, 492, 1000):
-        plan = sf.Plan(2, [frame.shape], band_rows=band)
+        plan = sf.Plan(2, [frame.shape], band_rows=band, form=2)
         plan.run([buf.ptr])
         plan.wait()
         assert same(plan.output(0), expected), band
