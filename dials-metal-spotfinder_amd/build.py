@@ -4,9 +4,9 @@
 
 Outputs (all git-ignored, all travel with the tree):
     dials-metal-spotfinder_amd/libdext_hip.so   the product: HIP kernels + C ABI
-    dials-metal-spotfinder_amd/test_dropin.bin.exe
-                                                the C++ drop-in shim's parity
-                                                program (links the .so)
+    tests/test_dropin.bin.exe                   the C++ drop-in headers' parity
+                                                program (links the .so and,
+                                                as its checker, the oracle)
     oracle/liboracle.so                         the CPU oracle (checker only)
     oracle/_ref/libdext_ref.so                  the reference's own CPU path,
                                                 when its sources are present
@@ -28,7 +28,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "libdext_hip.so")
-DROPIN = os.path.join(HERE, "test_dropin.bin.exe")
+DROPIN = os.path.join(ROOT, "tests", "test_dropin.bin.exe")
 REFERENCE = "/root/reference"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -82,20 +82,19 @@ def build_library(jobs: int = 8) -> str:
 
 
 def build_dropin() -> str:
-    """The C++ shim's parity program, built against the library."""
-    source = os.path.join(HERE, "host", "test_dropin.cc")
-    if not os.path.exists(source):
-        return ""
+    """The parity program of the C++ drop-in headers (host/), a test."""
+    source = os.path.join(ROOT, "tests", "test_dropin.cc")
     _run([
-        "g++", "-O2", "-std=c++20", "-ffp-contract=off",
+        "g++", "-O2", "-std=c++20", "-ffp-contract=off", "-Wall",
         "-I", os.path.join(ROOT, "include"),
         "-I", os.path.join(HERE, "host"),
         "-I", os.path.join(ROOT, "oracle"),
         source,
         os.path.join(ROOT, "oracle", "dext_oracle.c"),
         "-o", DROPIN,
-        "-L", HERE, "-ldext_hip", f"-Wl,-rpath,{HERE}",
-        "-lpthread",
+        "-L", HERE, "-ldext_hip",
+        "-Wl,-rpath,$ORIGIN/../dials-metal-spotfinder_amd",
+        "-lpthread", "-lm",
     ])
     return DROPIN
 

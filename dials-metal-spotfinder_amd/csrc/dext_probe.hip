@@ -446,13 +446,15 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   if (tid == 0) {
     int n = 0;
     sh.rounds[n++] = 0;
-    uint32_t start = 0;
-    for (int r = 0; r < G::TH; r++) {
-      const uint32_t end =
-          r + 1 < G::TH ? sh.word_off[(r + 1) * G::WORDS] : total;
-      if (end - start > (uint32_t)G::LIST) {
-        sh.rounds[n++] = (uint32_t)r;
-        start = sh.word_off[r * G::WORDS];
+    if (total > (uint32_t)G::LIST) { // rare: split into rounds of whole rows
+      uint32_t start = 0;
+      for (int r = 0; r < G::TH; r++) {
+        const uint32_t end =
+            r + 1 < G::TH ? sh.word_off[(r + 1) * G::WORDS] : total;
+        if (end - start > (uint32_t)G::LIST) {
+          sh.rounds[n++] = (uint32_t)r;
+          start = sh.word_off[r * G::WORDS];
+        }
       }
     }
     sh.rounds[n] = G::TH;
@@ -525,6 +527,14 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     uint32_t at = scan256(sum, &survivors, sh.tmp);
     if (tid == 0)
       sh.scalars[2] = survivors ? atomicAdd(fd.counter, survivors) : 0u;
+    // each entry's exclusive survivor rank, for the row offsets below
+#pragma unroll
+    for (int i = 0, r = (int)at; i < LPER; i++) {
+      const uint32_t e = tid * LPER + i;
+      if (e < count)
+        sh.rank[e] = (uint32_t)r;
+      r += (int)flags[i];
+    }
     __syncthreads();
     const uint32_t base = sh.scalars[2];
 #pragma unroll
@@ -549,15 +559,14 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
         fd.staging[dst] = px;
       }
     }
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t run = base;
-      for (int r = ra; r < rb; r++) {
-        sh.rowbase[r] = run;
-        run += sh.rowok[r];
-      }
-      sh.scalars[1] += survivors;
+    // a row's survivors start at the survivor rank of its first candidate
+    for (int r = ra + (int)tid; r < rb; r += 256) {
+      const uint32_t fi = sh.word_off[r * G::WORDS] - first;
+      if (fi < count)
+        sh.rowbase[r] = base + sh.rank[fi];
     }
+    if (tid == 0)
+      sh.scalars[1] += survivors;
     __syncthreads();
   }
   for (int r = tid; r < out_rows; r += 256) {
