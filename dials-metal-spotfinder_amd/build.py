@@ -10,6 +10,10 @@ Outputs (all git-ignored, all travel with the tree):
     oracle/liboracle.so                         the CPU oracle (checker only)
     oracle/_ref/libdext_ref.so                  the reference's own CPU path,
                                                 when its sources are present
+    oracle/_ref/test_dext_gpu, bench_dext_gpu   the reference's own device
+                                                test and benchmark linked
+                                                with backend/dext_hip_backend.cc
+                                                over the library, likewise
 
 Every HIP translation unit is compiled -ffp-contract=off (no fused
 multiply-add may change a threshold test) and without packed FP32.
@@ -108,12 +112,22 @@ def build_checkers() -> None:
               f"REFERENCE={REFERENCE}"])
 
 
+def build_reference_dropin() -> None:
+    """The reference's tests/test_dext_gpu.cc and tests/bench_dext_gpu.cc,
+    unchanged, with this library as their gpu:: backend (needs the library
+    and the reference sources)."""
+    if os.path.isdir(os.path.join(REFERENCE, "src")):
+        _run(["make", "-B", "-s", "-C", os.path.join(ROOT, "oracle"),
+              "dropin", f"REFERENCE={REFERENCE}"])
+
+
 def build_all() -> None:
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     build_checkers()
     build_library()
     build_dropin()
+    build_reference_dropin()
 
 
 if __name__ == "__main__":

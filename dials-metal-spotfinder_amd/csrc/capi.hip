@@ -800,3 +800,49 @@ int plan_lists(sf_plan *p, std::vector<const sf_signal_pixel *> &lists,
   return SF_OK;
 }
 } // namespace sf
+
+namespace sf {
+// The fused forms' layout of one frame (form 0: the probe kernel at 16 bits,
+// the sweep at 32): strips, bands and the list capacity. Pointers, item
+// offsets and the vector flag are the caller's to fill.
+void fused_layout(int pixel_bytes, uint32_t height, uint32_t width,
+                  FrameDesc &d) {
+  d.height = height;
+  d.width = width;
+  if (pixel_bytes == 2) {
+    const uint32_t tw = (uint32_t)probe_tile_width();
+    d.n_strips = (width + tw - 1) / tw;
+    d.band_rows = (uint32_t)probe_tile_height();
+  } else {
+    const uint32_t ow = strip_width<uint32_t>();
+    d.n_strips = (width + ow - 1) / ow;
+    d.band_rows = 236;
+  }
+  d.n_bands = (height + d.band_rows - 1) / d.band_rows;
+  d.capacity = (uint32_t)((size_t)height * width * (size_t)pixel_bytes / 16);
+}
+
+// Cells of the per-row survivor table a frame needs, whatever the form.
+size_t cell_entries(uint32_t height, uint32_t width) {
+  return (size_t)height * ((width + 63) / 64);
+}
+
+// Zero the counters, threshold every frame of `d` (device descriptors), and
+// lay each list out in index order: one batch on `stream`.
+hipError_t launch_batch(int pixel_bytes, bool any_vec, bool any_scalar,
+                        const FrameDesc *d, int n, uint32_t items,
+                        uint32_t max_bands, hipStream_t stream) {
+  hipError_t e = launch_zero(d, n, stream);
+  if (e != hipSuccess)
+    return e;
+  e = pixel_bytes == 2
+          ? launch_probe(d, n, items, 0, stream)
+          : launch_fused(pixel_bytes, any_vec, any_scalar, d, n, items, 0,
+                         stream);
+  if (e != hipSuccess)
+    return e;
+  return launch_order(d, n, max_bands, stream);
+}
+
+void ensure_device_tables() { ensure_tables(); }
+} // namespace sf

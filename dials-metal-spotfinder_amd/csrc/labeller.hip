@@ -865,6 +865,53 @@ int flush_pending(sf_labeller *l) {
 
 } // namespace
 
+namespace sf {
+// Label frames whose ordered lists are already on the device, any frame
+// numbers in ascending order (gaps close components as in
+// src/dials_spots.cc:68-69). `producer` is the stream the lists were written
+// on; the gather runs there, so the producer may reuse the lists once work
+// it enqueues afterwards runs. The serving loop's path into the labeller.
+int labeller_add_lists(sf_labeller *l, int n, const int64_t *numbers,
+                       const sf_signal_pixel *const *lists,
+                       const uint32_t *const *count_ptrs,
+                       const uint32_t *capacities, const uint32_t *counts,
+                       hipStream_t producer) {
+  if (!l || n <= 0 || !numbers || !lists || !count_ptrs || !capacities ||
+      !counts) {
+    g_label_error = "labeller_add_lists: bad argument";
+    return SF_BAD_ARGUMENT;
+  }
+  int status = flush_pending(l);
+  if (status != SF_OK)
+    return status;
+  for (int k = 0; k < n; k++) {
+    const int64_t before = k ? numbers[k - 1] : l->last_frame;
+    if ((k > 0 || l->started) && numbers[k] <= before) {
+      g_label_error = "frames must be added in ascending order";
+      return SF_ORDER_ERROR;
+    }
+  }
+  std::vector<sf::ChunkDesc> frames(n);
+  std::vector<uint32_t> host_counts(counts, counts + n);
+  for (int k = 0; k < n; k++) {
+    frames[k].list = lists[k];
+    frames[k].count = count_ptrs[k];
+    frames[k].capacity = capacities[k];
+    frames[k].number = (int32_t)numbers[k];
+    const int64_t before = k ? numbers[k - 1] : l->last_frame;
+    frames[k].linked = (k > 0 || l->started) && numbers[k] == before + 1;
+  }
+  status = run_chunk(l, frames, host_counts, false, producer);
+  if (status != SF_OK)
+    return status;
+  l->started = true;
+  l->last_frame = numbers[n - 1];
+  return SF_OK;
+}
+
+hipStream_t labeller_stream(sf_labeller *l) { return l ? l->stream : nullptr; }
+} // namespace sf
+
 extern "C" {
 
 sf_labeller *sf_hip_labeller_create(size_t height, size_t width,

@@ -62,6 +62,12 @@ class Options(ctypes.Structure):
                 ("two_d", ctypes.c_int32)]
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "batches", "batched_frames", "admission_waits",
+        "max_batches_in_flight", "bytes_in_use", "peak_bytes", "budget")]
+
+
 def _declare(lib: ctypes.CDLL) -> None:
     def fn(name, restype, *argtypes):
         f = getattr(lib, name)
@@ -123,6 +129,25 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_labeller_pixel_count", _sz, _p)
     fn("sf_hip_labeller_results", _i, _p, _p, _p, _p)
     fn("sf_hip_labeller_last_error", ctypes.c_char_p)
+    fn("sf_hip_server_create", _p, _sz, _i, _i)
+    fn("sf_hip_server_destroy", None, _p)
+    fn("sf_hip_server_open", _i, _p, _i, _sz, _sz, ctypes.POINTER(Options))
+    fn("sf_hip_server_submit", _i, _p, _i, ctypes.c_int64, _p)
+    fn("sf_hip_server_submit_at", _i, _p, _i, ctypes.c_int64, _p,
+       ctypes.c_double)
+    fn("sf_hip_server_clock", ctypes.c_double)
+    fn("sf_hip_server_drain", _i, _p)
+    fn("sf_hip_server_close", _i, _p, _i)
+    fn("sf_hip_server_labeller", _p, _p, _i)
+    fn("sf_hip_server_frames", _i, _p, _i, _p, _p, _p, _p, _sz,
+       ctypes.POINTER(_sz))
+    fn("sf_hip_server_frame_list", _i, _p, _i, _sz, _p, _sz,
+       ctypes.POINTER(_sz))
+    fn("sf_hip_server_stats", _i, _p, ctypes.POINTER(ServerStats))
+    fn("sf_hip_server_play", _i, _p, _i, ctypes.POINTER(_i),
+       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+       ctypes.POINTER(_p), ctypes.POINTER(_i), ctypes.c_int64)
+    fn("sf_hip_server_last_error", ctypes.c_char_p)
 
 
 _lib: ctypes.CDLL | None = None
@@ -429,21 +454,153 @@ class Labeller:
     def results(self):
         """(spots, pixels, counts dict), as the reference's spots(),
         pixels() and counts()."""
-        n_spots, n_pixels = _sz(0), _sz(0)
-        self._status(lib().sf_hip_labeller_sizes(
-            self.handle, ctypes.byref(n_spots), ctypes.byref(n_pixels)),
-            "sizes")
-        spots = np.zeros(n_spots.value, dtype=SPOT)
-        pixels = np.zeros(n_pixels.value, dtype=PIXEL)
-        counts = (ctypes.c_uint64 * 6)()
-        self._status(lib().sf_hip_labeller_results(
-            self.handle, spots.ctypes.data if len(spots) else None,
-            pixels.ctypes.data if len(pixels) else None, counts), "results")
-        return spots, pixels, dict(zip(COUNTS, [int(c) for c in counts]))
+        return _labeller_results(self.handle, self._status)
 
     def close(self) -> None:
         if self.handle:
             lib().sf_hip_labeller_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _labeller_results(handle, status):
+    n_spots, n_pixels = _sz(0), _sz(0)
+    status(lib().sf_hip_labeller_sizes(handle, ctypes.byref(n_spots),
+                                       ctypes.byref(n_pixels)), "sizes")
+    spots = np.zeros(n_spots.value, dtype=SPOT)
+    pixels = np.zeros(n_pixels.value, dtype=PIXEL)
+    counts = (ctypes.c_uint64 * 6)()
+    status(lib().sf_hip_labeller_results(
+        handle, spots.ctypes.data if len(spots) else None,
+        pixels.ctypes.data if len(pixels) else None, counts), "results")
+    return spots, pixels, dict(zip(COUNTS, [int(c) for c in counts]))
+
+
+# --- the serving loop ---------------------------------------------------------
+
+class Server:
+    """Frames of several concurrent series, batched as they arrive, grouped
+    per series on the device, admitted by an HBM budget (see serve.hip)."""
+
+    def __init__(self, hbm_budget: int, max_batch: int = 16,
+                 keep_lists: bool = False):
+        self.handle = lib().sf_hip_server_create(int(hbm_budget),
+                                                 int(max_batch),
+                                                 1 if keep_lists else 0)
+        if not self.handle:
+            raise ValueError(self._error())
+
+    @staticmethod
+    def _error() -> str:
+        return lib().sf_hip_server_last_error().decode()
+
+    def _status(self, status: int, what: str) -> None:
+        if status == SF_OK:
+            return
+        if status == SF_DEVICE_ERROR:
+            raise DeviceError(f"{what}: {self._error()}")
+        raise ValueError(f"{what}: status {status}: {self._error()}")
+
+    def open(self, pixel_bytes: int, height: int, width: int,
+             min_spot_size: int = 3, max_spot_size: int = 1000,
+             max_separation: float = 2.0, two_d: bool = False) -> int:
+        options = Options(min_spot_size, max_spot_size, float(max_separation),
+                          1 if two_d else 0)
+        sid = lib().sf_hip_server_open(self.handle, pixel_bytes, height,
+                                       width, ctypes.byref(options))
+        if sid < 0:
+            self._status(sid, "open")
+        return sid
+
+    def submit(self, series: int, frame: int, device_ptr: int,
+               arrival: float | None = None) -> None:
+        if arrival is None:
+            status = lib().sf_hip_server_submit(self.handle, series,
+                                                int(frame), int(device_ptr))
+        else:
+            status = lib().sf_hip_server_submit_at(
+                self.handle, series, int(frame), int(device_ptr),
+                float(arrival))
+        self._status(status, "submit")
+
+    @staticmethod
+    def clock() -> float:
+        return float(lib().sf_hip_server_clock())
+
+    def drain(self) -> None:
+        self._status(lib().sf_hip_server_drain(self.handle), "drain")
+
+    def close_series(self, series: int) -> None:
+        self._status(lib().sf_hip_server_close(self.handle, series), "close")
+
+    def results(self, series: int):
+        """The series' labeller tables (spots, pixels, counts dict)."""
+        handle = lib().sf_hip_server_labeller(self.handle, series)
+        if not handle:
+            raise ValueError("no such series")
+
+        def status(code, what):
+            if code != SF_OK:
+                raise ValueError(
+                    f"{what}: {lib().sf_hip_labeller_last_error().decode()}")
+        return _labeller_results(handle, status)
+
+    def frames(self, series: int) -> dict:
+        """Per completed frame: number, count, mask_ms and table_ms."""
+        n = _sz(0)
+        self._status(lib().sf_hip_server_frames(
+            self.handle, series, None, None, None, None, 0, ctypes.byref(n)),
+            "frames")
+        k = n.value
+        numbers = np.zeros(k, dtype=np.int64)
+        counts = np.zeros(k, dtype=np.uint32)
+        mask = np.zeros(k, dtype=np.float64)
+        table = np.zeros(k, dtype=np.float64)
+        if k:
+            self._status(lib().sf_hip_server_frames(
+                self.handle, series, numbers.ctypes.data, counts.ctypes.data,
+                mask.ctypes.data, table.ctypes.data, k, ctypes.byref(n)),
+                "frames")
+        return {"frame": numbers, "count": counts, "mask_ms": mask,
+                "table_ms": table}
+
+    def frame_list(self, series: int, k: int) -> np.ndarray:
+        n = _sz(0)
+        self._status(lib().sf_hip_server_frame_list(
+            self.handle, series, k, None, 0, ctypes.byref(n)), "frame_list")
+        out = np.zeros(n.value, dtype=SIGNAL)
+        if n.value:
+            self._status(lib().sf_hip_server_frame_list(
+                self.handle, series, k, out.ctypes.data, n.value,
+                ctypes.byref(n)), "frame_list")
+        return out
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._status(lib().sf_hip_server_stats(self.handle, ctypes.byref(st)),
+                     "stats")
+        return {name: int(getattr(st, name)) for name, _ in st._fields_}
+
+    def play(self, series, rates_hz, n_frames, rings, first_frame: int = 0):
+        """Arrival schedule: series[i] gets n_frames[i] frames at rates_hz[i],
+        cycling through the device pointers rings[i]; then drains."""
+        m = len(series)
+        flat = [int(p) for ring in rings for p in ring]
+        self._status(lib().sf_hip_server_play(
+            self.handle, m, (_i * m)(*series),
+            (ctypes.c_double * m)(*[float(r) for r in rates_hz]),
+            (ctypes.c_int64 * m)(*[int(n) for n in n_frames]),
+            (_p * len(flat))(*flat), (_i * m)(*[len(r) for r in rings]),
+            int(first_frame)), "play")
+
+    def close(self) -> None:
+        if self.handle:
+            lib().sf_hip_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -204,6 +204,70 @@ int sf_hip_labeller_results(sf_labeller *labeller, sf_spot *spots,
                             sf_pixel *pixels, sf_counts *counts);
 const char *sf_hip_labeller_last_error(void);
 
+/* ---- the serving loop ----------------------------------------------------
+ * No counterpart in the reference, whose driver runs one series at a time
+ * (src/find_spots.cc:560-640): this takes device-resident frames of several
+ * concurrent series as they arrive, batches whatever is admitted into shared
+ * launches (continuous batching over a few streams), keeps each series'
+ * union-find frontier in its own device labeller, and admits frames by an
+ * HBM budget for their per-frame buffers (slots). A series' lists and
+ * tables are the same alone and in any mix. Calls are thread-safe. */
+typedef struct sf_server sf_server;
+typedef struct sf_server_stats {
+  uint64_t batches;               /* launches */
+  uint64_t batched_frames;        /* frames over all launches */
+  uint64_t admission_waits;       /* times a frame found no slot in budget */
+  uint64_t max_batches_in_flight;
+  uint64_t bytes_in_use;          /* slot bytes allocated */
+  uint64_t peak_bytes;
+  uint64_t budget;
+} sf_server_stats;
+/* hbm_budget: bytes of slots the server may hold; max_batch_frames: frames
+ * per launch; keep_lists: keep every frame's ordered list on the host (for
+ * checking; costs a copy per frame). */
+sf_server *sf_hip_server_create(size_t hbm_budget, int max_batch_frames,
+                                int keep_lists);
+/* Drains, then frees everything (labellers included). */
+void sf_hip_server_destroy(sf_server *server);
+/* Opens a series; returns its id (>= 0) or a negative status. The labeller
+ * options are as sf_hip_labeller_create's. */
+int sf_hip_server_open(sf_server *server, int pixel_bytes, size_t height,
+                       size_t width, const sf_options *options);
+/* Hands over one frame (device memory, valid until the frame completes);
+ * frame numbers ascend per series. Returns at once. _at stamps the arrival
+ * at a time of sf_hip_server_clock() (seconds) instead of now. */
+int sf_hip_server_submit(sf_server *server, int series, int64_t frame,
+                         const void *device_pixels);
+int sf_hip_server_submit_at(sf_server *server, int series, int64_t frame,
+                            const void *device_pixels, double arrival_s);
+double sf_hip_server_clock(void);
+/* Waits until every submitted frame is thresholded and labelled. */
+int sf_hip_server_drain(sf_server *server);
+/* Drains and finishes the series' labeller; its results are then final. */
+int sf_hip_server_close(sf_server *server, int series);
+/* The series' labeller, for sf_hip_labeller_results and friends. */
+sf_labeller *sf_hip_server_labeller(sf_server *server, int series);
+/* Per completed frame of a series, in frame order: number, signal count,
+ * arrival-to-ordered-list latency and arrival-to-labelled latency (ms). Any
+ * output may be NULL; *n is the number of frames completed. */
+int sf_hip_server_frames(sf_server *server, int series, int64_t *frames,
+                         uint32_t *counts, double *mask_ms, double *table_ms,
+                         size_t capacity, size_t *n);
+/* The k-th completed frame's list (keep_lists servers only). */
+int sf_hip_server_frame_list(sf_server *server, int series, size_t k,
+                             sf_signal_pixel *out, size_t capacity,
+                             size_t *n);
+int sf_hip_server_stats(sf_server *server, sf_server_stats *stats);
+/* Plays an arrival schedule and drains: frame f of series i (f < n_frames[i])
+ * arrives at t0 + f / rate_hz[i] and is numbered first_frame + f; its pixels
+ * are rings[sum(ring_lengths[:i]) + f % ring_lengths[i]]. Latencies are
+ * measured from the scheduled arrival. */
+int sf_hip_server_play(sf_server *server, int n_series, const int *series,
+                       const double *rate_hz, const int64_t *n_frames,
+                       const void *const *rings, const int *ring_lengths,
+                       int64_t first_frame);
+const char *sf_hip_server_last_error(void);
+
 /* ---- synthetic frames on the device (benchmark and test input) ----------
  * Fills `frames` device frames of height x width (pixel_bytes 2 or 4) at
  * `device` with a Poisson background of mean `background`, masked module

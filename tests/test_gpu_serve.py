@@ -1,0 +1,140 @@
+"""The serving loop: several series at once, continuous batching, admission
+by HBM budget, per-series device frontier.
+
+The bar: every frame's ordered list and every series' Spot/Pixel/Counts
+tables are byte-identical whether the series runs alone or in a mix with
+others under a budget tight enough to make frames wait for slots, and both
+equal the oracle's (threshold and Labeller) on the same frames. No
+tolerance: the outputs are integers and reference-order FP64.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import frames as fr
+from oracle import cpu
+
+pytestmark = pytest.mark.gpu
+
+# (pixel_bytes, height, width, frames, seed, background, labeller options)
+SERIES = [
+    (2, 256, 384, 24, 31, 0.3, {}),
+    (2, 300, 517, 16, 32, 0.3, {}),
+    (4, 200, 260, 12, 33, 50.0, {}),
+    (2, 150, 260, 20, 34, 0.3, {"two_d": True}),
+]
+
+
+def slot_bytes(pixel_bytes, h, w):
+    """An upper bound of one frame's device buffers in the server."""
+    cap = h * w * pixel_bytes // 16
+    return 2 * 16 * cap + h * math.ceil(w / 64) * 5 + (h // 16 + 2) * 4
+
+
+def make_series(sf):
+    out = []
+    for pb, h, w, n, seed, bg, opts in SERIES:
+        buf = sf.DeviceBuffer(pb * h * w * n)
+        spots = fr.diffraction_spots(seed, h, w, n, 100.0 * h * w / 98304,
+                                     peak=20000.0 if pb == 2 else 3e6)
+        sf.synth_frames(buf, pb, h, w, n, seed, bg, True, 0.0005, spots)
+        host = np.zeros((n, h, w), dtype=np.uint16 if pb == 2 else np.uint32)
+        buf.download(host)
+        ptrs = [buf.ptr + pb * h * w * k for k in range(n)]
+        out.append((buf, host, ptrs, opts))
+    return out
+
+
+def run(sf, data, which, budget, max_batch, rates=None):
+    server = sf.Server(budget, max_batch=max_batch, keep_lists=True)
+    ids = []
+    for i in which:
+        pb, h, w = SERIES[i][:3]
+        ids.append(server.open(pb, h, w, **data[i][3]))
+    if rates is None:
+        for sid, i in zip(ids, which):
+            for k, p in enumerate(data[i][2]):
+                server.submit(sid, 1000 + k, p)
+    else:
+        server.play(ids, rates, [len(data[i][2]) for i in which],
+                    [data[i][2] for i in which], first_frame=1000)
+    out = {}
+    for sid, i in zip(ids, which):
+        server.close_series(sid)
+        frames = server.frames(sid)
+        lists = [server.frame_list(sid, k) for k in range(len(frames["frame"]))]
+        out[i] = (frames, lists, server.results(sid))
+    stats = server.stats()
+    server.close()
+    return out, stats
+
+
+def same_tables(a, b):
+    return (a[0].tobytes() == b[0].tobytes() and
+            a[1].tobytes() == b[1].tobytes() and a[2] == b[2])
+
+
+def test_alone_equals_mix_equals_oracle(sf):
+    data = make_series(sf)
+    big = 1 << 30
+    tight = sum(slot_bytes(*s[:3]) for s in SERIES) + 4096
+    mix, stats = run(sf, data, [0, 1, 2, 3], tight, max_batch=5,
+                     rates=[3000.0, 2000.0, 1500.0, 2500.0])
+    assert stats["batched_frames"] == sum(s[3] for s in SERIES)
+    assert stats["peak_bytes"] <= tight
+    # the budget held one slot per series: frames had to wait for slots
+    assert stats["admission_waits"] > 0, stats
+    for i, (pb, h, w, n, seed, bg, opts) in enumerate(SERIES):
+        alone, _ = run(sf, data, [i], big, max_batch=8)
+        a_frames, a_lists, a_tables = alone[i]
+        m_frames, m_lists, m_tables = mix[i]
+        assert list(a_frames["frame"]) == list(range(1000, 1000 + n))
+        assert list(m_frames["frame"]) == list(range(1000, 1000 + n))
+        assert (a_frames["count"] == m_frames["count"]).all(), i
+        # the oracle on the same frames
+        host = data[i][1]
+        oracle = cpu.Labeller(h, w, **opts)
+        for k in range(n):
+            status, expected, found = cpu.dext(host[k])
+            assert status == 0
+            assert 0 < found <= h * w * pb // 16, (i, k, found)
+            alone_ok = a_lists[k].tobytes() == expected.tobytes()
+            mix_ok = m_lists[k].tobytes() == expected.tobytes()
+            assert alone_ok and mix_ok, (i, k, len(a_lists[k]),
+                                         len(m_lists[k]), found)
+            assert oracle.add(1000 + k, expected) == 0
+        oracle.finish()
+        expected_tables = oracle.results()
+        alone_mix = same_tables(a_tables, m_tables)
+        mix_oracle = same_tables(m_tables, expected_tables)
+        assert alone_mix, (i, a_tables[2], m_tables[2])
+        assert mix_oracle, (i, m_tables[2], expected_tables[2])
+        assert expected_tables[2]["accepted"] > 0, i
+        # latencies are recorded for every frame and ordered sensibly
+        assert (m_frames["mask_ms"] > 0).all()
+        assert (m_frames["table_ms"] >= m_frames["mask_ms"]).all()
+
+
+def test_submit_rules(sf):
+    buf = sf.DeviceBuffer(2 * 64 * 64 * 2)
+    server = sf.Server(1 << 24, max_batch=4)
+    sid = server.open(2, 64, 64)
+    server.submit(sid, 5, buf.ptr)
+    with pytest.raises(ValueError):
+        server.submit(sid, 5, buf.ptr)  # not ascending
+    with pytest.raises(ValueError):
+        server.submit(sid + 1, 0, buf.ptr)  # no such series
+    server.drain()
+    with pytest.raises(ValueError):
+        server.open(2, 4096, 4096)  # one frame exceeds the budget
+    with pytest.raises(ValueError):
+        server.open(3, 64, 64)  # not a pixel type
+    server.close_series(sid)
+    assert len(server.frames(sid)["frame"]) == 1
+    server.close()
+    with pytest.raises(ValueError):
+        sf.Server(0)
