@@ -814,9 +814,11 @@ void fused_layout(int pixel_bytes, uint32_t height, uint32_t width,
     d.n_strips = (width + tw - 1) / tw;
     d.band_rows = (uint32_t)probe_tile_height();
   } else {
+    // Short bands: a frame or two per launch still fills the chip (at 16
+    // frames 108-row bands are 5% faster, at one frame 44-row bands 35%).
     const uint32_t ow = strip_width<uint32_t>();
     d.n_strips = (width + ow - 1) / ow;
-    d.band_rows = 236;
+    d.band_rows = 44;
   }
   d.n_bands = (height + d.band_rows - 1) / d.band_rows;
   d.capacity = (uint32_t)((size_t)height * width * (size_t)pixel_bytes / 16);

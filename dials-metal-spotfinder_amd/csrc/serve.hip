@@ -15,15 +15,16 @@
 //   batch     the scheduler thread gathers every admitted frame of one pixel
 //             type, whatever its series or shape, into one launch (zero
 //             counters, threshold, ordering) on one of a few batch streams,
-//             so batches overlap on the device and frames arriving while
-//             others run join the next launch;
-//   complete  the completion thread takes batches in launch order: frame to
-//             mask latency is stamped when the batch's event completes, then
-//             each series' frames of the batch go to that series' device
-//             labeller (its union-find frontier stays on the device), the
-//             slots are released once the labeller has gathered the lists,
-//             and chunk to table latency is stamped when the labeller's
-//             chunk is done.
+//             taking the pixel types in turn, oldest waiting frame first, so
+//             batches overlap on the device and frames arriving while others
+//             run join the next launch;
+//   watch     a watcher thread waits on each batch's event in launch order
+//             and stamps frame-to-mask latency the moment it completes;
+//   label     a dispatcher hands each series' frames of the batch to that
+//             series' own labelling thread, which adds them to the series'
+//             device labeller (its union-find frontier stays on the device)
+//             and stamps chunk-to-table latency when the labeller is done;
+//             the last of them releases the batch and its slots.
 //
 // Every frame is computed by the same kernels as a plan, and each series has
 // its own labeller, so a series' results are identical alone and in a mix.
@@ -34,6 +35,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <iterator>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -60,7 +62,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 thread_local std::string g_serve_error;
 
-constexpr int kBatchStreams = 4;
+constexpr int kBatchStreams = 8;
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -81,6 +83,15 @@ struct Request {
   Clock::time_point arrival;
 };
 
+struct Batch;
+
+// One series' frames of one batch, for that series' labelling worker.
+struct Job {
+  Batch *batch = nullptr;
+  int series = 0;
+  std::vector<int> frames; // indices into the batch, in frame order
+};
+
 struct Series {
   int id = 0;
   int pixel_bytes = 2;
@@ -96,6 +107,11 @@ struct Series {
   std::vector<uint32_t> counts;
   std::vector<double> mask_ms, table_ms;
   std::vector<std::vector<sf_signal_pixel>> lists;
+  // the labelling worker
+  std::thread worker;
+  std::deque<Job> jobs;
+  std::condition_variable cv;
+  bool stop = false;
 };
 
 struct Batch {
@@ -107,6 +123,10 @@ struct Batch {
   uint32_t *d_counters = nullptr, *d_found = nullptr, *h_found = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t masked = nullptr, released = nullptr;
+  Clock::time_point masked_at; // when the watcher saw `masked` complete
+  hipError_t mask_error = hipSuccess;
+  int jobs_left = 0;            // labelling jobs still reading the lists
+  std::vector<std::vector<sf_signal_pixel>> lists; // keep_lists copies
 };
 
 } // namespace
@@ -116,14 +136,15 @@ struct sf_server {
   int max_batch = 16;
   bool keep_lists = false;
   std::mutex m;
-  std::condition_variable cv_sched, cv_done, cv_drain;
+  std::condition_variable cv_sched, cv_done, cv_label, cv_drain;
   std::deque<Request> pending;
   bool dirty = false;
   std::vector<Series *> series;
   std::vector<Batch *> batches, free_batches;
-  std::deque<Batch *> inflight;
-  std::thread scheduler, completer;
-  bool stop = false, scheduler_done = false;
+  std::deque<Batch *> inflight; // launched, list not yet ready
+  std::deque<Batch *> to_label; // lists ready, waiting for the labellers
+  std::thread scheduler, watcher, dispatcher;
+  bool stop = false, scheduler_done = false, watcher_done = false;
   uint64_t submitted = 0, completed = 0;
   int status = SF_OK;
   std::string error;
@@ -377,11 +398,17 @@ void scheduler_loop(sf_server *s) {
       s->cv_drain.notify_all();
       continue;
     }
-    for (int pixel_bytes : {2, 4}) {
-      for (;;) {
+    // One batch per pixel type in turn, the type of the oldest waiting
+    // frame first, so that neither type can take every batch stream.
+    const int first =
+        s->pending.empty() ? 2 : s->series[s->pending.front().series]->pixel_bytes;
+    for (bool progress = true; progress && !s->pending.empty();) {
+      progress = false;
+      for (int pixel_bytes : {first, 6 - first}) {
         Batch *b = form_batch(s, pixel_bytes);
         if (!b)
-          break;
+          continue;
+        progress = true;
         lk.unlock();
         const hipError_t e = launch(b);
         lk.lock();
@@ -390,7 +417,8 @@ void scheduler_loop(sf_server *s) {
         s->n_batches++;
         s->batched_frames += b->requests.size();
         s->inflight.push_back(b);
-        s->max_inflight = std::max(s->max_inflight, (int)s->inflight.size());
+        s->max_inflight = std::max(
+            s->max_inflight, (int)(s->batches.size() - s->free_batches.size()));
         s->cv_done.notify_one();
       }
     }
@@ -399,116 +427,9 @@ void scheduler_loop(sf_server *s) {
   s->cv_done.notify_all();
 }
 
-// Everything after a batch's event: latencies, lists, labelling, release.
-int complete(sf_server *s, Batch *b) {
-  {
-    std::lock_guard<std::mutex> lk(s->m);
-    if (s->status != SF_OK) {
-      g_serve_error = s->error;
-      (void)hipStreamSynchronize(b->stream);
-      return s->status;
-    }
-  }
-  hipError_t e = hipEventSynchronize(b->masked);
-  const Clock::time_point masked_at = Clock::now();
-  if (e != hipSuccess) {
-    g_serve_error = hip_message("a batch failed", e);
-    return SF_DEVICE_ERROR;
-  }
-  const int n = (int)b->requests.size();
-  std::vector<std::vector<sf_signal_pixel>> lists(s->keep_lists ? n : 0);
-  if (s->keep_lists) {
-    for (int f = 0; f < n; f++) {
-      const uint32_t c = std::min(b->h_found[f], b->h_desc[f].capacity);
-      lists[f].resize(c);
-      if (c)
-        e = hipMemcpy(lists[f].data(), b->h_desc[f].out,
-                      c * sizeof(sf_signal_pixel), hipMemcpyDeviceToHost);
-      if (e != hipSuccess) {
-        g_serve_error = hip_message("reading a list", e);
-        return SF_DEVICE_ERROR;
-      }
-    }
-  }
-  // Group the batch by series, keeping each series' frames in order.
-  std::vector<int> order(n);
-  for (int f = 0; f < n; f++)
-    order[f] = f;
-  std::stable_sort(order.begin(), order.end(), [b](int x, int y) {
-    return b->requests[x].series < b->requests[y].series;
-  });
-  std::vector<Series *> touched;
-  std::vector<std::pair<int, int>> runs; // [begin, end) in `order`
-  for (int i = 0; i < n;) {
-    int j = i;
-    while (j < n &&
-           b->requests[order[j]].series == b->requests[order[i]].series)
-      j++;
-    runs.emplace_back(i, j);
-    i = j;
-  }
-  for (const auto &run : runs) {
-    const int m = run.second - run.first;
-    std::vector<int64_t> numbers(m);
-    std::vector<const sf_signal_pixel *> ptrs(m);
-    std::vector<const uint32_t *> count_ptrs(m);
-    std::vector<uint32_t> caps(m), counts(m);
-    for (int k = 0; k < m; k++) {
-      const int f = order[run.first + k];
-      numbers[k] = b->requests[f].frame;
-      ptrs[k] = b->h_desc[f].out;
-      count_ptrs[k] = b->d_found + f;
-      caps[k] = b->h_desc[f].capacity;
-      counts[k] = b->h_found[f];
-    }
-    Series *se = s->series[b->requests[order[run.first]].series];
-    const int status =
-        sf::labeller_add_lists(se->labeller, m, numbers.data(), ptrs.data(),
-                               count_ptrs.data(), caps.data(), counts.data(),
-                               b->stream);
-    if (status != SF_OK) {
-      g_serve_error = std::string("labelling: ") +
-                      sf_hip_labeller_last_error();
-      return status;
-    }
-    touched.push_back(se);
-  }
-  // The labellers' gathers ran on the batch stream: after this event the
-  // slots and the batch may be reused.
-  e = hipEventRecord(b->released, b->stream);
-  if (e == hipSuccess)
-    e = hipEventSynchronize(b->released);
-  if (e != hipSuccess) {
-    g_serve_error = hip_message("releasing a batch", e);
-    return SF_DEVICE_ERROR;
-  }
-  std::vector<Clock::time_point> tabled(touched.size());
-  for (size_t t = 0; t < touched.size(); t++) {
-    e = hipStreamSynchronize(sf::labeller_stream(touched[t]->labeller));
-    tabled[t] = Clock::now();
-    if (e != hipSuccess) {
-      g_serve_error = hip_message("labelling", e);
-      return SF_DEVICE_ERROR;
-    }
-  }
-  std::lock_guard<std::mutex> lk(s->m);
-  for (size_t t = 0; t < runs.size(); t++) {
-    for (int i = runs[t].first; i < runs[t].second; i++) {
-      const int f = order[i];
-      const Request &r = b->requests[f];
-      Series *se = s->series[r.series];
-      se->frames.push_back(r.frame);
-      se->counts.push_back(b->h_found[f]);
-      se->mask_ms.push_back(ms_between(r.arrival, masked_at));
-      se->table_ms.push_back(ms_between(r.arrival, tabled[t]));
-      if (s->keep_lists)
-        se->lists.push_back(std::move(lists[f]));
-    }
-  }
-  return SF_OK;
-}
-
-void completion_loop(sf_server *s) {
+// Stamps each batch's frame-to-mask time as its event completes, in launch
+// order, on a thread of its own so that labelling never delays the stamp.
+void watch_loop(sf_server *s) {
   std::unique_lock<std::mutex> lk(s->m);
   for (;;) {
     s->cv_done.wait(lk, [s] {
@@ -517,21 +438,176 @@ void completion_loop(sf_server *s) {
     if (s->inflight.empty() && s->scheduler_done)
       break;
     Batch *b = s->inflight.front();
+    s->inflight.pop_front();
     lk.unlock();
-    const int status = complete(s, b);
+    b->mask_error = hipEventSynchronize(b->masked);
+    b->masked_at = Clock::now();
+    lk.lock();
+    s->to_label.push_back(b);
+    s->cv_label.notify_one();
+  }
+  s->watcher_done = true;
+  s->cv_label.notify_all();
+}
+
+// A batch's lists are ready: copy them out if asked, then hand each series'
+// frames of the batch to that series' labelling worker. The batch is
+// released by the last worker to finish with it.
+void release_batch(sf_server *s, Batch *b) {
+  // called with s->m held
+  for (Slot *slot : b->slots)
+    slot->series->free_slots.push_back(slot);
+  s->completed += b->requests.size();
+  b->requests.clear();
+  b->slots.clear();
+  s->free_batches.push_back(b);
+  s->dirty = true;
+  s->cv_sched.notify_one();
+  s->cv_drain.notify_all();
+}
+
+void dispatch(sf_server *s, Batch *b) {
+  const int n = (int)b->requests.size();
+  int status = SF_OK;
+  std::string what;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    if (s->status != SF_OK)
+      status = s->status;
+  }
+  if (status == SF_OK && b->mask_error != hipSuccess) {
+    status = SF_DEVICE_ERROR;
+    what = hip_message("a batch failed", b->mask_error);
+  }
+  b->lists.assign(s->keep_lists && status == SF_OK ? n : 0, {});
+  for (int f = 0; f < (int)b->lists.size() && status == SF_OK; f++) {
+    const uint32_t c = std::min(b->h_found[f], b->h_desc[f].capacity);
+    b->lists[f].resize(c);
+    const hipError_t e =
+        c ? hipMemcpy(b->lists[f].data(), b->h_desc[f].out,
+                      c * sizeof(sf_signal_pixel), hipMemcpyDeviceToHost)
+          : hipSuccess;
+    if (e != hipSuccess) {
+      status = SF_DEVICE_ERROR;
+      what = hip_message("reading a list", e);
+    }
+  }
+  std::lock_guard<std::mutex> lk(s->m);
+  if (status != SF_OK) {
+    if (!what.empty())
+      set_failure(s, status, what);
+    (void)hipStreamSynchronize(b->stream);
+    release_batch(s, b);
+    return;
+  }
+  // one job per series, its frames in batch order (which is frame order)
+  std::vector<Job> jobs;
+  for (int f = 0; f < n; f++) {
+    const int sid = b->requests[f].series;
+    auto it = std::find_if(jobs.begin(), jobs.end(),
+                           [sid](const Job &j) { return j.series == sid; });
+    if (it == jobs.end()) {
+      jobs.push_back(Job{b, sid, {}});
+      it = jobs.end() - 1;
+    }
+    it->frames.push_back(f);
+  }
+  b->jobs_left = (int)jobs.size();
+  for (Job &job : jobs) {
+    Series *se = s->series[job.series];
+    se->jobs.push_back(std::move(job));
+    se->cv.notify_one();
+  }
+}
+
+void dispatch_loop(sf_server *s) {
+  std::unique_lock<std::mutex> lk(s->m);
+  for (;;) {
+    s->cv_label.wait(lk, [s] {
+      return !s->to_label.empty() || s->watcher_done;
+    });
+    if (s->to_label.empty() && s->watcher_done)
+      break;
+    Batch *b = s->to_label.front();
+    s->to_label.pop_front();
+    lk.unlock();
+    dispatch(s, b);
+    lk.lock();
+  }
+}
+
+// Label a series' queued jobs -- its frames of one or more batches, oldest
+// first -- as one chunk on the series' own labeller. Every list is complete
+// (the watcher saw its batch's event) and stays untouched until the job's
+// batch is released after this returns, so the gather needs no producer
+// stream.
+int label_jobs(Series *se, const std::vector<Job> &jobs,
+               Clock::time_point *tabled) {
+  std::vector<int64_t> numbers;
+  std::vector<const sf_signal_pixel *> ptrs;
+  std::vector<const uint32_t *> count_ptrs;
+  std::vector<uint32_t> caps, counts;
+  for (const Job &job : jobs) {
+    const Batch *b = job.batch;
+    for (int f : job.frames) {
+      numbers.push_back(b->requests[f].frame);
+      ptrs.push_back(b->h_desc[f].out);
+      count_ptrs.push_back(b->d_found + f);
+      caps.push_back(b->h_desc[f].capacity);
+      counts.push_back(b->h_found[f]);
+    }
+  }
+  int status = sf::labeller_add_lists(
+      se->labeller, (int)numbers.size(), numbers.data(), ptrs.data(),
+      count_ptrs.data(), caps.data(), counts.data(), nullptr);
+  if (status != SF_OK) {
+    g_serve_error = std::string("labelling: ") + sf_hip_labeller_last_error();
+    return status;
+  }
+  const hipError_t e = hipStreamSynchronize(sf::labeller_stream(se->labeller));
+  *tabled = Clock::now();
+  if (e != hipSuccess) {
+    g_serve_error = hip_message("labelling", e);
+    return SF_DEVICE_ERROR;
+  }
+  return SF_OK;
+}
+
+void series_loop(sf_server *s, Series *se) {
+  std::unique_lock<std::mutex> lk(s->m);
+  for (;;) {
+    se->cv.wait(lk, [se] { return !se->jobs.empty() || se->stop; });
+    if (se->jobs.empty() && se->stop)
+      break;
+    // everything queued for this series so far becomes one chunk
+    std::vector<Job> jobs(std::make_move_iterator(se->jobs.begin()),
+                          std::make_move_iterator(se->jobs.end()));
+    se->jobs.clear();
+    const bool failed = s->status != SF_OK;
+    lk.unlock();
+    Clock::time_point tabled = Clock::now();
+    int status = SF_OK;
+    if (!failed)
+      status = label_jobs(se, jobs, &tabled);
     lk.lock();
     if (status != SF_OK)
       set_failure(s, status, g_serve_error);
-    s->inflight.pop_front();
-    for (Slot *slot : b->slots)
-      slot->series->free_slots.push_back(slot);
-    s->completed += b->requests.size();
-    b->requests.clear();
-    b->slots.clear();
-    s->free_batches.push_back(b);
-    s->dirty = true;
-    s->cv_sched.notify_one();
-    s->cv_drain.notify_all();
+    for (Job &job : jobs) {
+      Batch *b = job.batch;
+      if (!failed && status == SF_OK) {
+        for (int f : job.frames) {
+          const Request &r = b->requests[f];
+          se->frames.push_back(r.frame);
+          se->counts.push_back(b->h_found[f]);
+          se->mask_ms.push_back(ms_between(r.arrival, b->masked_at));
+          se->table_ms.push_back(ms_between(r.arrival, tabled));
+          if (s->keep_lists)
+            se->lists.push_back(std::move(b->lists[f]));
+        }
+      }
+      if (--b->jobs_left == 0)
+        release_batch(s, b);
+    }
   }
 }
 
@@ -571,7 +647,8 @@ sf_server *sf_hip_server_create(size_t hbm_budget, int max_batch_frames,
   s->max_batch = max_batch_frames;
   s->keep_lists = keep_lists != 0;
   s->scheduler = std::thread(scheduler_loop, s);
-  s->completer = std::thread(completion_loop, s);
+  s->watcher = std::thread(watch_loop, s);
+  s->dispatcher = std::thread(dispatch_loop, s);
   return s;
 }
 
@@ -585,7 +662,16 @@ void sf_hip_server_destroy(sf_server *s) {
     s->cv_sched.notify_all();
   }
   s->scheduler.join();
-  s->completer.join();
+  s->watcher.join();
+  s->dispatcher.join();
+  for (Series *se : s->series) {
+    {
+      std::lock_guard<std::mutex> lk(s->m);
+      se->stop = true;
+      se->cv.notify_all();
+    }
+    se->worker.join();
+  }
   for (Batch *b : s->batches)
     free_batch(b);
   for (Series *se : s->series) {
@@ -622,6 +708,7 @@ int sf_hip_server_open(sf_server *s, int pixel_bytes, size_t height,
   std::lock_guard<std::mutex> lk(s->m);
   se->id = (int)s->series.size();
   s->series.push_back(se);
+  se->worker = std::thread(series_loop, s, se);
   return se->id;
 }
 
@@ -683,7 +770,8 @@ int sf_hip_server_close(sf_server *s, int series) {
     return status;
   if (se->closed)
     return SF_OK;
-  se->closed = true;
+  se->closed = true; // no more submissions; its worker is idle
+  lk.unlock();
   const int done = sf_hip_labeller_finish(se->labeller);
   if (done != SF_OK)
     g_serve_error = sf_hip_labeller_last_error();
