@@ -347,6 +347,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
 
   // -- the unmasked pixels of positive cells, inside the E region ----------
   // E region: rows [5, TH+15), columns [7, TW+17) of the tile's LDS frame.
+  int any_need = 0;
   for (int k = tid; k < NW; k += 256) {
     const int lr = k / G::WORDS, j = k % G::WORDS;
     if (lr < 5 || lr >= G::TH + 15)
@@ -358,8 +359,18 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
                  : ((hi - lo == 32 ? 0xFFFFFFFFu : ((1u << (hi - lo)) - 1u))
                     << lo);
     sh.need[k] = sh.cellmask[(ci - ci0) * G::WORDS + j] & cols & ~sh.mbits[k];
+    any_need |= sh.need[k] != 0;
   }
-  __syncthreads();
+  // No unmasked pixel of a positive cell: E is 0 at every unmasked pixel, so
+  // nothing in the tile can be signal (about half the tiles of a diffraction
+  // frame). Record empty rows and stop.
+  if (!__syncthreads_or(any_need)) {
+    const int rows = min(G::TH, height - r0);
+    for (int r = tid; r < rows; r += 256)
+      fd.cell_count[(size_t)(r0 + r) * fd.n_strips + tile_x] = 0;
+    __syncthreads();
+    return;
+  }
 
   // -- refine, round one: mask0 at those pixels ------------------------------
   refine<G>(sh, sh.need, 5, G::TH + 15);
@@ -472,45 +483,65 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     const uint32_t count = last - first;
     list_bits<G>(sh, sh.work + 10 * G::WORDS, 0, G::TH, first, count);
     // the Poisson test over the clipped 11x11 window of pixels neither
-    // masked nor eroded-set (src/dext.cc:165-207)
-    for (uint32_t i = tid; i < count; i += 256) {
-      const int lr = 10 + (int)(sh.list[i] >> 16);
-      const int lc = (int)(sh.list[i] & 0xFFFFu);
-      const int a = lc - 5;
-      const int w = a >> 1;
-      const uint32_t drop = (a & 1) ? 0xFFFF0000u : 0xFFFFFFFFu;
-      const uint32_t keep = (a & 1) ? 0xFFFFFFFFu : 0x0000FFFFu;
+    // masked nor eroded-set (src/dext.cc:165-207); eight lanes a candidate,
+    // lane g summing rows g and g + 8 of the window, then added across the
+    // eight (integer sums, so the order is immaterial)
+    const uint32_t g = tid & 7u;
+    for (uint32_t at0 = 0; at0 < count; at0 += 32) {
+      const uint32_t i = at0 + (tid >> 3);
+      const bool active = i < count;
       uint32_t excluded = 0, s_all = 0, s_e = 0;
-      for (int r = lr - 5; r <= lr + 5; r++) {
-        const uint32_t *mrow = sh.mbits + r * G::WORDS;
-        const uint32_t *erow = sh.ebits + r * G::WORDS;
-        const uint32_t e = row_bits(erow, a, 11);
-        excluded += __builtin_popcount(row_bits(mrow, a, 11) | e);
-        const uint32_t *p = sh.px + r * G::PITCH + w;
-        s_all = udot2(p[0] & drop, 0x00010001u, s_all);
-        s_all = udot2(p[1], 0x00010001u, s_all);
-        s_all = udot2(p[2], 0x00010001u, s_all);
-        s_all = udot2(p[3], 0x00010001u, s_all);
-        s_all = udot2(p[4], 0x00010001u, s_all);
-        s_all = udot2(p[5] & keep, 0x00010001u, s_all);
-        uint32_t eb = e;
-        while (eb) {
-          const int b = __builtin_ctz(eb);
-          eb &= eb - 1;
-          const int c = a + b;
-          const uint32_t pair = sh.px[r * G::PITCH + (c >> 1)];
-          s_e += (c & 1) ? pair >> 16 : pair & 0xFFFFu;
+      int lr = 0, lc = 0;
+      if (active) {
+        lr = 10 + (int)(sh.list[i] >> 16);
+        lc = (int)(sh.list[i] & 0xFFFFu);
+        const int a = lc - 5;
+        const int w = a >> 1;
+        const uint32_t drop = (a & 1) ? 0xFFFF0000u : 0xFFFFFFFFu;
+        const uint32_t keep = (a & 1) ? 0xFFFFFFFFu : 0x0000FFFFu;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+          const int row = (int)g + 8 * k;
+          if (row < 11) {
+            const int r = lr - 5 + row;
+            const uint32_t e = row_bits(sh.ebits + r * G::WORDS, a, 11);
+            excluded +=
+                __builtin_popcount(row_bits(sh.mbits + r * G::WORDS, a, 11) | e);
+            const uint32_t *p = sh.px + r * G::PITCH + w;
+            s_all = udot2(p[0] & drop, 0x00010001u, s_all);
+            s_all = udot2(p[1], 0x00010001u, s_all);
+            s_all = udot2(p[2], 0x00010001u, s_all);
+            s_all = udot2(p[3], 0x00010001u, s_all);
+            s_all = udot2(p[4], 0x00010001u, s_all);
+            s_all = udot2(p[5] & keep, 0x00010001u, s_all);
+            uint32_t eb = e;
+            while (eb) {
+              const int b = __builtin_ctz(eb);
+              eb &= eb - 1;
+              const int c = a + b;
+              const uint32_t pair = sh.px[r * G::PITCH + (c >> 1)];
+              s_e += (c & 1) ? pair >> 16 : pair & 0xFFFFu;
+            }
+          }
         }
       }
-      const uint32_t n = 121u - excluded;
-      const uint32_t s2 = s_all - s_e;
-      const uint32_t pair = sh.px[lr * G::PITCH + (lc >> 1)];
-      const uint32_t p = (lc & 1) ? pair >> 16 : pair & 0xFFFFu;
-      const float mean = n >= 2 ? (float)s2 / (float)n : 0.0f;
-      const bool ok = (float)p >= mean + 3.0f * sqrtf(mean);
-      sh.rank[i] = ok ? 1u : 0u;
-      sh.bg[i] = mean;
-      sh.pop[i] = n;
+#pragma unroll
+      for (int d = 1; d < 8; d <<= 1) {
+        excluded += __shfl_xor(excluded, d, 64);
+        s_all += __shfl_xor(s_all, d, 64);
+        s_e += __shfl_xor(s_e, d, 64);
+      }
+      if (active && g == 0) {
+        const uint32_t n = 121u - excluded;
+        const uint32_t s2 = s_all - s_e;
+        const uint32_t pair = sh.px[lr * G::PITCH + (lc >> 1)];
+        const uint32_t p = (lc & 1) ? pair >> 16 : pair & 0xFFFFu;
+        const float mean = n >= 2 ? (float)s2 / (float)n : 0.0f;
+        const bool ok = (float)p >= mean + 3.0f * sqrtf(mean);
+        sh.rank[i] = ok ? 1u : 0u;
+        sh.bg[i] = mean;
+        sh.pop[i] = n;
+      }
     }
     __syncthreads();
     // survivors in list order

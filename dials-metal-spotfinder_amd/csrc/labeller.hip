@@ -242,142 +242,310 @@ struct SpotOut {
   uint32_t members;  // its pixel count (when kept)
 };
 
-// One thread per closed component, in emission order: the reference's emit()
-// (src/dials_spots.cc:221-360) operation for operation.
-__global__ void finalise(LabelArgs a, const uint32_t *starts,
-                         const uint32_t *n_starts, const uint32_t *n_closed,
-                         const uint32_t *order, const int32_t *node_frame,
-                         const uint32_t *node_index,
-                         const uint32_t *node_value, SpotOut *out,
-                         unsigned long long *counts) {
+enum Emitted { kAccepted = 0, kTooSmall, kTooLarge, kSeparated };
+
+// The part of the reference's emit() (src/dials_spots.cc:297-360) that
+// follows the sums: centroid, variance, the single-frame z rule and the
+// separation test. `delta_sq` is filled by `variance_sums`, which must add
+// the terms in member order.
+template <class VarianceSums>
+__device__ int finish_spot(const LabelArgs &a, sf_spot &spot, double sum,
+                           double sum_sq, const double (&weighted)[3],
+                           const double (&peak)[3], VarianceSums variance_sums) {
+  spot.intensity = sum;
+  spot.intensity_variance = sum;
+  if (sum > 0.0) {
+    for (int axis = 0; axis < 3; axis++)
+      spot.position[axis] = weighted[axis] / sum;
+    double delta_sq[3] = {0.0, 0.0, 0.0};
+    variance_sums(spot.position, delta_sq);
+    const double twelfth = 1.0 / 12.0;
+    if (sum * sum > sum_sq) {
+      for (int axis = 0; axis < 3; axis++) {
+        const double variance = delta_sq[axis] * sum / (sum * sum - sum_sq);
+        spot.variance[axis] = variance / sum + twelfth;
+      }
+    } else {
+      for (int axis = 0; axis < 3; axis++)
+        spot.variance[axis] = twelfth;
+    }
+    if (spot.bbox[5] == spot.bbox[4] + 1)
+      spot.position[2] = (double)spot.bbox[4] + 0.5;
+  } else {
+    spot.position[0] = (spot.bbox[1] + spot.bbox[0]) / 2.0;
+    spot.position[1] = (spot.bbox[3] + spot.bbox[2]) / 2.0;
+    spot.position[2] = (spot.bbox[5] + spot.bbox[4]) / 2.0;
+  }
+  if (a.max_separation > 0.0) {
+    const double dx = peak[0] - spot.position[0];
+    const double dy = peak[1] - spot.position[1];
+    const double dz = peak[2] - spot.position[2];
+    if (sqrt(dx * dx + dy * dy + dz * dz) > a.max_separation)
+      return kSeparated;
+  }
+  return kAccepted;
+}
+
+// One component by one thread: the reference's emit() (src/dials_spots.cc:
+// 221-360) operation for operation, every sum in member order.
+__device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
+                               const uint32_t *order,
+                               const int32_t *node_frame,
+                               const uint32_t *node_index,
+                               const uint32_t *node_value, sf_spot &spot) {
+  spot.n_signal = n;
+  double sum = 0.0, sum_sq = 0.0;
+  double weighted[3] = {0.0, 0.0, 0.0};
+  int32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
+  uint32_t peak_value = 0;
+  double peak[3] = {0.0, 0.0, 0.0};
+  for (uint32_t k = 0; k < n; k++) {
+    const uint32_t g = order[s0 + k];
+    const uint32_t idx = node_index[g];
+    const int32_t x = (int32_t)(idx % a.width);
+    const int32_t y = (int32_t)(idx / a.width);
+    const int32_t z = node_frame[g];
+    if (k == 0) {
+      x0 = x;
+      x1 = x + 1;
+      y0 = y;
+      y1 = y + 1;
+      z0 = z;
+      z1 = z + 1;
+    } else {
+      x0 = x < x0 ? x : x0;
+      x1 = x + 1 > x1 ? x + 1 : x1;
+      y0 = y < y0 ? y : y0;
+      y1 = y + 1 > y1 ? y + 1 : y1;
+      z0 = z < z0 ? z : z0;
+      z1 = z + 1 > z1 ? z + 1 : z1;
+    }
+    const double value = (double)node_value[g];
+    const double position[3] = {(double)x + 0.5, (double)y + 0.5,
+                                (double)z + 0.5};
+    sum += value;
+    sum_sq += value * value;
+    for (int axis = 0; axis < 3; axis++)
+      weighted[axis] += value * position[axis];
+    if (node_value[g] > peak_value) {
+      peak_value = node_value[g];
+      peak[0] = position[0];
+      peak[1] = position[1];
+      peak[2] = position[2];
+    }
+  }
+  spot.bbox[0] = x0;
+  spot.bbox[1] = x1;
+  spot.bbox[2] = y0;
+  spot.bbox[3] = y1;
+  spot.bbox[4] = z0;
+  spot.bbox[5] = z1;
+  return finish_spot(a, spot, sum, sum_sq, weighted, peak,
+                     [&](const double (&centre)[3], double (&delta_sq)[3]) {
+                       for (uint32_t k = 0; k < n; k++) {
+                         const uint32_t g = order[s0 + k];
+                         const uint32_t idx = node_index[g];
+                         const double value = (double)node_value[g];
+                         const double position[3] = {
+                             (double)(idx % a.width) + 0.5,
+                             (double)(idx / a.width) + 0.5,
+                             (double)node_frame[g] + 0.5};
+                         for (int axis = 0; axis < 3; axis++) {
+                           const double delta = position[axis] - centre[axis];
+                           delta_sq[axis] += value * delta * delta;
+                         }
+                       }
+                     });
+}
+
+__device__ __forceinline__ double wave_read(double v, uint32_t lane) {
+  const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits,
+                                                          (int)lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(
+      (int)(uint32_t)(bits >> 32), (int)lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1)
+    v += __shfl_xor(v, d, 64);
+  return v;
+}
+template <typename T> __device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T o = __shfl_xor(v, d, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+template <typename T> __device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T o = __shfl_xor(v, d, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// One wave per closed component, in emission order, with the result the
+// sequential emit() gives bit for bit:
+//  * bbox, peak (first strict maximum in member order) and the sums are
+//    taken across the wave. The reference sums in double in member order;
+//    when every term and partial sum is an integer (or half-integer) below
+//    2^53 -- always so for 16-bit frames -- those double sums are exact and
+//    equal the integer sums computed here in any order. Otherwise one lane
+//    runs the sequential emit();
+//  * the variance terms v (p - c)^2 are formed in parallel, then added in
+//    member order, as src/dials_spots.cc:304-316 adds them.
+__global__ __launch_bounds__(256) void finalise(
+    LabelArgs a, const uint32_t *starts, const uint32_t *n_starts,
+    const uint32_t *n_closed, const uint32_t *order,
+    const int32_t *node_frame, const uint32_t *node_index,
+    const uint32_t *node_value, SpotOut *out, unsigned long long *counts) {
   const uint32_t comps = *n_starts;
   const uint32_t end_all = *n_closed;
-  uint32_t small = 0, large = 0, separated = 0, accepted = 0;
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
-       c += gridDim.x * blockDim.x) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  uint32_t tally[4] = {0, 0, 0, 0};
+  for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       c < comps; c += waves) {
     const uint32_t s0 = starts[c];
     const uint32_t s1 = c + 1 < comps ? starts[c + 1] : end_all;
     const uint32_t n = s1 - s0;
     SpotOut r;
     memset(&r, 0, sizeof(r));
+    int kind = kAccepted;
     if (n < a.min_size) {
-      small++;
-      out[c] = r;
-      continue;
-    }
-    if (n > a.max_size) {
-      large++;
-      out[c] = r;
-      continue;
-    }
-    sf_spot &spot = r.spot;
-    spot.n_signal = n;
-    double sum = 0.0, sum_sq = 0.0;
-    double weighted[3] = {0.0, 0.0, 0.0};
-    int32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
-    uint32_t peak_value = 0;
-    double peak[3] = {0.0, 0.0, 0.0};
-    for (uint32_t k = 0; k < n; k++) {
-      const uint32_t g = order[s0 + k];
-      const uint32_t idx = node_index[g];
-      const int32_t x = (int32_t)(idx % a.width);
-      const int32_t y = (int32_t)(idx / a.width);
-      const int32_t z = node_frame[g];
-      if (k == 0) {
-        x0 = x;
-        x1 = x + 1;
-        y0 = y;
-        y1 = y + 1;
-        z0 = z;
-        z1 = z + 1;
-      } else {
-        x0 = x < x0 ? x : x0;
-        x1 = x + 1 > x1 ? x + 1 : x1;
-        y0 = y < y0 ? y : y0;
-        y1 = y + 1 > y1 ? y + 1 : y1;
-        z0 = z < z0 ? z : z0;
-        z1 = z + 1 > z1 ? z + 1 : z1;
-      }
-      const double value = (double)node_value[g];
-      const double position[3] = {(double)x + 0.5, (double)y + 0.5,
-                                  (double)z + 0.5};
-      sum += value;
-      sum_sq += value * value;
-      for (int axis = 0; axis < 3; axis++)
-        weighted[axis] += value * position[axis];
-      if (node_value[g] > peak_value) {
-        peak_value = node_value[g];
-        peak[0] = position[0];
-        peak[1] = position[1];
-        peak[2] = position[2];
-      }
-    }
-    spot.bbox[0] = x0;
-    spot.bbox[1] = x1;
-    spot.bbox[2] = y0;
-    spot.bbox[3] = y1;
-    spot.bbox[4] = z0;
-    spot.bbox[5] = z1;
-    spot.intensity = sum;
-    spot.intensity_variance = sum;
-    if (sum > 0.0) {
-      for (int axis = 0; axis < 3; axis++)
-        spot.position[axis] = weighted[axis] / sum;
-      double delta_sq[3] = {0.0, 0.0, 0.0};
-      for (uint32_t k = 0; k < n; k++) {
+      kind = kTooSmall;
+    } else if (n > a.max_size) {
+      kind = kTooLarge;
+    } else {
+      // pass 1: bbox, peak, integer sums, bounds
+      int32_t x0 = INT32_MAX, x1 = INT32_MIN, y0 = INT32_MAX, y1 = INT32_MIN,
+              z0 = INT32_MAX, z1 = INT32_MIN;
+      uint64_t sv = 0, svv = 0, sx = 0, sy = 0;
+      int64_t sz = 0;
+      uint32_t vmax = 0, kmax = 0xFFFFFFFFu;
+      int32_t zabs = 0;
+      for (uint32_t k = lane; k < n; k += 64) {
         const uint32_t g = order[s0 + k];
         const uint32_t idx = node_index[g];
-        const double value = (double)node_value[g];
-        const double position[3] = {(double)(idx % a.width) + 0.5,
-                                    (double)(idx / a.width) + 0.5,
-                                    (double)node_frame[g] + 0.5};
-        for (int axis = 0; axis < 3; axis++) {
-          const double delta = position[axis] - spot.position[axis];
-          delta_sq[axis] += value * delta * delta;
+        const uint32_t v = node_value[g];
+        const int32_t x = (int32_t)(idx % a.width);
+        const int32_t y = (int32_t)(idx / a.width);
+        const int32_t z = node_frame[g];
+        x0 = min(x0, x);
+        x1 = max(x1, x + 1);
+        y0 = min(y0, y);
+        y1 = max(y1, y + 1);
+        z0 = min(z0, z);
+        z1 = max(z1, z + 1);
+        sv += v;
+        svv += (uint64_t)v * v; // exact while v < 2^32
+        sx += (uint64_t)v * (uint64_t)(2 * x + 1);
+        sy += (uint64_t)v * (uint64_t)(2 * y + 1);
+        sz += (int64_t)v * (int64_t)(2 * (int64_t)z + 1);
+        zabs = max(zabs, z < 0 ? -z : z);
+        if (v > vmax) { // lanes see members in order, so first per lane
+          vmax = v;
+          kmax = k;
         }
       }
-      const double twelfth = 1.0 / 12.0;
-      if (sum * sum > sum_sq) {
-        for (int axis = 0; axis < 3; axis++) {
-          const double variance = delta_sq[axis] * sum / (sum * sum - sum_sq);
-          spot.variance[axis] = variance / sum + twelfth;
-        }
+      x0 = wave_min(x0);
+      x1 = wave_max(x1);
+      y0 = wave_min(y0);
+      y1 = wave_max(y1);
+      z0 = wave_min(z0);
+      z1 = wave_max(z1);
+      zabs = wave_max(zabs);
+      const uint32_t vtop = wave_max(vmax);
+      const uint32_t ktop = wave_min(vmax == vtop ? kmax : 0xFFFFFFFFu);
+      sf_spot &spot = r.spot;
+      // Exactness of the reference's double sums: every term and partial
+      // sum below 2^52 (with margin) in magnitude.
+      const double bound = 4503599627370496.0; // 2^52
+      const double top = (double)vtop, cnt = (double)n;
+      const double reach = (double)max(max(a.width, a.pixels / a.width),
+                                       (uint32_t)zabs) * 2.0 + 1.0;
+      const bool exact = cnt * top * top < bound && cnt * top * reach < bound;
+      if (!exact) {
+        if (lane == 0)
+          kind = emit_sequential(a, s0, n, order, node_frame, node_index,
+                                 node_value, spot);
+        kind = __shfl(kind, 0, 64);
       } else {
-        for (int axis = 0; axis < 3; axis++)
-          spot.variance[axis] = twelfth;
+        sv = wave_sum(sv);
+        svv = wave_sum(svv);
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        sz = wave_sum(sz);
+        spot.n_signal = n;
+        spot.bbox[0] = x0;
+        spot.bbox[1] = x1;
+        spot.bbox[2] = y0;
+        spot.bbox[3] = y1;
+        spot.bbox[4] = z0;
+        spot.bbox[5] = z1;
+        const double sum = (double)sv, sum_sq = (double)svv;
+        const double weighted[3] = {(double)sx * 0.5, (double)sy * 0.5,
+                                    (double)sz * 0.5};
+        double peak[3] = {0.0, 0.0, 0.0};
+        if (vtop > 0) {
+          const uint32_t g = order[s0 + ktop];
+          const uint32_t idx = node_index[g];
+          peak[0] = (double)(idx % a.width) + 0.5;
+          peak[1] = (double)(idx / a.width) + 0.5;
+          peak[2] = (double)node_frame[g] + 0.5;
+        }
+        kind = finish_spot(
+            a, spot, sum, sum_sq, weighted, peak,
+            [&](const double (&centre)[3], double (&delta_sq)[3]) {
+              for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+                const uint32_t k = k0 + lane;
+                double t[3] = {0.0, 0.0, 0.0};
+                if (k < n) {
+                  const uint32_t g = order[s0 + k];
+                  const uint32_t idx = node_index[g];
+                  const double value = (double)node_value[g];
+                  const double position[3] = {(double)(idx % a.width) + 0.5,
+                                              (double)(idx / a.width) + 0.5,
+                                              (double)node_frame[g] + 0.5};
+                  for (int axis = 0; axis < 3; axis++) {
+                    const double delta = position[axis] - centre[axis];
+                    t[axis] = value * delta * delta;
+                  }
+                }
+                const uint32_t m = min(64u, n - k0);
+                for (uint32_t i = 0; i < m; i++)
+                  for (int axis = 0; axis < 3; axis++)
+                    delta_sq[axis] += wave_read(t[axis], i);
+              }
+            });
       }
-      if (spot.bbox[5] == spot.bbox[4] + 1)
-        spot.position[2] = (double)spot.bbox[4] + 0.5;
-    } else {
-      spot.position[0] = (spot.bbox[1] + spot.bbox[0]) / 2.0;
-      spot.position[1] = (spot.bbox[3] + spot.bbox[2]) / 2.0;
-      spot.position[2] = (spot.bbox[5] + spot.bbox[4]) / 2.0;
     }
-    if (a.max_separation > 0.0) {
-      const double dx = peak[0] - spot.position[0];
-      const double dy = peak[1] - spot.position[1];
-      const double dz = peak[2] - spot.position[2];
-      if (sqrt(dx * dx + dy * dy + dz * dz) > a.max_separation) {
-        separated++;
-        r.accepted = 0;
-        out[c] = r;
-        continue;
-      }
+    if (kind == kAccepted) {
+      r.accepted = 1;
+      r.members = n;
     }
-    r.accepted = 1;
-    r.members = n;
-    accepted++;
-    out[c] = r;
+    if (lane == 0) {
+      out[c] = r;
+      tally[kind]++;
+    }
   }
   // counts: groups, too_small, too_large, separated, accepted
-  if (small)
-    atomicAdd(&counts[2], (unsigned long long)small);
-  if (large)
-    atomicAdd(&counts[3], (unsigned long long)large);
-  if (separated)
-    atomicAdd(&counts[4], (unsigned long long)separated);
-  if (accepted)
-    atomicAdd(&counts[5], (unsigned long long)accepted);
+  if (lane == 0) {
+    if (tally[kTooSmall])
+      atomicAdd(&counts[2], (unsigned long long)tally[kTooSmall]);
+    if (tally[kTooLarge])
+      atomicAdd(&counts[3], (unsigned long long)tally[kTooLarge]);
+    if (tally[kSeparated])
+      atomicAdd(&counts[4], (unsigned long long)tally[kSeparated]);
+    if (tally[kAccepted])
+      atomicAdd(&counts[5], (unsigned long long)tally[kAccepted]);
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0)
     atomicAdd(&counts[1], (unsigned long long)comps);
 }
@@ -392,16 +560,12 @@ __global__ void spot_flags(const SpotOut *res, const uint32_t *n_comps,
   }
 }
 
-// Appends the kept spots and their pixels after what earlier chunks wrote.
+// Appends the kept spots after what earlier chunks wrote.
 __global__ void write_spots(const SpotOut *res, const uint32_t *n_comps,
-                            const uint32_t *starts, const uint32_t *order,
                             const uint32_t *spot_rank,
                             const uint32_t *pixel_rank,
-                            const int32_t *node_frame,
-                            const uint32_t *node_index,
-                            const uint32_t *node_value,
                             const unsigned long long *totals,
-                            sf_spot *spots, sf_pixel *pixels) {
+                            sf_spot *spots) {
   const uint32_t comps = *n_comps;
   const unsigned long long spot_base = totals[0], pixel_base = totals[1];
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
@@ -409,18 +573,37 @@ __global__ void write_spots(const SpotOut *res, const uint32_t *n_comps,
     if (!res[c].accepted)
       continue;
     sf_spot s = res[c].spot;
-    const unsigned long long first = pixel_base + pixel_rank[c];
-    s.first = first;
+    s.first = pixel_base + pixel_rank[c];
     spots[spot_base + spot_rank[c]] = s;
-    const uint32_t s0 = starts[c];
-    for (uint32_t k = 0; k < res[c].members; k++) {
-      const uint32_t g = order[s0 + k];
-      sf_pixel p;
-      p.frame = node_frame[g];
-      p.index = node_index[g];
-      p.value = node_value[g];
-      pixels[first + k] = p;
-    }
+  }
+}
+
+// Appends the kept spots' pixels, one thread per member: entry i of the
+// emission order belongs to component start_rank[i] + start_flag[i] - 1 and
+// is member i - starts[c] of it.
+__global__ void write_pixels(const SpotOut *res, const uint32_t *n_closed,
+                             const uint32_t *start_flag,
+                             const uint32_t *start_rank,
+                             const uint32_t *starts, const uint32_t *order,
+                             const uint32_t *pixel_rank,
+                             const int32_t *node_frame,
+                             const uint32_t *node_index,
+                             const uint32_t *node_value,
+                             const unsigned long long *totals,
+                             sf_pixel *pixels) {
+  const uint32_t end = *n_closed;
+  const unsigned long long pixel_base = totals[1];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t c = start_rank[i] + start_flag[i] - 1;
+    if (!res[c].accepted)
+      continue;
+    const uint32_t g = order[i];
+    sf_pixel p;
+    p.frame = node_frame[g];
+    p.index = node_index[g];
+    p.value = node_value[g];
+    pixels[pixel_base + pixel_rank[c] + (i - starts[c])] = p;
   }
 }
 
@@ -780,10 +963,13 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
                                     l->mem_rank.ptr, 0u, (size_t)n,
                                     rocprim::plus<uint32_t>(), s),
             "placing pixels");
-  sf::write_spots<<<grid, 256, 0, s>>>(
-      l->results.ptr, l->d_scalars, l->starts.ptr, l->order.ptr,
-      l->acc_rank.ptr, l->mem_rank.ptr, l->node_frame.ptr, l->node_index.ptr,
-      l->node_value.ptr, l->d_totals, l->spots.ptr, l->pixels.ptr);
+  sf::write_spots<<<grid, 256, 0, s>>>(l->results.ptr, l->d_scalars,
+                                       l->acc_rank.ptr, l->mem_rank.ptr,
+                                       l->d_totals, l->spots.ptr);
+  sf::write_pixels<<<grid, 256, 0, s>>>(
+      l->results.ptr, l->d_scalars + 1, l->start_flag.ptr, l->start_rank.ptr,
+      l->starts.ptr, l->order.ptr, l->mem_rank.ptr, l->node_frame.ptr,
+      l->node_index.ptr, l->node_value.ptr, l->d_totals, l->pixels.ptr);
   sf::advance_totals<<<1, 1, 0, s>>>(l->d_scalars, l->acc_rank.ptr,
                                      l->mem_rank.ptr, l->results.ptr,
                                      l->d_totals);

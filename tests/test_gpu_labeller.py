@@ -112,6 +112,30 @@ def test_random_sequences_match_oracle(sf):
         assert same_tables(a, b), (trial, options, a[2], b[2])
 
 
+def test_large_values_and_large_components(sf):
+    """Values where the double sums stop being exact (so the device takes
+    the sequential member-order path) next to components of hundreds of
+    members (several wave-sized chunks of the variance loop), against the
+    oracle."""
+    rng = np.random.default_rng(33)
+    for trial, top in enumerate((65535, 2 ** 26, 2 ** 32 - 3, 3000)):
+        h, w = 40, 50
+        seq = []
+        for z in range(6):
+            mask = rng.random(h * w) < (0.45 if trial % 2 else 0.25)
+            mask[5 * w:25 * w] = True  # a slab that joins into one big spot
+            idx = np.flatnonzero(mask).astype(np.uint32)
+            p = np.zeros(len(idx), dtype=cpu.SIGNAL)
+            p["index"] = idx
+            p["value"] = rng.integers(1, top, len(idx), dtype=np.uint64)
+            seq.append((z, p))
+        for options in ({}, {"max_spot_size": 100000}, {"two_d": True}):
+            a = label_host(cpu.Labeller, seq, h, w, **options)
+            b = label_host(sf.Labeller, seq, h, w, **options)
+            assert same_tables(a, b), (trial, options, a[2], b[2])
+            assert a[2]["accepted"] > 0 or a[2]["too_large"] > 0
+
+
 def test_golden_tables_end_to_end(sf):
     """Device threshold then device grouping against the tables the
     compiled reference wrote."""
