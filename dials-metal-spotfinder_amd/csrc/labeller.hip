@@ -35,7 +35,6 @@
 
 namespace sf {
 
-constexpr uint64_t kOpenKey = ~0ull;
 
 struct ChunkDesc {
   const sf_signal_pixel *list; // device list of this frame
@@ -146,36 +145,97 @@ __global__ void gather_nodes(LabelArgs a, const ChunkDesc *frames,
   }
 }
 
+// row_start[slot * (height + 1) + y] = the first node of `slot` whose index
+// is at least y * width (slot 0: the carried nodes of the last carried
+// frame; slot k + 1: chunk frame k). One thread per (slot, row).
+__global__ void fill_rows(LabelArgs a, const ChunkDesc *frames,
+                          const uint32_t *node_index, uint32_t *row_start) {
+  const uint32_t height = a.pixels / a.width;
+  const uint32_t total = (a.n_frames + 1) * (height + 1);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += gridDim.x * blockDim.x) {
+    const uint32_t slot = t / (height + 1), y = t % (height + 1);
+    uint32_t from, to;
+    if (slot == 0) {
+      from = a.carry_tail;
+      to = a.n_carry;
+    } else {
+      from = frames[slot - 1].offset;
+      to = slot < a.n_frames ? frames[slot].offset : a.n_nodes;
+    }
+    const uint32_t want = y * a.width; // y == height: past every index
+    uint32_t lo = from, hi = to;
+    if (y == height) {
+      lo = to;
+    } else {
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (node_index[mid] < want)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+    }
+    row_start[t] = lo;
+  }
+}
+
+// Horizontal runs without atomics: every chunk node points at the first
+// node of its run (consecutive indices on one row of one frame), which is
+// lower, so roots still point down.
+__global__ void link_runs(LabelArgs a, const ChunkDesc *frames,
+                          const uint32_t *node_index, uint32_t *parent,
+                          uint32_t *run) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
+       g += gridDim.x * blockDim.x) {
+    if (g < a.n_carry) { // carried nodes already point at their roots
+      run[g] = g;
+      continue;
+    }
+    const int k = chunk_frame_of(frames, a.n_frames, g);
+    const uint32_t from = frames[k].offset;
+    uint32_t r = g, idx = node_index[g];
+    while (r > from && idx % a.width != 0 && node_index[r - 1] == idx - 1) {
+      r--;
+      idx--;
+    }
+    parent[g] = r;
+    run[g] = r;
+  }
+}
+
+// The up and previous-frame edges (src/dials_spots.cc:106-137). An edge is
+// skipped when the left neighbour has the same edge one column over: the two
+// targets are then neighbours on their own row and already joined, as are
+// the node and its left neighbour.
 __global__ void unite_edges(LabelArgs a, const ChunkDesc *frames,
-                            const uint32_t *node_index, uint32_t *parent) {
+                            const uint32_t *node_index,
+                            const uint32_t *row_start, uint32_t *parent) {
+  const uint32_t stride = a.pixels / a.width + 1;
   for (uint32_t g = a.n_carry + blockIdx.x * blockDim.x + threadIdx.x;
        g < a.n_nodes; g += gridDim.x * blockDim.x) {
     const int k = chunk_frame_of(frames, a.n_frames, g);
     const uint32_t from = frames[k].offset;
-    const uint32_t to =
-        k + 1 < (int)a.n_frames ? frames[k + 1].offset : a.n_nodes;
     const uint32_t idx = node_index[g];
-    if (g > from && idx % a.width != 0 && node_index[g - 1] == idx - 1)
-      unite(parent, g, g - 1);
-    if (idx >= a.width) {
-      const long up = find_index(node_index, from, g, idx - a.width);
-      if (up >= 0)
+    const uint32_t y = idx / a.width;
+    const uint32_t *rows = row_start + (size_t)(k + 1) * stride;
+    const bool left =
+        g > from && idx % a.width != 0 && node_index[g - 1] == idx - 1;
+    if (y > 0) {
+      const uint32_t lo = rows[y - 1];
+      const long up = find_index(node_index, lo, rows[y], idx - a.width);
+      if (up >= 0 && !(left && (uint32_t)up > lo &&
+                       node_index[up - 1] == idx - a.width - 1))
         unite(parent, g, (uint32_t)up);
     }
     if (!a.two_d && frames[k].linked) {
-      uint32_t pf, pt;
-      if (k == 0) {
-        pf = a.carry_tail;
-        pt = a.n_carry;
-      } else {
-        pf = frames[k - 1].offset;
-        pt = from;
-      }
-      const long back = find_index(node_index, pf, pt, idx);
-      if (back >= 0)
+      const uint32_t *prev = row_start + (size_t)k * stride;
+      const uint32_t lo = prev[y];
+      const long back = find_index(node_index, lo, prev[y + 1], idx);
+      if (back >= 0 &&
+          !(left && (uint32_t)back > lo && node_index[back - 1] == idx - 1))
         unite(parent, g, (uint32_t)back);
     }
-    (void)to;
   }
 }
 
@@ -184,53 +244,71 @@ __global__ void unite_edges(LabelArgs a, const ChunkDesc *frames,
 // walk it would let a path-halving write replace a finished root with an
 // intermediate ancestor.)
 __global__ void resolve_roots(LabelArgs a, const uint32_t *parent,
-                              const uint32_t *slot, uint32_t *root,
-                              uint32_t *last_slot) {
+                              const uint32_t *run, const uint32_t *slot,
+                              uint32_t *root, uint32_t *last_slot) {
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
        g += gridDim.x * blockDim.x) {
+    if (run[g] != g)
+      continue; // resolved from its run's first node below
     uint32_t r = g;
     while (parent[r] != r)
       r = parent[r];
     root[g] = r;
-    atomicMax(&last_slot[r], slot[g]);
+    if (slot[g]) // a run lies on one frame: one update per run
+      atomicMax(&last_slot[r], slot[g]);
   }
 }
 
-// Sort keys: closed components by (last slot, root); open ones last. Also
-// the flags of the nodes that stay open.
+__global__ void resolve_runs(LabelArgs a, const uint32_t *run,
+                             uint32_t *root) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
+       g += gridDim.x * blockDim.x) {
+    const uint32_t r = run[g];
+    if (r != g)
+      root[g] = root[r];
+  }
+}
+
+// Sort keys: closed components by (last slot, root), the slot shifted left
+// by `shift`; open ones last (all bits set). Also the flags of the nodes that
+// stay open.
+template <typename K>
 __global__ void make_keys(LabelArgs a, const uint32_t *root,
-                          const uint32_t *last_slot, uint64_t *keys,
-                          uint32_t *values, uint32_t *open_flag) {
+                          const uint32_t *last_slot, K *keys,
+                          uint32_t *values, uint32_t *open_flag, int shift) {
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
        g += gridDim.x * blockDim.x) {
     const uint32_t r = root[g];
     const uint32_t last = last_slot[r];
     const bool open = !a.closing && !a.two_d && a.n_frames > 0 &&
                       last == a.n_frames;
-    keys[g] = open ? kOpenKey : (((uint64_t)last << 32) | r);
+    keys[g] = open ? ~(K)0 : (K)(((K)last << shift) | r);
     values[g] = g;
     open_flag[g] = open ? 1u : 0u;
   }
 }
 
 // start_flag[i] = sorted entry i begins a closed component.
-__global__ void mark_starts(LabelArgs a, const uint64_t *keys,
+template <typename K>
+__global__ void mark_starts(LabelArgs a, const K *keys, K open_key,
                             uint32_t *start_flag) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_nodes;
        i += gridDim.x * blockDim.x)
     start_flag[i] =
-        (keys[i] != kOpenKey && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
+        (keys[i] != open_key && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
 }
 
+template <typename K>
 __global__ void list_starts(LabelArgs a, const uint32_t *start_flag,
                             const uint32_t *start_rank, uint32_t *starts,
-                            const uint64_t *keys, uint32_t *n_closed_nodes) {
+                            const K *keys, K open_key,
+                            uint32_t *n_closed_nodes) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_nodes;
        i += gridDim.x * blockDim.x) {
     if (start_flag[i])
       starts[start_rank[i]] = i;
-    const bool closed = keys[i] != kOpenKey;
-    const bool next_open = i + 1 == a.n_nodes || keys[i + 1] == kOpenKey;
+    const bool closed = keys[i] != open_key;
+    const bool next_open = i + 1 == a.n_nodes || keys[i + 1] == open_key;
     if (closed && next_open)
       *n_closed_nodes = i + 1;
   }
@@ -423,6 +501,24 @@ __global__ __launch_bounds__(256) void finalise(
     } else if (n > a.max_size) {
       kind = kTooLarge;
     } else {
+      // Members in registers: lane holds members lane + 64 j, j < KMAX.
+      constexpr int KMAX = 16;
+      const bool held = n <= 64u * KMAX;
+      uint32_t m_idx[KMAX], m_val[KMAX];
+      int32_t m_z[KMAX];
+#pragma unroll
+      for (int j = 0; j < KMAX; j++) {
+        const uint32_t k = lane + 64u * j;
+        m_idx[j] = 0;
+        m_val[j] = 0;
+        m_z[j] = 0;
+        if (held && k < n) {
+          const uint32_t g = order[s0 + k];
+          m_idx[j] = node_index[g];
+          m_val[j] = node_value[g];
+          m_z[j] = node_frame[g];
+        }
+      }
       // pass 1: bbox, peak, integer sums, bounds
       int32_t x0 = INT32_MAX, x1 = INT32_MIN, y0 = INT32_MAX, y1 = INT32_MIN,
               z0 = INT32_MAX, z1 = INT32_MIN;
@@ -430,13 +526,9 @@ __global__ __launch_bounds__(256) void finalise(
       int64_t sz = 0;
       uint32_t vmax = 0, kmax = 0xFFFFFFFFu;
       int32_t zabs = 0;
-      for (uint32_t k = lane; k < n; k += 64) {
-        const uint32_t g = order[s0 + k];
-        const uint32_t idx = node_index[g];
-        const uint32_t v = node_value[g];
+      auto visit = [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
         const int32_t x = (int32_t)(idx % a.width);
         const int32_t y = (int32_t)(idx / a.width);
-        const int32_t z = node_frame[g];
         x0 = min(x0, x);
         x1 = max(x1, x + 1);
         y0 = min(y0, y);
@@ -452,6 +544,19 @@ __global__ __launch_bounds__(256) void finalise(
         if (v > vmax) { // lanes see members in order, so first per lane
           vmax = v;
           kmax = k;
+        }
+      };
+      if (held) {
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) {
+          const uint32_t k = lane + 64u * j;
+          if (k < n)
+            visit(k, m_idx[j], m_val[j], m_z[j]);
+        }
+      } else {
+        for (uint32_t k = lane; k < n; k += 64) {
+          const uint32_t g = order[s0 + k];
+          visit(k, node_index[g], node_value[g], node_frame[g]);
         }
       }
       x0 = wave_min(x0);
@@ -500,28 +605,47 @@ __global__ __launch_bounds__(256) void finalise(
           peak[1] = (double)(idx / a.width) + 0.5;
           peak[2] = (double)node_frame[g] + 0.5;
         }
+        // the variance terms of one 64-member chunk, added in member order
+        auto chunk_terms = [&](uint32_t k0, uint32_t idx, uint32_t v,
+                               int32_t z, const double (&centre)[3],
+                               double (&delta_sq)[3]) {
+          double t[3] = {0.0, 0.0, 0.0};
+          if (k0 + lane < n) {
+            const double value = (double)v;
+            const double position[3] = {(double)(idx % a.width) + 0.5,
+                                        (double)(idx / a.width) + 0.5,
+                                        (double)z + 0.5};
+            for (int axis = 0; axis < 3; axis++) {
+              const double delta = position[axis] - centre[axis];
+              t[axis] = value * delta * delta;
+            }
+          }
+          const uint32_t m = min(64u, n - k0);
+          for (uint32_t i = 0; i < m; i++)
+            for (int axis = 0; axis < 3; axis++)
+              delta_sq[axis] += wave_read(t[axis], i);
+        };
         kind = finish_spot(
             a, spot, sum, sum_sq, weighted, peak,
             [&](const double (&centre)[3], double (&delta_sq)[3]) {
-              for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-                const uint32_t k = k0 + lane;
-                double t[3] = {0.0, 0.0, 0.0};
-                if (k < n) {
-                  const uint32_t g = order[s0 + k];
-                  const uint32_t idx = node_index[g];
-                  const double value = (double)node_value[g];
-                  const double position[3] = {(double)(idx % a.width) + 0.5,
-                                              (double)(idx / a.width) + 0.5,
-                                              (double)node_frame[g] + 0.5};
-                  for (int axis = 0; axis < 3; axis++) {
-                    const double delta = position[axis] - centre[axis];
-                    t[axis] = value * delta * delta;
+              if (held) {
+#pragma unroll
+                for (int j = 0; j < KMAX; j++)
+                  if (64u * j < n)
+                    chunk_terms(64u * j, m_idx[j], m_val[j], m_z[j], centre,
+                                delta_sq);
+              } else {
+                for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+                  uint32_t idx = 0, v = 0;
+                  int32_t z = 0;
+                  if (k0 + lane < n) {
+                    const uint32_t g = order[s0 + k0 + lane];
+                    idx = node_index[g];
+                    v = node_value[g];
+                    z = node_frame[g];
                   }
+                  chunk_terms(k0, idx, v, z, centre, delta_sq);
                 }
-                const uint32_t m = min(64u, n - k0);
-                for (uint32_t i = 0; i < m; i++)
-                  for (int axis = 0; axis < 3; axis++)
-                    delta_sq[axis] += wave_read(t[axis], i);
               }
             });
       }
@@ -535,19 +659,25 @@ __global__ __launch_bounds__(256) void finalise(
       tally[kind]++;
     }
   }
-  // counts: groups, too_small, too_large, separated, accepted
-  if (lane == 0) {
-    if (tally[kTooSmall])
-      atomicAdd(&counts[2], (unsigned long long)tally[kTooSmall]);
-    if (tally[kTooLarge])
-      atomicAdd(&counts[3], (unsigned long long)tally[kTooLarge]);
-    if (tally[kSeparated])
-      atomicAdd(&counts[4], (unsigned long long)tally[kSeparated]);
-    if (tally[kAccepted])
-      atomicAdd(&counts[5], (unsigned long long)tally[kAccepted]);
+  // counts: groups, too_small, too_large, separated, accepted -- summed
+  // over the block first, one atomic per counter and block
+  __shared__ uint32_t block_tally[4];
+  if (threadIdx.x < 4)
+    block_tally[threadIdx.x] = 0;
+  __syncthreads();
+  if (lane == 0)
+    for (int t = 0; t < 4; t++)
+      if (tally[t])
+        atomicAdd(&block_tally[t], tally[t]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int slot[4] = {5, 2, 3, 4}; // kAccepted, kTooSmall, ...
+    for (int t = 0; t < 4; t++)
+      if (block_tally[t])
+        atomicAdd(&counts[slot[t]], (unsigned long long)block_tally[t]);
+    if (blockIdx.x == 0)
+      atomicAdd(&counts[1], (unsigned long long)comps);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0)
-    atomicAdd(&counts[1], (unsigned long long)comps);
 }
 
 __global__ void spot_flags(const SpotOut *res, const uint32_t *n_comps,
@@ -738,6 +868,7 @@ struct sf_labeller {
   sf::DevArray<uint32_t> node_index, node_value, node_slot, parent, root,
       last_slot;
   sf::DevArray<uint64_t> keys, keys_sorted;
+  sf::DevArray<uint32_t> row_start; // per (slot, row): first node
   sf::DevArray<uint32_t> values, order, open_flag, open_rank, start_flag,
       start_rank, starts;
   sf::DevArray<sf::SpotOut> results;
@@ -880,10 +1011,25 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(l->chunk.ensure(std::max<uint32_t>(n_frames, 1)), "alloc");
   // temp storage: the largest of the rocPRIM calls
   size_t need = 0, bytes = 0;
-  LAB_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, l->keys.ptr,
-                                      l->keys_sorted.ptr, l->values.ptr,
-                                      l->order.ptr, n, 0, 64, s),
-            "sort size");
+  // Sort keys: 32 bits when (last slot, root) fit, else 64.
+  int root_bits = 1, slot_bits = 1;
+  while (root_bits < 32 && (1ull << root_bits) < (uint64_t)n)
+    root_bits++;
+  while ((1ull << slot_bits) < (uint64_t)n_frames + 2)
+    slot_bits++;
+  const bool narrow = root_bits + slot_bits <= 32;
+  uint32_t *keys32 = reinterpret_cast<uint32_t *>(l->keys.ptr);
+  uint32_t *keys32_sorted = reinterpret_cast<uint32_t *>(l->keys_sorted.ptr);
+  if (narrow)
+    LAB_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, keys32, keys32_sorted,
+                                        l->values.ptr, l->order.ptr, n, 0,
+                                        root_bits + slot_bits, s),
+              "sort size");
+  else
+    LAB_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, l->keys.ptr,
+                                        l->keys_sorted.ptr, l->values.ptr,
+                                        l->order.ptr, n, 0, 64, s),
+              "sort size");
   need = std::max(need, bytes);
   LAB_CHECK(rocprim::exclusive_scan(nullptr, bytes, l->open_flag.ptr,
                                     l->open_rank.ptr, 0u, (size_t)n,
@@ -919,21 +1065,50 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
     LAB_CHECK(hipEventRecord(l->done, gs), "event");
     LAB_CHECK(hipStreamWaitEvent(s, l->done, 0), "event wait");
   }
+  {
+    const size_t rows = (size_t)(n_frames + 1) * (l->height + 1);
+    LAB_CHECK(l->row_start.ensure(rows), "alloc");
+    sf::fill_rows<<<grid_for(rows), 256, 0, s>>>(a, l->chunk.ptr,
+                                                 l->node_index.ptr,
+                                                 l->row_start.ptr);
+  }
+  // `values` holds each node's run start until make_keys reuses it
+  sf::link_runs<<<grid, 256, 0, s>>>(a, l->chunk.ptr, l->node_index.ptr,
+                                     l->parent.ptr, l->values.ptr);
   sf::unite_edges<<<grid, 256, 0, s>>>(a, l->chunk.ptr, l->node_index.ptr,
-                                       l->parent.ptr);
-  sf::resolve_roots<<<grid, 256, 0, s>>>(a, l->parent.ptr, l->node_slot.ptr,
-                                         l->root.ptr, l->last_slot.ptr);
-  sf::make_keys<<<grid, 256, 0, s>>>(a, l->root.ptr, l->last_slot.ptr,
-                                     l->keys.ptr, l->values.ptr,
-                                     l->open_flag.ptr);
+                                       l->row_start.ptr, l->parent.ptr);
+  sf::resolve_roots<<<grid, 256, 0, s>>>(a, l->parent.ptr, l->values.ptr,
+                                         l->node_slot.ptr, l->root.ptr,
+                                         l->last_slot.ptr);
+  sf::resolve_runs<<<grid, 256, 0, s>>>(a, l->values.ptr, l->root.ptr);
+  if (narrow)
+    sf::make_keys<uint32_t><<<grid, 256, 0, s>>>(
+        a, l->root.ptr, l->last_slot.ptr, keys32, l->values.ptr,
+        l->open_flag.ptr, root_bits);
+  else
+    sf::make_keys<uint64_t><<<grid, 256, 0, s>>>(
+        a, l->root.ptr, l->last_slot.ptr, l->keys.ptr, l->values.ptr,
+        l->open_flag.ptr, 32);
   LAB_CHECK(hipGetLastError(), "label kernels");
   bytes = l->temp.cap;
-  LAB_CHECK(rocprim::radix_sort_pairs(l->temp.ptr, bytes, l->keys.ptr,
-                                      l->keys_sorted.ptr, l->values.ptr,
-                                      l->order.ptr, n, 0, 64, s),
-            "sorting components");
-  sf::mark_starts<<<grid, 256, 0, s>>>(a, l->keys_sorted.ptr,
-                                       l->start_flag.ptr);
+  if (narrow) {
+    LAB_CHECK(rocprim::radix_sort_pairs(l->temp.ptr, bytes, keys32,
+                                        keys32_sorted, l->values.ptr,
+                                        l->order.ptr, n, 0,
+                                        root_bits + slot_bits, s),
+              "sorting components");
+    // closed keys stay below 2^(root_bits + slot_bits) - 1, so the open key
+    // (all ones) sorts last on those bits too
+    sf::mark_starts<uint32_t><<<grid, 256, 0, s>>>(a, keys32_sorted,
+                                                   ~0u, l->start_flag.ptr);
+  } else {
+    LAB_CHECK(rocprim::radix_sort_pairs(l->temp.ptr, bytes, l->keys.ptr,
+                                        l->keys_sorted.ptr, l->values.ptr,
+                                        l->order.ptr, n, 0, 64, s),
+              "sorting components");
+    sf::mark_starts<uint64_t><<<grid, 256, 0, s>>>(a, l->keys_sorted.ptr,
+                                                   ~0ull, l->start_flag.ptr);
+  }
   bytes = l->temp.cap;
   LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->start_flag.ptr,
                                     l->start_rank.ptr, 0u, (size_t)n,
@@ -941,12 +1116,20 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
             "ranking components");
   LAB_CHECK(hipMemsetAsync(l->d_scalars, 0, 2 * sizeof(uint32_t), s),
             "memset");
-  sf::list_starts<<<grid, 256, 0, s>>>(a, l->start_flag.ptr,
-                                       l->start_rank.ptr, l->starts.ptr,
-                                       l->keys_sorted.ptr, l->d_scalars + 1);
+  if (narrow)
+    sf::list_starts<uint32_t><<<grid, 256, 0, s>>>(
+        a, l->start_flag.ptr, l->start_rank.ptr, l->starts.ptr, keys32_sorted,
+        ~0u, l->d_scalars + 1);
+  else
+    sf::list_starts<uint64_t><<<grid, 256, 0, s>>>(
+        a, l->start_flag.ptr, l->start_rank.ptr, l->starts.ptr,
+        l->keys_sorted.ptr, ~0ull, l->d_scalars + 1);
   sf::count_components<<<1, 1, 0, s>>>(l->start_rank.ptr, l->start_flag.ptr,
                                        n, l->d_scalars);
-  sf::finalise<<<grid, 256, 0, s>>>(a, l->starts.ptr, l->d_scalars,
+  // a bounded grid: each wave loops over components, each block adds its
+  // tallies once
+  sf::finalise<<<std::min<uint32_t>(grid, 1024), 256, 0, s>>>(
+      a, l->starts.ptr, l->d_scalars,
                                     l->d_scalars + 1, l->order.ptr,
                                     l->node_frame.ptr, l->node_index.ptr,
                                     l->node_value.ptr, l->results.ptr,
@@ -1167,6 +1350,7 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   l->root.release();
   l->last_slot.release();
   l->keys.release();
+  l->row_start.release();
   l->keys_sorted.release();
   l->values.release();
   l->order.release();
