@@ -624,7 +624,9 @@ __global__ __launch_bounds__(256, 4) void probe_kernel(const FrameDesc *frames,
   const uint32_t padded = (items + 7) & ~7u;
   const uint32_t blocks = gridDim.x;
   for (uint32_t b = blockIdx.x; b < padded; b += blocks) {
-    const uint32_t item = blocks >= padded ? xcd_local_item(b, blocks) : b;
+    // each XCD takes a contiguous run of tiles (blocks and padded are
+    // multiples of 8, so a block stays on its XCD's run)
+    const uint32_t item = xcd_local_item(b, padded);
     if (item >= items)
       continue;
     const int f = frame_of_item(frames, n_frames, item);
@@ -642,9 +644,8 @@ int probe_tile_height() { return ProbeDefault::TH; }
 hipError_t launch_probe(const FrameDesc *frames, int n_frames, uint32_t items,
                         int blocks, hipStream_t stream) {
   const uint32_t padded = (items + 7) & ~7u;
-  const uint32_t grid =
-      blocks > 0 ? ((uint32_t)blocks < padded ? (uint32_t)blocks : padded)
-                 : padded;
+  const uint32_t asked = ((uint32_t)blocks + 7) & ~7u; // whole XCD rounds
+  const uint32_t grid = blocks > 0 ? (asked < padded ? asked : padded) : padded;
   probe_kernel<ProbeDefault><<<grid, 256, 0, stream>>>(frames, n_frames,
                                                        items);
   return hipGetLastError();
