@@ -2,21 +2,22 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--chunk C]
 
-Workload: a series of 1800 device-resident
-4148 x 4362 (W x H) uint16 frames, Poisson background 0.3, ~1500 Gaussian
-spots a frame (sigma 0.8-2 px, 1-5 frames, peaks up to 20,000), masked
-module gaps at the 514 x 1030 pitch and 0.05% hot pixels -- generated on the
-device from a seed. A step is one chunk of C frames through the hot path:
-the fused threshold with ordered compaction (one graph replay), then the
-device 3-D labelling and centroid pass for the chunk. `value` is frames/s
-over the timed steps with the inputs already in HBM.
+Workload: a series of 1800 device-resident 4148 x 4362 (W x H) uint16
+frames, Poisson background 0.3, ~1500 Gaussian spots a frame (sigma 0.8-2
+px, 1-5 frames, peaks up to 20,000), masked module gaps at the 514 x 1030
+pitch and 0.05% hot pixels -- generated on the device from a seed. A step is
+one chunk of C frames through the hot path: the threshold with ordered
+compaction (one graph replay), then the device 3-D labelling and centroid
+pass for the chunk. `value` is frames/s over the timed steps with the inputs
+already in HBM.
 
-Also reported: the fused kernel's roofline (algorithmic bytes per launch /
-its HIP-event time on the plan's stream, against 8 TB/s), the plain
-one-frame-per-launch form on the same frames, and the reference's own CPU
-path (oracle/_ref, compiled from the reference sources) timed on this
-host's cores on a bounded sample, whose lists are also compared with the
-device's (mask_diffs).
+Also reported: the threshold kernel's roofline (algorithmic bytes per
+launch / its HIP-event time on the plan's stream, against 8 TB/s), the plain
+one-frame-per-launch form on the same frames, the reference's own CPU path
+(oracle/_ref, compiled from the reference sources) timed on this host's
+cores on a bounded sample, whose lists are also compared with the device's
+(mask_diffs); then chunks of uneven frame sizes, a 32-bit series, and the
+serving loop on four concurrent series in real time.
 """
 
 from __future__ import annotations
@@ -97,6 +98,133 @@ def cpu_baseline(sf, host_frames, device_lists, threads, seconds):
         "signal_pixels_per_pass": int(total // passes),
         "parity_frames": min(n, 8),
     }, diffs
+
+
+BATCH_SHAPES = [(2162, 2068)] * 32 + [(3269, 3110)] * 16 + [(4362, 4148)] * 8
+
+
+def chunk_step_timing(sf, plan, chunks, steps, labeller=None, frames_per=0):
+    """Graph replays of `plan` over `chunks` (lists of device pointers) in
+    turn, with the labeller fed from each replay when given; returns
+    (seconds per step, median threshold kernel ms, median ordering ms)."""
+    plan.capture(chunks[0])
+    first = 0
+
+    def step(s):
+        nonlocal first
+        plan.bind(chunks[s % len(chunks)])
+        plan.replay()
+        if labeller is not None:
+            labeller.add_plan(plan, first)
+            first += frames_per
+
+    step(0)
+    plan.wait()
+    sf.lib().sf_hip_synchronize()
+    t0 = time.perf_counter()
+    for s in range(1, steps + 1):
+        step(s)
+    plan.wait()
+    if labeller is not None:
+        labeller.wait()
+    sf.lib().sf_hip_synchronize()
+    per_step = (time.perf_counter() - t0) / steps
+    kernel, order = [], []
+    plan.set_timing(True)
+    for s in range(3):
+        plan.run(chunks[s % len(chunks)])
+        plan.wait()
+        a, b = plan.last_times()
+        kernel.append(a)
+        order.append(b)
+    plan.set_timing(False)
+    return per_step, float(np.median(kernel)), float(np.median(order))
+
+
+def batched_workload(sf, fr, steps):
+    """Chunks of uneven frame sizes in one launch: 32 frames of 2162 x 2068,
+    16 of 3269 x 3110 and 8 of 4362 x 4148 (H x W) u16, the 1800-frame
+    series' value model scaled by area, two distinct chunks alternated."""
+    bufs, chunks = [], [[], []]
+    for k, (h, w) in enumerate(sorted(set(BATCH_SHAPES))):
+        count = BATCH_SHAPES.count((h, w))
+        buf = sf.DeviceBuffer(2 * h * w * count * 2)
+        spots = fr.diffraction_spots(200 + k, h, w, 2 * count,
+                                     1500.0 * h * w / (HEIGHT * WIDTH))
+        sf.synth_frames(buf, 2, h, w, 2 * count, 200 + k, 0.3, True, 0.0005,
+                        spots)
+        bufs.append(buf)
+        for c in range(2):
+            chunks[c] += [buf.ptr + 2 * h * w * (c * count + f)
+                          for f in range(count)]
+    shapes = sorted(BATCH_SHAPES)
+    plan = sf.Plan(2, shapes)
+    tuned = plan.autotune(chunks[0], 3)
+    per_step, kernel_ms, order_ms = chunk_step_timing(sf, plan, chunks, steps)
+    plan.run(chunks[0])
+    plan.wait()
+    signal = float(np.sum(plan.counts()))
+    pixel_bytes = float(sum(2 * h * w for h, w in shapes))
+    algorithmic = pixel_bytes + 16.0 * signal
+    plan.close()
+    for buf in bufs:
+        buf.close()
+    return {
+        "workload": "56-frame chunks: 32 x 2162x2068 + 16 x 3269x3110 + "
+                    "8 x 4362x4148 uint16 (H x W), one graph replay a chunk",
+        "frames_per_s": round(len(shapes) / per_step, 1),
+        "chunks_per_s": round(1.0 / per_step, 1),
+        "gb_per_s": round(algorithmic / per_step / 1e9, 1),
+        "kernel_ms_per_chunk": round(kernel_ms, 4),
+        "order_ms_per_chunk": round(order_ms, 4),
+        "kernel_gb_per_s": round(algorithmic / (kernel_ms / 1e3) / 1e9, 1),
+        "kernel_hbm_frac": round(algorithmic / (kernel_ms / 1e3) / PEAK_HBM,
+                                 4),
+        "algorithmic_bytes_per_chunk": int(algorithmic),
+        "autotuned": {"band_rows": tuned[0], "blocks": tuned[1]},
+    }
+
+
+def u32_workload(sf, fr, steps, chunk=30, series=120):
+    """The 32-bit path: a device-resident series of 4148 x 4362 uint32
+    frames (Poisson 50, spots up to ~10^7 counts, masked gaps at
+    0xFFFFFFFF, hot pixels), thresholded in chunks by the sweep kernel with
+    autotuned bands, then labelled on the device."""
+    buf = sf.DeviceBuffer(4 * HEIGHT * WIDTH * series)
+    spots = fr.diffraction_spots(300, HEIGHT, WIDTH, series, 1500.0,
+                                 peak=1.0e6, min_peak=100.0)
+    sf.synth_frames(buf, 4, HEIGHT, WIDTH, series, 300, 50.0, True, 0.0005,
+                    spots)
+    ptr = [buf.ptr + 4 * HEIGHT * WIDTH * f for f in range(series)]
+    chunks = [ptr[c * chunk:(c + 1) * chunk] for c in range(series // chunk)]
+    plan = sf.Plan(4, [(HEIGHT, WIDTH)] * chunk)
+    tuned = plan.autotune(chunks[0], 3)
+    labeller = sf.Labeller(HEIGHT, WIDTH)
+    per_step, kernel_ms, order_ms = chunk_step_timing(
+        sf, plan, chunks, steps, labeller, chunk)
+    plan.run(chunks[0])
+    plan.wait()
+    signal = float(np.mean(plan.counts()))
+    frame_bytes = 4.0 * HEIGHT * WIDTH
+    algorithmic = chunk * (frame_bytes + 16.0 * signal)
+    plan.close()
+    labeller.close()
+    buf.close()
+    return {
+        "workload": f"{series}-frame series of 4148x4362 uint32, Poisson 50, "
+                    f"~1500 spots/frame up to ~1e7 counts, gaps, 0.05% hot; "
+                    f"{chunk} frames a step",
+        "frames_per_s": round(chunk / per_step, 1),
+        "path": "sweep threshold + ordered compaction + 3-D labelling",
+        "kernel_ms_per_chunk": round(kernel_ms, 4),
+        "kernel_us_per_frame": round(kernel_ms / chunk * 1e3, 2),
+        "kernel_gb_per_s": round(algorithmic / (kernel_ms / 1e3) / 1e9, 1),
+        "kernel_hbm_frac": round(algorithmic / (kernel_ms / 1e3) / PEAK_HBM,
+                                 4),
+        "order_ms_per_chunk": round(order_ms, 4),
+        "mean_signal_pixels_per_frame": round(signal, 1),
+        "autotuned": {"band_rows": tuned[0], "blocks": tuned[1]},
+    }
 
 
 SERVE_SERIES = (
@@ -186,6 +314,8 @@ def main() -> int:
     ap.add_argument("--serve-ring", type=int, default=48)
     ap.add_argument("--serve-budget-gb", type=float, default=4.0)
     ap.add_argument("--serve-batch", type=int, default=16)
+    ap.add_argument("--other-steps", type=int, default=10,
+                    help="steps of the batched and 32-bit workloads (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy, three times (the "
                          "known-size kernel that scales FETCH_SIZE and "
@@ -227,9 +357,13 @@ def main() -> int:
     n_chunks = n_frames // chunk
 
     plan = sf.Plan(2, [(HEIGHT, WIDTH)] * chunk, band_rows=args.band)
+    tuned = plan.autotune(ptr[:chunk], 3)
+    log(f"autotuned (band rows, blocks): {tuned}")
     labeller = None
     if not args.no_label and hasattr(sf, "Labeller"):
         labeller = sf.Labeller(HEIGHT, WIDTH)
+    path = ("threshold + ordered compaction (one graph replay)"
+            + (" + 3-D labelling and centroids" if labeller else ""))
 
     # One captured graph (zero counters, threshold, ordering) replayed per
     # chunk; binding a chunk only rewrites the frame descriptors it reads.
@@ -274,7 +408,7 @@ def main() -> int:
     algorithmic = chunk * (frame_bytes + 16.0 * mean_signal)
     achieved = algorithmic / kernel_s
     traffic = None
-    traffic_file = os.path.join(ROOT, "profiles", "fused_threshold_traffic.json")
+    traffic_file = os.path.join(ROOT, "profiles", "threshold_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
             t = json.load(fh)
@@ -306,6 +440,16 @@ def main() -> int:
     baseline, diffs = cpu_baseline(sf, host, device_lists, threads,
                                    args.cpu_seconds)
 
+    others = {}
+    if args.other_steps > 0:
+        plan.close()
+        labeller = None
+        series.close()
+        others["batched_uneven"] = batched_workload(sf, fr, args.other_steps)
+        log(f"batched: {others['batched_uneven']}")
+        others["u32"] = u32_workload(sf, fr, args.other_steps)
+        log(f"u32: {others['u32']}")
+
     latency, serving = None, None
     if args.serve_seconds > 0:
         plan.close()
@@ -331,12 +475,12 @@ def main() -> int:
         "dtype": "u16",
         "data": "synthetic (device-generated, seeded)",
         "config": {
-            "workload": "config 1: 1800-frame series of 4148x4362 uint16, "
+            "workload": "1800-frame series of 4148x4362 uint16, "
                         "Poisson 0.3, ~1500 spots/frame, gaps, 0.05% hot",
             "frames_per_step": chunk,
+            "autotuned": {"band_rows": tuned[0], "blocks": tuned[1]},
             "series_frames": n_frames,
-            "path": "fused threshold + ordered compaction"
-                    + (" + 3-D labelling and centroids" if labeller else ""),
+            "path": path,
         },
         "gb_per_s": round(fps * (frame_bytes + 16 * mean_signal) / 1e9, 2),
         "hbm_frac_whole_path": round(
@@ -360,6 +504,7 @@ def main() -> int:
         "cpu_baseline": baseline,
         "p99_latency_ms": latency,
         "serving": serving,
+        "other_workloads": others,
     }
     print(json.dumps(record))
     plan.close()

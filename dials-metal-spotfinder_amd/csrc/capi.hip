@@ -130,6 +130,7 @@ struct sf_plan {
   uint8_t *d_cell_count = nullptr;
   uint32_t *d_cell_base = nullptr, *d_band_total = nullptr;
   uint32_t max_bands = 0;
+  size_t band_capacity = 0; // entries of d_band_total
   uint32_t *d_counters = nullptr, *d_found = nullptr;
   uint8_t *d_mask0 = nullptr, *d_mask1 = nullptr;
   size_t max_pixels = 0;
@@ -360,6 +361,7 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
                  hipSuccess;
   ok = ok && hipMalloc(&p->d_band_total, sizeof(uint32_t) * bands) ==
                  hipSuccess;
+  p->band_capacity = bands;
   ok = ok && hipMalloc(&p->d_counters, sizeof(uint32_t) * n_frames) ==
                  hipSuccess;
   ok = ok && hipMalloc(&p->d_found, sizeof(uint32_t) * n_frames) ==
@@ -408,6 +410,94 @@ int sf_hip_plan_set_grid(sf_plan *p, int blocks) {
     return bad("sf_hip_plan_set_grid: bad argument");
   p->blocks = blocks;
   p->graph_form = -1;
+  return SF_OK;
+}
+
+int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
+                         int *band_rows, int *blocks) {
+  if (!p || !frames || trials <= 0)
+    return bad("sf_hip_plan_autotune: bad argument");
+  const Kernel kernel = kernel_for(p, p->form);
+  if (kernel == kPlain)
+    return bad("sf_hip_plan_autotune: the plain form has nothing to tune");
+  int cus = 256;
+  {
+    int device = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&device) == hipSuccess &&
+        hipGetDeviceProperties(&prop, device) == hipSuccess)
+      cus = prop.multiProcessorCount;
+  }
+  std::vector<int> bands;
+  if (kernel == kSweep)
+    bands = {44, 108, 236, 492};
+  else
+    bands = {p->band_rows};
+  const int grids[] = {0, 4 * cus, 8 * cus, 16 * cus};
+  // band totals for the shortest candidate band
+  size_t need_bands = 0;
+  for (int f = 0; f < p->n; f++)
+    need_bands += (p->heights[f] + 43) / 44;
+  SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  if (p->exec) {
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  p->graph_form = -1;
+  if (kernel == kSweep && need_bands > p->band_capacity) {
+    uint32_t *grown = nullptr;
+    SF_CHECK(hipMalloc(&grown, sizeof(uint32_t) * need_bands),
+             "allocating band totals");
+    (void)hipFree(p->d_band_total);
+    p->d_band_total = grown;
+    p->band_capacity = need_bands;
+  }
+  const bool was_timing = p->timing;
+  double best = 1e30;
+  int best_band = p->band_rows, best_grid = p->blocks;
+  for (const int band : bands) {
+    p->band_rows = band;
+    layout(p);
+    p->bound.clear(); // force a descriptor upload
+    for (const int grid : grids) {
+      p->blocks = grid;
+      int status = sf_hip_plan_run(p, frames); // warm-up
+      if (status != SF_OK)
+        return status;
+      std::vector<float> times;
+      for (int t = 0; t < trials; t++) {
+        status = enqueue(p, true);
+        if (status != SF_OK)
+          return status;
+        SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        float ms = 0.0f;
+        SF_CHECK(hipEventElapsedTime(&ms, p->ev[0], p->ev[4]), "event time");
+        times.push_back(ms);
+      }
+      std::sort(times.begin(), times.end());
+      const double median = times[times.size() / 2];
+      if (median < best) {
+        best = median;
+        best_band = band;
+        best_grid = grid;
+      }
+    }
+  }
+  p->timing = was_timing;
+  p->band_rows = best_band;
+  p->blocks = best_grid;
+  layout(p);
+  p->bound.clear();
+  SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  p->upload_pending = false;
+  if (band_rows)
+    *band_rows = best_band;
+  if (blocks)
+    *blocks = best_grid;
   return SF_OK;
 }
 

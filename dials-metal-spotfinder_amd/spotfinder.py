@@ -101,6 +101,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_plan_destroy", None, _p)
     fn("sf_hip_plan_set_form", _i, _p, _i)
     fn("sf_hip_plan_set_grid", _i, _p, _i)
+    fn("sf_hip_plan_autotune", _i, _p, ctypes.POINTER(_p), _i,
+       ctypes.POINTER(_i), ctypes.POINTER(_i))
     fn("sf_hip_plan_run", _i, _p, ctypes.POINTER(_p))
     fn("sf_hip_plan_bind", _i, _p, ctypes.POINTER(_p))
     fn("sf_hip_plan_capture", _i, _p, ctypes.POINTER(_p))
@@ -343,6 +345,15 @@ class Plan:
 
     def set_grid(self, blocks: int) -> None:
         _check(lib().sf_hip_plan_set_grid(self.handle, blocks), "set_grid")
+
+    def autotune(self, frames, trials: int = 3):
+        """Time the candidate band heights / grids on these frames and keep
+        the fastest; returns (band_rows, blocks). Capture again afterwards."""
+        band, blocks = _i(0), _i(0)
+        _check(lib().sf_hip_plan_autotune(self.handle, self._frames(frames),
+                                          trials, ctypes.byref(band),
+                                          ctypes.byref(blocks)), "autotune")
+        return band.value, blocks.value
 
     def set_timing(self, on: bool) -> None:
         _check(lib().sf_hip_plan_set_timing(self.handle, 1 if on else 0),

@@ -142,6 +142,15 @@ int sf_hip_plan_set_form(sf_plan *plan, int form);
 /* Persistent grid: 0 = one block per work item, >0 = that many blocks
  * striding over the items. */
 int sf_hip_plan_set_grid(sf_plan *plan, int blocks);
+/* The autotuner: times the candidate launch shapes of the plan's kernel on
+ * these frames (each `trials` times, after a warm-up run) and keeps the
+ * fastest. For the sweep kernel the candidates are band heights (the tile's
+ * rows: 44, 108, 236, 492) crossed with grids (0 and 4 / 8 / 16 resident
+ * blocks a CU); for the probe kernel, whose 64 x 104 tile is fixed, the
+ * grids alone. Reports the choice through band_rows / blocks (either may be
+ * NULL). Drops a captured graph: capture again afterwards. */
+int sf_hip_plan_autotune(sf_plan *plan, const void *const *device_frames,
+                         int trials, int *band_rows, int *blocks);
 int sf_hip_plan_run(sf_plan *plan, const void *const *device_frames);
 /* Point the plan at these device frames without launching (the descriptor
  * upload that run() does first). A captured graph reads the descriptors at

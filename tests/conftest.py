@@ -17,7 +17,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def sf():
-    """The product binding; on a GPU box the library must load and a device
+    """The product binding; on a machine with a GPU the library must load and a device
     must be present (no fallback)."""
     import spotfinder
 

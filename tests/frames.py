@@ -6,11 +6,12 @@
 * `nxmx_fixture()` -- the 12-frame, 256 x 384 series of
   tests/make_test_nxmx.py:45-98 (a deterministic numpy formula), whose
   published answer is 8,928 signal pixels and 48 spots (README.md:95-107).
-* `planted_series()` -- config 0: Poisson(1) background with 48 Gaussian
+* `planted_series()` -- the 12-frame CPU workload: Poisson(1) background
+  with 48 Gaussian
   reflections spanning frames (sigma_xy 1.0-1.5, sigma_z 0.7, 500-5000
   counts), seeded.
-* `diffraction_spots()` -- a spot list for the device generator (configs
-  1-4's value model): ~spots_per_frame spots a frame, sigma 0.8-2 px,
+* `diffraction_spots()` -- a spot list for the device generator (the value
+  model of the device workloads): ~spots_per_frame spots a frame, sigma 0.8-2 px,
   spanning 1-5 frames, peak heights log-uniform up to `peak`.
 """
 
@@ -66,7 +67,7 @@ def nxmx_fixture(frames: int = 12) -> np.ndarray:
 
 def planted_series(seed: int = 2024, frames: int = 12, height: int = 256,
                    width: int = 384, reflections: int = 48):
-    """Config 0: returns (stack, planted list of (x, y, z))."""
+    """The 12-frame CPU workload: returns (stack, planted list of (x, y, z))."""
     rng = np.random.default_rng(seed)
     expected = np.ones((frames, height, width), dtype=np.float64)
     rows = np.arange(height)[:, None] + 0.5

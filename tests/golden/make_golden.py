@@ -68,8 +68,8 @@ def labeller_fixtures() -> dict:
         ("nxmx_2d", nx, list(range(12)), {"two_d": True}),
         ("nxmx_holey", np.delete(nx, 5, axis=0),
          [z for z in range(12) if z != 5], {}),
-        ("config0", series, list(range(12)), {}),
-        ("config0_nosep", series, list(range(12)),
+        ("planted", series, list(range(12)), {}),
+        ("planted_nosep", series, list(range(12)),
          {"max_separation": 0.0, "min_spot_size": 1}),
     ]
     for k, (name, stack, numbers, options) in enumerate(cases):

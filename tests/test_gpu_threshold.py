@@ -2,7 +2,7 @@
 the reference's golden vectors (tests/test_dext_gpu.cc's bar: no tolerance).
 
 Sizes stay where the CPU oracle finishes in seconds; full-size frames are
-covered by the config-1-model test (two frames, oracle ~1 s each).
+covered by the full-size test (two frames, oracle ~1 s each).
 """
 
 from __future__ import annotations
@@ -176,7 +176,7 @@ def test_overflow_returns_minus_two(sf):
     sf.stage2_window(1)
 
 
-def config1_frames(sf, n_frames, height, width, seed, pixel_bytes=2,
+def model_frames(sf, n_frames, height, width, seed, pixel_bytes=2,
                    background=0.3, spots_per_frame=1500.0):
     buf = sf.DeviceBuffer(pixel_bytes * height * width * n_frames)
     spots = fr.diffraction_spots(seed, height, width, n_frames,
@@ -191,11 +191,11 @@ def config1_frames(sf, n_frames, height, width, seed, pixel_bytes=2,
     return buf, host
 
 
-def test_full_size_config1_frames_fused_and_plain(sf):
-    """Two 4362 x 4148 frames of the config-1 value model (gaps, hot pixels,
+def test_full_size_model_frames_fused_and_plain(sf):
+    """Two 4362 x 4148 frames of the 1800-frame series value model (gaps, hot pixels,
     20,000-count peaks that wrap the 32-bit sum of squares)."""
     h, w = 4362, 4148
-    buf, host = config1_frames(sf, 2, h, w, seed=101)
+    buf, host = model_frames(sf, 2, h, w, seed=101)
     expected = [cpu.dext(host[f], capacity=h * w // 8)[1] for f in range(2)]
     assert all(len(e) > 1000 for e in expected)
     for form in (0, 1, 2):
@@ -210,13 +210,13 @@ def test_full_size_config1_frames_fused_and_plain(sf):
 
 
 def test_batched_uneven_sizes_graph_and_persistent(sf):
-    """Config 2's shape mix in one launch (scaled to keep the oracle quick),
+    """The uneven-chunk shape mix (2162x2068, 3269x3110, 4362x4148) in one launch (scaled to keep the oracle quick),
     replayed from a captured graph with eager work between replays, and
     through a persistent grid."""
     shapes = [(216, 206), (327, 311), (436, 414), (216, 206), (101, 3)]
     bufs, hosts = [], []
     for k, (h, w) in enumerate(shapes):
-        b, host = config1_frames(sf, 1, h, w, seed=200 + k,
+        b, host = model_frames(sf, 1, h, w, seed=200 + k,
                                  spots_per_frame=30000.0)
         bufs.append(b)
         hosts.append(host[0])

@@ -478,9 +478,9 @@ def test_labeller_matches_reference_golden_tables():
     nx = fr.nxmx_fixture()
     series, _ = fr.planted_series()
     stacks = {"nxmx": nx, "nxmx_2d": nx, "nxmx_holey": np.delete(nx, 5, axis=0),
-              "config0": series, "config0_nosep": series}
+              "planted": series, "planted_nosep": series}
     options = {"nxmx": {}, "nxmx_2d": {"two_d": True}, "nxmx_holey": {},
-               "config0": {}, "config0_nosep": {"max_separation": 0.0,
+               "planted": {}, "planted_nosep": {"max_separation": 0.0,
                                                 "min_spot_size": 1}}
     for k in range(int(data["n_labeller"])):
         name = str(data[f"l{k}_name"])
