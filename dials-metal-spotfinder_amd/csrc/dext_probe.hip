@@ -364,20 +364,31 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // No unmasked pixel of a positive cell: E is 0 at every unmasked pixel, so
   // nothing in the tile can be signal (about half the tiles of a diffraction
   // frame). Record empty rows and stop.
-  if (!__syncthreads_or(any_need)) {
+  const auto no_signal = [&] {
     const int rows = min(G::TH, height - r0);
     for (int r = tid; r < rows; r += 256)
       fd.cell_count[(size_t)(r0 + r) * fd.n_strips + tile_x] = 0;
     __syncthreads();
+  };
+  if (!__syncthreads_or(any_need)) {
+    no_signal();
     return;
   }
 
   // -- refine, round one: mask0 at those pixels ------------------------------
   refine<G>(sh, sh.need, 5, G::TH + 15);
+  int any_m0 = 0;
   for (int k = tid; k < NW; k += 256)
-    if (sh.m0[k])
+    if (sh.m0[k]) {
       atomicOr(&sh.rowany[k / G::WORDS], 1u);
-  __syncthreads();
+      any_m0 = 1;
+    }
+  // Every refined pixel negative: E(x) needs mask0(x) = 1 at the unmasked x
+  // itself, so E is 0 on the tile (typical beside module gaps).
+  if (!__syncthreads_or(any_m0)) {
+    no_signal();
+    return;
+  }
 
   // -- round two: the unmasked neighbours (5x5) of positive ones ------------
   // mask0 region: rows [3, TH+17).
@@ -454,6 +465,10 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // Row-major ranks of the candidates, and rounds of whole rows each at most
   // LIST candidates.
   const uint32_t total = rank_bits<G>(sh, sh.work, 10, 10 + G::TH);
+  if (total == 0) { // uniform: no candidate in the tile
+    no_signal();
+    return;
+  }
   if (tid == 0) {
     int n = 0;
     sh.rounds[n++] = 0;
