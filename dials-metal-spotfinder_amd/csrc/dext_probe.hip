@@ -462,6 +462,22 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   }
   __syncthreads();
 
+  // The 11x11 sums leave out masked and eroded-set pixels. Masked ones are
+  // already 0 in LDS; zero the eroded-set ones too (candidates' own values
+  // are read from the frame), so a window sum is a plain sum.
+  {
+    uint16_t *px16 = reinterpret_cast<uint16_t *>(sh.px);
+    for (int k = tid; k < NW; k += 256) {
+      uint32_t bits = sh.ebits[k];
+      const int lr = k / G::WORDS, j = k % G::WORDS;
+      while (bits) {
+        const int b = __builtin_ctz(bits);
+        bits &= bits - 1;
+        px16[lr * 2 * G::PITCH + 32 * j + b] = 0;
+      }
+    }
+  }
+
   // Row-major ranks of the candidates, and rounds of whole rows each at most
   // LIST candidates.
   const uint32_t total = rank_bits<G>(sh, sh.work, 10, 10 + G::TH);
@@ -502,60 +518,51 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     // lane g summing rows g and g + 8 of the window, then added across the
     // eight (integer sums, so the order is immaterial)
     const uint32_t g = tid & 7u;
+    typedef __attribute__((address_space(1))) const uint16_t *global_u16;
+    const uint16_t *pix = static_cast<const uint16_t *>(fd.pixels);
     for (uint32_t at0 = 0; at0 < count; at0 += 32) {
       const uint32_t i = at0 + (tid >> 3);
       const bool active = i < count;
-      uint32_t excluded = 0, s_all = 0, s_e = 0;
-      int lr = 0, lc = 0;
+      uint32_t n = 0, s2 = 0, p = 0;
       if (active) {
-        lr = 10 + (int)(sh.list[i] >> 16);
-        lc = (int)(sh.list[i] & 0xFFFFu);
+        const int lr = 10 + (int)(sh.list[i] >> 16);
+        const int lc = (int)(sh.list[i] & 0xFFFFu);
+        if (g == 0) // the candidate's own value: eroded-set, so 0 in LDS
+          p = *(global_u16)(pix + (size_t)(r0 + lr - 10) * (size_t)width +
+                            (size_t)(c0 + lc - 12));
         const int a = lc - 5;
         const int w = a >> 1;
         const uint32_t drop = (a & 1) ? 0xFFFF0000u : 0xFFFFFFFFu;
         const uint32_t keep = (a & 1) ? 0xFFFFFFFFu : 0x0000FFFFu;
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-          const int row = (int)g + 8 * k;
+        for (int t = 0; t < 2; t++) {
+          const int row = (int)g + 8 * t;
           if (row < 11) {
             const int r = lr - 5 + row;
-            const uint32_t e = row_bits(sh.ebits + r * G::WORDS, a, 11);
-            excluded +=
-                __builtin_popcount(row_bits(sh.mbits + r * G::WORDS, a, 11) | e);
-            const uint32_t *p = sh.px + r * G::PITCH + w;
-            s_all = udot2(p[0] & drop, 0x00010001u, s_all);
-            s_all = udot2(p[1], 0x00010001u, s_all);
-            s_all = udot2(p[2], 0x00010001u, s_all);
-            s_all = udot2(p[3], 0x00010001u, s_all);
-            s_all = udot2(p[4], 0x00010001u, s_all);
-            s_all = udot2(p[5] & keep, 0x00010001u, s_all);
-            uint32_t eb = e;
-            while (eb) {
-              const int b = __builtin_ctz(eb);
-              eb &= eb - 1;
-              const int c = a + b;
-              const uint32_t pair = sh.px[r * G::PITCH + (c >> 1)];
-              s_e += (c & 1) ? pair >> 16 : pair & 0xFFFFu;
-            }
+            n += 11u - __builtin_popcount(
+                           row_bits(sh.mbits + r * G::WORDS, a, 11) |
+                           row_bits(sh.ebits + r * G::WORDS, a, 11));
+            const uint32_t *q = sh.px + r * G::PITCH + w;
+            s2 = udot2(q[0] & drop, 0x00010001u, s2);
+            s2 = udot2(q[1], 0x00010001u, s2);
+            s2 = udot2(q[2], 0x00010001u, s2);
+            s2 = udot2(q[3], 0x00010001u, s2);
+            s2 = udot2(q[4], 0x00010001u, s2);
+            s2 = udot2(q[5] & keep, 0x00010001u, s2);
           }
         }
       }
 #pragma unroll
       for (int d = 1; d < 8; d <<= 1) {
-        excluded += __shfl_xor(excluded, d, 64);
-        s_all += __shfl_xor(s_all, d, 64);
-        s_e += __shfl_xor(s_e, d, 64);
+        n += __shfl_xor(n, d, 64);
+        s2 += __shfl_xor(s2, d, 64);
       }
       if (active && g == 0) {
-        const uint32_t n = 121u - excluded;
-        const uint32_t s2 = s_all - s_e;
-        const uint32_t pair = sh.px[lr * G::PITCH + (lc >> 1)];
-        const uint32_t p = (lc & 1) ? pair >> 16 : pair & 0xFFFFu;
         const float mean = n >= 2 ? (float)s2 / (float)n : 0.0f;
         const bool ok = (float)p >= mean + 3.0f * sqrtf(mean);
         sh.rank[i] = ok ? 1u : 0u;
         sh.bg[i] = mean;
-        sh.pop[i] = n;
+        sh.pop[i] = n | (p << 16);
       }
     }
     __syncthreads();
@@ -594,13 +601,12 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
       const uint32_t dst = base + at;
       at++;
       if (dst < fd.capacity) {
-        const uint32_t pair = sh.px[lr * G::PITCH + (lc >> 1)];
         sf_signal_pixel px;
         px.index = (uint32_t)(r0 + lr - 10) * (uint32_t)width +
                    (uint32_t)(c0 + lc - 12);
-        px.value = (lc & 1) ? pair >> 16 : pair & 0xFFFFu;
+        px.value = sh.pop[e] >> 16;
         px.background = sh.bg[e];
-        px.population = (uint16_t)sh.pop[e];
+        px.population = (uint16_t)(sh.pop[e] & 0xFFFFu);
         px.reserved = 0;
         fd.staging[dst] = px;
       }
