@@ -154,7 +154,6 @@ template <class G> struct ProbeShared {
   uint32_t rank[G::LIST];
   uint32_t rowok[G::TH];
   uint32_t rowbase[G::TH];
-  uint32_t rowany[G::ROWS];
   uint32_t rounds[G::TH + 2];
   uint32_t scalars[8];
   uint32_t tmp[8];
@@ -254,8 +253,6 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     sh.mbits[NW] = 0;
   for (int r = tid; r < G::TH; r += 256)
     sh.rowok[r] = 0;
-  for (int r = tid; r < G::ROWS; r += 256)
-    sh.rowany[r] = 0;
 
   // -- stage the tile: values with masked pixels zeroed, masked bitmap -----
   // A thread keeps one 4-pixel column group and steps down 8 rows at a time;
@@ -379,10 +376,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   refine<G>(sh, sh.need, 5, G::TH + 15);
   int any_m0 = 0;
   for (int k = tid; k < NW; k += 256)
-    if (sh.m0[k]) {
-      atomicOr(&sh.rowany[k / G::WORDS], 1u);
-      any_m0 = 1;
-    }
+    any_m0 |= sh.m0[k] != 0;
   // Every refined pixel negative: E(x) needs mask0(x) = 1 at the unmasked x
   // itself, so E is 0 on the tile (typical beside module gaps).
   if (!__syncthreads_or(any_m0)) {
@@ -391,55 +385,64 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   }
 
   // -- round two: the unmasked neighbours (5x5) of positive ones ------------
-  // mask0 region: rows [3, TH+17).
+  // Separable dilation of mask0: 5 columns wide into ebits (scratch until the
+  // erosion), then 5 rows tall over the mask0 region, rows [3, TH+17).
   for (int k = tid; k < NW; k += 256) {
     const int lr = k / G::WORDS, j = k % G::WORDS;
-    if (lr < 3 || lr >= G::TH + 17)
-      continue;
-    if (!(sh.rowany[lr - 2] | sh.rowany[lr - 1] | sh.rowany[lr] |
-          sh.rowany[lr + 1] | sh.rowany[lr + 2]))
-      continue;
-    uint32_t grown = 0;
-    for (int r = lr - 2; r <= lr + 2; r++) {
-      const uint32_t *row = sh.m0 + r * G::WORDS;
-      grown |= moved<G::WORDS>(row, j, -2) | moved<G::WORDS>(row, j, -1) |
-               row[j] | moved<G::WORDS>(row, j, 1) |
-               moved<G::WORDS>(row, j, 2);
+    uint32_t h = 0;
+    if (lr >= 1 && lr < G::TH + 19) {
+      const uint32_t *row = sh.m0 + lr * G::WORDS;
+      h = moved<G::WORDS>(row, j, -2) | moved<G::WORDS>(row, j, -1) | row[j] |
+          moved<G::WORDS>(row, j, 1) | moved<G::WORDS>(row, j, 2);
     }
-    sh.work[k] = grown & ~sh.mbits[k] & ~sh.need[k];
+    sh.ebits[k] = h;
+  }
+  __syncthreads();
+  for (int k = tid; k < NW; k += 256) {
+    const int lr = k / G::WORDS;
+    uint32_t grown = 0;
+    if (lr >= 3 && lr < G::TH + 17) {
+#pragma unroll
+      for (int r = -2; r <= 2; r++)
+        grown |= sh.ebits[k + r * G::WORDS];
+      grown &= ~sh.mbits[k] & ~sh.need[k];
+    }
+    sh.work[k] = grown;
   }
   __syncthreads();
   refine<G>(sh, sh.work, 3, G::TH + 17);
 
   // -- E: 5x5 erosion of (mask0 or masked), kept at the refined pixels ------
-  // Only words that hold refined pixels, spread over the block.
-  {
-    // list the words themselves, as one bit each
-    for (int k = tid; k < NW; k += 256)
-      sh.work[k] = sh.need[k] ? 1u : 0u;
-    __syncthreads();
-    const uint32_t words = rank_bits<G>(sh, sh.work, 0, G::ROWS);
-    for (uint32_t first = 0; first < words; first += G::LIST) {
-      const uint32_t count = min((uint32_t)G::LIST, words - first);
-      list_bits<G>(sh, sh.work, 0, G::ROWS, first, count);
-      for (uint32_t i = tid; i < count; i += 256) {
-        // a listed "bit" is bit 0 of word (lr, j): column 32 j
-        const int lr = (int)(sh.list[i] >> 16);
-        const int j = (int)(sh.list[i] & 0xFFFFu) >> 5;
-        uint32_t all = 0xFFFFFFFFu;
-        for (int r = lr - 2; r <= lr + 2; r++) {
-          uint32_t acc = 0xFFFFFFFFu;
+  // Separable: 5 columns wide into work over rows [3, TH+17), then 5 rows
+  // tall over the E region, rows [5, TH+15). Off-frame pixels are masked, so
+  // they count as set, the reference's erode edge (src/dext.cc:117-142).
+  for (int k = tid; k < NW; k += 256) {
+    const int lr = k / G::WORDS, j = k % G::WORDS;
+    uint32_t h = 0;
+    if (lr >= 3 && lr < G::TH + 17) {
+      const uint32_t *m0 = sh.m0 + lr * G::WORDS;
+      const uint32_t *mb = sh.mbits + lr * G::WORDS;
+      h = 0xFFFFFFFFu;
 #pragma unroll
-          for (int d = -2; d <= 2; d++)
-            acc &= moved<G::WORDS>(sh.m0 + r * G::WORDS, j, d) |
-                   moved<G::WORDS>(sh.mbits + r * G::WORDS, j, d);
-          all &= acc;
-        }
-        sh.ebits[lr * G::WORDS + j] = all & sh.need[lr * G::WORDS + j];
-      }
-      __syncthreads();
+      for (int d = -2; d <= 2; d++)
+        h &= moved<G::WORDS>(m0, j, d) | moved<G::WORDS>(mb, j, d);
     }
+    sh.work[k] = h;
   }
+  __syncthreads();
+  for (int k = tid; k < NW; k += 256) {
+    const int lr = k / G::WORDS;
+    uint32_t e = 0;
+    const uint32_t need = sh.need[k];
+    if (need && lr >= 5 && lr < G::TH + 15) {
+      e = need;
+#pragma unroll
+      for (int r = -2; r <= 2; r++)
+        e &= sh.work[k + r * G::WORDS];
+    }
+    sh.ebits[k] = e;
+  }
+  __syncthreads();
 
   // -- candidates: E, unmasked, p > 0, inside the output tile ---------------
   // output: rows [10, TH+10), columns [12, TW+12).
@@ -468,7 +471,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   {
     uint16_t *px16 = reinterpret_cast<uint16_t *>(sh.px);
     for (int k = tid; k < NW; k += 256) {
-      uint32_t bits = sh.ebits[k];
+      uint32_t bits = sh.ebits[k]; // set only in the E region
       const int lr = k / G::WORDS, j = k % G::WORDS;
       while (bits) {
         const int b = __builtin_ctz(bits);
