@@ -241,12 +241,9 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   constexpr int NW = G::ROWS * G::WORDS;
 
   // -- clear the bitmaps ----------------------------------------------------
-  for (int k = tid; k < NW + 1; k += 256) {
-    sh.need[k] = 0;
+  // (need, work and ebits are written whole by the phases that make them)
+  for (int k = tid; k < NW + 1; k += 256)
     sh.m0[k] = 0;
-    sh.work[k] = 0;
-    sh.ebits[k] = 0;
-  }
   for (int k = tid; k < G::NCR * G::WORDS; k += 256)
     sh.cellmask[k] = 0;
   if (tid == 0)
@@ -347,8 +344,10 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   int any_need = 0;
   for (int k = tid; k < NW; k += 256) {
     const int lr = k / G::WORDS, j = k % G::WORDS;
-    if (lr < 5 || lr >= G::TH + 15)
+    if (lr < 5 || lr >= G::TH + 15) {
+      sh.need[k] = 0;
       continue;
+    }
     const int ci = floor_div5(r0 - 10 + lr);
     const int lo = max(7 - 32 * j, 0), hi = min(G::TW + 17 - 32 * j, 32);
     const uint32_t cols =
