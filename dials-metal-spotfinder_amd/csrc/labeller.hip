@@ -436,15 +436,6 @@ __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
                      });
 }
 
-__device__ __forceinline__ double wave_read(double v, uint32_t lane) {
-  const uint64_t bits = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits,
-                                                          (int)lane);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(
-      (int)(uint32_t)(bits >> 32), (int)lane);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1)
@@ -483,6 +474,7 @@ __global__ __launch_bounds__(256) void finalise(
     const uint32_t *n_closed, const uint32_t *order,
     const int32_t *node_frame, const uint32_t *node_index,
     const uint32_t *node_value, SpotOut *out, unsigned long long *counts) {
+  __shared__ double terms[4][3 * 64]; // a wave's variance terms, by axis
   const uint32_t comps = *n_starts;
   const uint32_t end_all = *n_closed;
   const uint32_t lane = threadIdx.x & 63;
@@ -620,10 +612,21 @@ __global__ __launch_bounds__(256) void finalise(
               t[axis] = value * delta * delta;
             }
           }
+          // terms through LDS, then every lane adds them in member order
+          // (the same sums in every lane; the reads are broadcasts)
+          double *mine = terms[threadIdx.x >> 6];
+          for (int axis = 0; axis < 3; axis++)
+            mine[axis * 64 + lane] = t[axis];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
           const uint32_t m = min(64u, n - k0);
           for (uint32_t i = 0; i < m; i++)
             for (int axis = 0; axis < 3; axis++)
-              delta_sq[axis] += wave_read(t[axis], i);
+              delta_sq[axis] += mine[axis * 64 + i];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         };
         kind = finish_spot(
             a, spot, sum, sum_sq, weighted, peak,
