@@ -956,9 +956,12 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   l->bound_spots += (n - l->n_carry) / std::max<uint64_t>(1, a.min_size) + 1;
   {
     unsigned long long totals[2] = {0, 0};
-    LAB_CHECK(hipMemcpy(totals, l->d_totals, sizeof(totals),
-                        hipMemcpyDeviceToHost),
+    // on the labeller's own stream (idle here), so that no null-stream copy
+    // waits for other streams' work
+    LAB_CHECK(hipMemcpyAsync(totals, l->d_totals, sizeof(totals),
+                             hipMemcpyDeviceToHost, s),
               "reading totals");
+    LAB_CHECK(hipStreamSynchronize(s), "reading totals");
     LAB_CHECK(l->spots.grow_keep(totals[0] + n / std::max<uint64_t>(
                                                   1, a.min_size) + 1,
                                  totals[0]),
@@ -1282,6 +1285,7 @@ int labeller_add_lists(sf_labeller *l, int n, const int64_t *numbers,
 }
 
 hipStream_t labeller_stream(sf_labeller *l) { return l ? l->stream : nullptr; }
+
 } // namespace sf
 
 extern "C" {
