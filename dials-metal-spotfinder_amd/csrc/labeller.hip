@@ -184,15 +184,15 @@ __global__ void fill_rows(LabelArgs a, const ChunkDesc *frames,
 // node of its run (consecutive indices on one row of one frame), which is
 // lower, so roots still point down.
 __global__ void link_runs(LabelArgs a, const ChunkDesc *frames,
-                          const uint32_t *node_index, uint32_t *parent,
-                          uint32_t *run) {
+                          const uint32_t *node_index, const uint32_t *slot,
+                          uint32_t *parent, uint32_t *run) {
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
        g += gridDim.x * blockDim.x) {
     if (g < a.n_carry) { // carried nodes already point at their roots
       run[g] = g;
       continue;
     }
-    const int k = chunk_frame_of(frames, a.n_frames, g);
+    const int k = (int)slot[g] - 1; // chunk frame, from the gather
     const uint32_t from = frames[k].offset;
     uint32_t r = g, idx = node_index[g];
     while (r > from && idx % a.width != 0 && node_index[r - 1] == idx - 1) {
@@ -209,12 +209,12 @@ __global__ void link_runs(LabelArgs a, const ChunkDesc *frames,
 // targets are then neighbours on their own row and already joined, as are
 // the node and its left neighbour.
 __global__ void unite_edges(LabelArgs a, const ChunkDesc *frames,
-                            const uint32_t *node_index,
+                            const uint32_t *node_index, const uint32_t *slot,
                             const uint32_t *row_start, uint32_t *parent) {
   const uint32_t stride = a.pixels / a.width + 1;
   for (uint32_t g = a.n_carry + blockIdx.x * blockDim.x + threadIdx.x;
        g < a.n_nodes; g += gridDim.x * blockDim.x) {
-    const int k = chunk_frame_of(frames, a.n_frames, g);
+    const int k = (int)slot[g] - 1; // chunk frame, from the gather
     const uint32_t from = frames[k].offset;
     const uint32_t idx = node_index[g];
     const uint32_t y = idx / a.width;
@@ -1080,9 +1080,11 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   }
   // `values` holds each node's run start until make_keys reuses it
   sf::link_runs<<<grid, 256, 0, s>>>(a, l->chunk.ptr, l->node_index.ptr,
-                                     l->parent.ptr, l->values.ptr);
+                                     l->node_slot.ptr, l->parent.ptr,
+                                     l->values.ptr);
   sf::unite_edges<<<grid, 256, 0, s>>>(a, l->chunk.ptr, l->node_index.ptr,
-                                       l->row_start.ptr, l->parent.ptr);
+                                       l->node_slot.ptr, l->row_start.ptr,
+                                       l->parent.ptr);
   sf::resolve_roots<<<grid, 256, 0, s>>>(a, l->parent.ptr, l->values.ptr,
                                          l->node_slot.ptr, l->root.ptr,
                                          l->last_slot.ptr);
