@@ -210,9 +210,10 @@ def test_full_size_model_frames_fused_and_plain(sf):
 
 
 def test_batched_uneven_sizes_graph_and_persistent(sf):
-    """The uneven-chunk shape mix (2162x2068, 3269x3110, 4362x4148) in one launch (scaled to keep the oracle quick),
-    replayed from a captured graph with eager work between replays, and
-    through a persistent grid."""
+    """The uneven-chunk shape mix (2162x2068, 3269x3110, 4362x4148, scaled
+    to keep the oracle quick) in one launch, replayed from a captured graph
+    with eager work between replays, through a persistent grid, and after
+    the autotuner has picked its launch shape (probe and sweep kernels)."""
     shapes = [(216, 206), (327, 311), (436, 414), (216, 206), (101, 3)]
     bufs, hosts = [], []
     for k, (h, w) in enumerate(shapes):
@@ -252,6 +253,17 @@ def test_batched_uneven_sizes_graph_and_persistent(sf):
     counts = plan.counts()
     for f in range(len(shapes)):
         assert same(plan.output(f, int(counts[f])), expected[f]), ("grid", f)
+    for form in (0, 2):
+        plan.set_form(form)
+        band, blocks = plan.autotune(ptrs, 1)
+        assert blocks >= 0 and band > 0
+        plan.capture(ptrs)
+        plan.replay()
+        plan.wait()
+        counts = plan.counts()
+        for f in range(len(shapes)):
+            assert same(plan.output(f, int(counts[f])), expected[f]), (
+                "autotuned", form, band, blocks, f)
     plan.close()
 
 
