@@ -28,9 +28,11 @@ hipError_t launch_order(const FrameDesc *, int, uint32_t, hipStream_t);
 hipError_t launch_plain(int, const FrameDesc *, int, uint32_t, uint32_t,
                         uint8_t *, uint8_t *, hipStream_t, hipEvent_t,
                         hipEvent_t);
-hipError_t launch_probe(const FrameDesc *, int, uint32_t, int, hipStream_t);
-int probe_tile_width();
-int probe_tile_height();
+hipError_t launch_probe(int, const FrameDesc *, int, uint32_t, int,
+                        hipStream_t);
+int probe_shapes();
+int probe_tile_width(int shape);
+int probe_tile_height(int shape);
 } // namespace sf
 
 namespace {
@@ -120,6 +122,7 @@ struct sf_plan {
   int form = 0;   // 0 fused (probe at 16 bits, sweep at 32), 1 plain,
                   // 2 sweep
   int blocks = 0; // 0: one block per item
+  int tile = 0;   // probe kernel tile shape (sf::probe_tile_width/height)
   hipStream_t stream = nullptr;
   bool timing = false;
 
@@ -164,7 +167,8 @@ uint32_t strips_for(const sf_plan *p, uint32_t width, int form) {
   if (k == kPlain)
     return (width + 63) / 64;
   if (k == kProbe)
-    return (width + sf::probe_tile_width() - 1) / sf::probe_tile_width();
+    return (width + sf::probe_tile_width(p->tile) - 1) /
+           sf::probe_tile_width(p->tile);
   const uint32_t ow = p->pixel_bytes == 2 ? sf::strip_width<uint16_t>()
                                           : sf::strip_width<uint32_t>();
   return (width + ow - 1) / ow;
@@ -212,7 +216,7 @@ void layout(sf_plan *p) {
     d.width = p->widths[f];
     d.n_strips = strips_for(p, d.width, p->form);
     d.band_rows = kernel_for(p, p->form) == kProbe
-                      ? (uint32_t)sf::probe_tile_height()
+                      ? (uint32_t)sf::probe_tile_height(p->tile)
                       : (uint32_t)p->band_rows;
     d.n_bands = (d.height + d.band_rows - 1) / d.band_rows;
     d.capacity = p->capacity[f];
@@ -269,7 +273,7 @@ int enqueue(sf_plan *p, bool timing) {
     SF_CHECK(hipEventRecord(p->ev[0], p->stream), "hipEventRecord");
   const Kernel kernel = kernel_for(p, p->form);
   if (kernel == kProbe) {
-    SF_CHECK(sf::launch_probe(p->d_desc, p->n, p->items, p->blocks,
+    SF_CHECK(sf::launch_probe(p->tile, p->d_desc, p->n, p->items, p->blocks,
                               p->stream),
              "launching the probe threshold");
     if (timing) {
@@ -340,7 +344,7 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
     cells += (size_t)heights[f] * ((widths[f] + 63) / 64);
     // enough band totals for the shortest band any form uses
     const uint32_t shortest = (uint32_t)std::min(p->band_rows,
-                                                 sf::probe_tile_height());
+                                                 sf::probe_tile_height(0));
     bands += (heights[f] + shortest - 1) / shortest;
     p->max_pixels = px > p->max_pixels ? px : p->max_pixels;
   }
@@ -428,11 +432,15 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
         hipGetDeviceProperties(&prop, device) == hipSuccess)
       cus = prop.multiProcessorCount;
   }
-  std::vector<int> bands;
+  // (band rows, tile shape) candidates: band heights for the sweep kernel,
+  // tile shapes for the probe kernel
+  std::vector<std::pair<int, int>> shapes;
   if (kernel == kSweep)
-    bands = {44, 108, 236, 492};
+    for (const int band : {44, 108, 236, 492})
+      shapes.emplace_back(band, p->tile);
   else
-    bands = {p->band_rows};
+    for (int tile = 0; tile < sf::probe_shapes(); tile++)
+      shapes.emplace_back(p->band_rows, tile);
   const int grids[] = {0, 4 * cus, 8 * cus, 16 * cus};
   // band totals for the shortest candidate band
   size_t need_bands = 0;
@@ -458,9 +466,11 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
   }
   const bool was_timing = p->timing;
   double best = 1e30;
-  int best_band = p->band_rows, best_grid = p->blocks;
-  for (const int band : bands) {
+  int best_band = p->band_rows, best_grid = p->blocks, best_tile = p->tile;
+  for (const auto &shape : shapes) {
+    const int band = shape.first;
     p->band_rows = band;
+    p->tile = shape.second;
     layout(p);
     p->bound.clear(); // force a descriptor upload
     for (const int grid : grids) {
@@ -484,22 +494,46 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
         best = median;
         best_band = band;
         best_grid = grid;
+        best_tile = shape.second;
       }
     }
   }
   p->timing = was_timing;
   p->band_rows = best_band;
   p->blocks = best_grid;
+  p->tile = best_tile;
   layout(p);
   p->bound.clear();
   SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
   p->upload_pending = false;
-  if (band_rows)
-    *band_rows = best_band;
+  if (band_rows) // the rows a tile covers: the band, or the probe tile
+    *band_rows = kernel == kProbe ? sf::probe_tile_height(best_tile)
+                                  : best_band;
   if (blocks)
     *blocks = best_grid;
   return SF_OK;
 }
+
+int sf_hip_plan_set_tile(sf_plan *p, int shape) {
+  if (!p || shape < 0 || shape >= sf::probe_shapes())
+    return bad("sf_hip_plan_set_tile: bad argument");
+  SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+  if (p->exec) {
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  p->graph_form = -1;
+  p->tile = shape;
+  layout(p);
+  p->bound.clear();
+  return SF_OK;
+}
+
+int sf_hip_plan_tile(sf_plan *p) { return p ? p->tile : -1; }
 
 int sf_hip_plan_set_timing(sf_plan *p, int on) {
   if (!p)
@@ -900,9 +934,9 @@ void fused_layout(int pixel_bytes, uint32_t height, uint32_t width,
   d.height = height;
   d.width = width;
   if (pixel_bytes == 2) {
-    const uint32_t tw = (uint32_t)probe_tile_width();
+    const uint32_t tw = (uint32_t)probe_tile_width(0);
     d.n_strips = (width + tw - 1) / tw;
-    d.band_rows = (uint32_t)probe_tile_height();
+    d.band_rows = (uint32_t)probe_tile_height(0);
   } else {
     // Short bands: a frame or two per launch still fills the chip (at 16
     // frames 108-row bands are 5% faster, at one frame 44-row bands 35%).
@@ -928,7 +962,7 @@ hipError_t launch_batch(int pixel_bytes, bool any_vec, bool any_scalar,
   if (e != hipSuccess)
     return e;
   e = pixel_bytes == 2
-          ? launch_probe(d, n, items, 0, stream)
+          ? launch_probe(0, d, n, items, 0, stream)
           : launch_fused(pixel_bytes, any_vec, any_scalar, d, n, items, 0,
                          stream);
   if (e != hipSuccess)

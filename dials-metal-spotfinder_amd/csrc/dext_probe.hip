@@ -19,7 +19,7 @@
 // at candidates (E, unmasked, p > 0). On diffraction frames about 1.6% of
 // cells need refinement and 0.4% of pixels are candidates.
 //
-// One 256-thread block per output tile of TH x TW pixels. The tile plus a
+// One block of NT threads per output tile of TH x TW pixels. The tile plus a
 // 10-row / 12-column halo is staged in LDS as u16 values with masked pixels
 // zeroed, plus a bitmap of masked (and off-frame) pixels. Window sums use
 // v_dot2_u32_u16 (two pixels' values and squares per instruction, wrapping
@@ -41,8 +41,9 @@ __device__ __forceinline__ int floor_div5(int v) {
   return v >= 0 ? v / 5 : -((-v + 4) / 5);
 }
 
-template <int TH_, int TW_> struct Probe {
+template <int TH_, int TW_, int NT_> struct Probe {
   static constexpr int TH = TH_, TW = TW_;
+  static constexpr int NT = NT_;            // threads a block
   static constexpr int ROWS = TH + 20;      // global rows r0-10 .. r0+TH+9
   static constexpr int COLS = TW + 24;      // global cols c0-12 .. c0+TW+11
   static constexpr int WORDS = COLS / 32;   // bitmap words per row
@@ -55,6 +56,8 @@ template <int TH_, int TW_> struct Probe {
   static constexpr int OUTWORDS = TH * WORDS;
   static_assert(COLS % 64 == 0, "rows load as whole waves of 4-pixel groups");
   static_assert(LIST >= TW, "a round holds at least one full row");
+  static_assert(NT % 64 == 0 && NT <= 1024 && LIST % NT == 0,
+                "whole waves; the list spreads evenly over the block");
 };
 
 // Bits [a, a + len) of a bitmap row (len <= 25), as the low bits of a word.
@@ -114,9 +117,10 @@ __device__ __forceinline__ bool dispersion(const uint32_t *px32,
   return lhs > rhs;
 }
 
-// Exclusive scan over the block (256 threads); *total = the block's sum.
-__device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *total,
-                                            uint32_t *tmp) {
+// Exclusive scan over the block (NT threads); *total = the block's sum.
+template <int NT>
+__device__ __forceinline__ uint32_t scan_block(uint32_t v, uint32_t *total,
+                                               uint32_t *tmp) {
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   uint32_t x = v;
 #pragma unroll
@@ -130,7 +134,7 @@ __device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *total,
   __syncthreads();
   uint32_t before = 0, all = 0;
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < NT / 64; k++) {
     before += (uint32_t)k < wave ? tmp[k] : 0u;
     all += tmp[k];
   }
@@ -156,7 +160,7 @@ template <class G> struct ProbeShared {
   uint32_t rowbase[G::TH];
   uint32_t rounds[G::TH + 2];
   uint32_t scalars[8];
-  uint32_t tmp[8];
+  uint32_t tmp[16];
   float lut[64];
 };
 
@@ -165,7 +169,7 @@ template <class G> struct ProbeShared {
 template <class G>
 __device__ uint32_t rank_bits(ProbeShared<G> &sh, const uint32_t *bits,
                               int ra, int rb) {
-  constexpr int PER = (G::ROWS * G::WORDS + 255) / 256;
+  constexpr int PER = (G::ROWS * G::WORDS + G::NT - 1) / G::NT;
   const int k0 = ra * G::WORDS, n = (rb - ra) * G::WORDS;
   uint32_t mine[PER];
   uint32_t sum = 0;
@@ -176,7 +180,7 @@ __device__ uint32_t rank_bits(ProbeShared<G> &sh, const uint32_t *bits,
     sum += mine[i];
   }
   uint32_t total = 0;
-  uint32_t at = scan256(sum, &total, sh.tmp);
+  uint32_t at = scan_block<G::NT>(sum, &total, sh.tmp);
 #pragma unroll
   for (int i = 0; i < PER; i++) {
     const int k = (int)threadIdx.x * PER + i;
@@ -193,7 +197,7 @@ template <class G>
 __device__ void list_bits(ProbeShared<G> &sh, const uint32_t *bits, int ra,
                           int rb, uint32_t first, uint32_t count) {
   const int n = (rb - ra) * G::WORDS;
-  for (int k = threadIdx.x; k < n; k += 256) {
+  for (int k = threadIdx.x; k < n; k += G::NT) {
     const uint32_t off = sh.word_off[k];
     uint32_t b = bits[ra * G::WORDS + k];
     if (!b || off >= first + count ||
@@ -221,7 +225,7 @@ __device__ void refine(ProbeShared<G> &sh, const uint32_t *bits, int ra,
   for (uint32_t first = 0; first < total; first += G::LIST) {
     const uint32_t count = min((uint32_t)G::LIST, total - first);
     list_bits<G>(sh, bits, ra, rb, first, count);
-    for (uint32_t i = threadIdx.x; i < count; i += 256) {
+    for (uint32_t i = threadIdx.x; i < count; i += G::NT) {
       const int lr = (int)(sh.list[i] >> 16), lc = (int)(sh.list[i] & 0xFFFFu);
       if (dispersion<G>(sh.px, sh.mbits, lr, lc, sh.lut))
         atomicOr(&sh.m0[lr * G::WORDS + (lc >> 5)], 1u << (lc & 31));
@@ -242,13 +246,13 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
 
   // -- clear the bitmaps ----------------------------------------------------
   // (need, work and ebits are written whole by the phases that make them)
-  for (int k = tid; k < NW + 1; k += 256)
+  for (int k = tid; k < NW + 1; k += G::NT)
     sh.m0[k] = 0;
-  for (int k = tid; k < G::NCR * G::WORDS; k += 256)
+  for (int k = tid; k < G::NCR * G::WORDS; k += G::NT)
     sh.cellmask[k] = 0;
   if (tid == 0)
     sh.mbits[NW] = 0;
-  for (int r = tid; r < G::TH; r += 256)
+  for (int r = tid; r < G::TH; r += G::NT)
     sh.rowok[r] = 0;
 
   // -- stage the tile: values with masked pixels zeroed, masked bitmap -----
@@ -256,7 +260,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // every load of the tile is issued before any is used.
   {
     constexpr int GROUPS = G::COLS / 4;         // 32 groups a row
-    constexpr int STEP = 256 / GROUPS;          // rows per iteration
+    constexpr int STEP = G::NT / GROUPS;        // rows per iteration
     constexpr int ITER = (G::ROWS + STEP - 1) / STEP;
     const int grp = (int)tid % GROUPS, sub = (int)tid / GROUPS;
     const int c = c0 - 12 + 4 * grp;
@@ -320,7 +324,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   const int ci0 = floor_div5(r0 - 5), ci1 = floor_div5(r0 + G::TH + 4);
   const int cj0 = floor_div5(c0 - 5), cj1 = floor_div5(c0 + G::TW + 4);
   const int ncr = ci1 - ci0 + 1, ncc = cj1 - cj0 + 1;
-  for (int k = tid; k < ncr * ncc; k += 256) {
+  for (int k = tid; k < ncr * ncc; k += G::NT) {
     const int ci = ci0 + k / ncc, cj = cj0 + k % ncc;
     const int plr = 5 * ci + 2 - r0 + 10, plc = 5 * cj + 2 - c0 + 12;
     const bool masked =
@@ -342,7 +346,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // -- the unmasked pixels of positive cells, inside the E region ----------
   // E region: rows [5, TH+15), columns [7, TW+17) of the tile's LDS frame.
   int any_need = 0;
-  for (int k = tid; k < NW; k += 256) {
+  for (int k = tid; k < NW; k += G::NT) {
     const int lr = k / G::WORDS, j = k % G::WORDS;
     if (lr < 5 || lr >= G::TH + 15) {
       sh.need[k] = 0;
@@ -362,7 +366,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // frame). Record empty rows and stop.
   const auto no_signal = [&] {
     const int rows = min(G::TH, height - r0);
-    for (int r = tid; r < rows; r += 256)
+    for (int r = tid; r < rows; r += G::NT)
       fd.cell_count[(size_t)(r0 + r) * fd.n_strips + tile_x] = 0;
     __syncthreads();
   };
@@ -374,7 +378,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // -- refine, round one: mask0 at those pixels ------------------------------
   refine<G>(sh, sh.need, 5, G::TH + 15);
   int any_m0 = 0;
-  for (int k = tid; k < NW; k += 256)
+  for (int k = tid; k < NW; k += G::NT)
     any_m0 |= sh.m0[k] != 0;
   // Every refined pixel negative: E(x) needs mask0(x) = 1 at the unmasked x
   // itself, so E is 0 on the tile (typical beside module gaps).
@@ -386,7 +390,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // -- round two: the unmasked neighbours (5x5) of positive ones ------------
   // Separable dilation of mask0: 5 columns wide into ebits (scratch until the
   // erosion), then 5 rows tall over the mask0 region, rows [3, TH+17).
-  for (int k = tid; k < NW; k += 256) {
+  for (int k = tid; k < NW; k += G::NT) {
     const int lr = k / G::WORDS, j = k % G::WORDS;
     uint32_t h = 0;
     if (lr >= 1 && lr < G::TH + 19) {
@@ -397,7 +401,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     sh.ebits[k] = h;
   }
   __syncthreads();
-  for (int k = tid; k < NW; k += 256) {
+  for (int k = tid; k < NW; k += G::NT) {
     const int lr = k / G::WORDS;
     uint32_t grown = 0;
     if (lr >= 3 && lr < G::TH + 17) {
@@ -415,7 +419,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // Separable: 5 columns wide into work over rows [3, TH+17), then 5 rows
   // tall over the E region, rows [5, TH+15). Off-frame pixels are masked, so
   // they count as set, the reference's erode edge (src/dext.cc:117-142).
-  for (int k = tid; k < NW; k += 256) {
+  for (int k = tid; k < NW; k += G::NT) {
     const int lr = k / G::WORDS, j = k % G::WORDS;
     uint32_t h = 0;
     if (lr >= 3 && lr < G::TH + 17) {
@@ -429,7 +433,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     sh.work[k] = h;
   }
   __syncthreads();
-  for (int k = tid; k < NW; k += 256) {
+  for (int k = tid; k < NW; k += G::NT) {
     const int lr = k / G::WORDS;
     uint32_t e = 0;
     const uint32_t need = sh.need[k];
@@ -445,7 +449,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
 
   // -- candidates: E, unmasked, p > 0, inside the output tile ---------------
   // output: rows [10, TH+10), columns [12, TW+12).
-  for (int k = tid; k < G::OUTWORDS; k += 256) {
+  for (int k = tid; k < G::OUTWORDS; k += G::NT) {
     const int lr = 10 + k / G::WORDS, j = k % G::WORDS;
     uint32_t bits = sh.ebits[lr * G::WORDS + j];
     uint32_t cand = 0;
@@ -469,7 +473,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   // are read from the frame), so a window sum is a plain sum.
   {
     uint16_t *px16 = reinterpret_cast<uint16_t *>(sh.px);
-    for (int k = tid; k < NW; k += 256) {
+    for (int k = tid; k < NW; k += G::NT) {
       uint32_t bits = sh.ebits[k]; // set only in the E region
       const int lr = k / G::WORDS, j = k % G::WORDS;
       while (bits) {
@@ -522,7 +526,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     const uint32_t g = tid & 7u;
     typedef __attribute__((address_space(1))) const uint16_t *global_u16;
     const uint16_t *pix = static_cast<const uint16_t *>(fd.pixels);
-    for (uint32_t at0 = 0; at0 < count; at0 += 32) {
+    for (uint32_t at0 = 0; at0 < count; at0 += G::NT / 8) {
       const uint32_t i = at0 + (tid >> 3);
       const bool active = i < count;
       uint32_t n = 0, s2 = 0, p = 0;
@@ -569,7 +573,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     }
     __syncthreads();
     // survivors in list order
-    constexpr int LPER = G::LIST / 256;
+    constexpr int LPER = G::LIST / G::NT;
     uint32_t flags[LPER];
     uint32_t sum = 0;
 #pragma unroll
@@ -579,7 +583,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
       sum += flags[i];
     }
     uint32_t survivors = 0;
-    uint32_t at = scan256(sum, &survivors, sh.tmp);
+    uint32_t at = scan_block<G::NT>(sum, &survivors, sh.tmp);
     if (tid == 0)
       sh.scalars[2] = survivors ? atomicAdd(fd.counter, survivors) : 0u;
     // each entry's exclusive survivor rank, for the row offsets below
@@ -614,7 +618,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
       }
     }
     // a row's survivors start at the survivor rank of its first candidate
-    for (int r = ra + (int)tid; r < rb; r += 256) {
+    for (int r = ra + (int)tid; r < rb; r += G::NT) {
       const uint32_t fi = sh.word_off[r * G::WORDS] - first;
       if (fi < count)
         sh.rowbase[r] = base + sh.rank[fi];
@@ -623,7 +627,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
       sh.scalars[1] += survivors;
     __syncthreads();
   }
-  for (int r = tid; r < out_rows; r += 256) {
+  for (int r = tid; r < out_rows; r += G::NT) {
     const size_t c = (size_t)(r0 + r) * fd.n_strips + tile_x;
     fd.cell_count[c] = (uint8_t)sh.rowok[r];
     if (sh.rowok[r])
@@ -635,9 +639,8 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
 }
 
 template <class G>
-__global__ __launch_bounds__(256, 4) void probe_kernel(const FrameDesc *frames,
-                                                    int n_frames,
-                                                    uint32_t items) {
+__global__ __launch_bounds__(G::NT, 4) void probe_kernel(
+    const FrameDesc *frames, int n_frames, uint32_t items) {
   __shared__ ProbeShared<G> sh;
   if (threadIdx.x < 64)
     sh.lut[threadIdx.x] = threadIdx.x < 2
@@ -659,18 +662,37 @@ __global__ __launch_bounds__(256, 4) void probe_kernel(const FrameDesc *frames,
   }
 }
 
-using ProbeDefault = Probe<64, 104>;
+// The tile shapes the autotuner chooses from: 64 x 104 with 256 threads
+// (four blocks a CU), and 128 x 104 or 64 x 232 with 512 threads (two
+// blocks a CU, about 12% less halo to stage).
+using ProbeShape0 = Probe<64, 104, 256>;
+using ProbeShape1 = Probe<128, 104, 512>;
+using ProbeShape2 = Probe<64, 232, 512>;
 
-int probe_tile_width() { return ProbeDefault::TW; }
-int probe_tile_height() { return ProbeDefault::TH; }
+int probe_shapes() { return 3; }
+int probe_tile_width(int shape) {
+  return shape == 1 ? ProbeShape1::TW
+                    : shape == 2 ? ProbeShape2::TW : ProbeShape0::TW;
+}
+int probe_tile_height(int shape) {
+  return shape == 1 ? ProbeShape1::TH
+                    : shape == 2 ? ProbeShape2::TH : ProbeShape0::TH;
+}
 
-hipError_t launch_probe(const FrameDesc *frames, int n_frames, uint32_t items,
-                        int blocks, hipStream_t stream) {
+hipError_t launch_probe(int shape, const FrameDesc *frames, int n_frames,
+                        uint32_t items, int blocks, hipStream_t stream) {
   const uint32_t padded = (items + 7) & ~7u;
   const uint32_t asked = ((uint32_t)blocks + 7) & ~7u; // whole XCD rounds
   const uint32_t grid = blocks > 0 ? (asked < padded ? asked : padded) : padded;
-  probe_kernel<ProbeDefault><<<grid, 256, 0, stream>>>(frames, n_frames,
-                                                       items);
+  if (shape == 1)
+    probe_kernel<ProbeShape1><<<grid, ProbeShape1::NT, 0, stream>>>(
+        frames, n_frames, items);
+  else if (shape == 2)
+    probe_kernel<ProbeShape2><<<grid, ProbeShape2::NT, 0, stream>>>(
+        frames, n_frames, items);
+  else
+    probe_kernel<ProbeShape0><<<grid, ProbeShape0::NT, 0, stream>>>(
+        frames, n_frames, items);
   return hipGetLastError();
 }
 

@@ -103,6 +103,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_plan_set_grid", _i, _p, _i)
     fn("sf_hip_plan_autotune", _i, _p, ctypes.POINTER(_p), _i,
        ctypes.POINTER(_i), ctypes.POINTER(_i))
+    fn("sf_hip_plan_set_tile", _i, _p, _i)
+    fn("sf_hip_plan_tile", _i, _p)
     fn("sf_hip_plan_run", _i, _p, ctypes.POINTER(_p))
     fn("sf_hip_plan_bind", _i, _p, ctypes.POINTER(_p))
     fn("sf_hip_plan_capture", _i, _p, ctypes.POINTER(_p))
@@ -354,6 +356,13 @@ class Plan:
                                           trials, ctypes.byref(band),
                                           ctypes.byref(blocks)), "autotune")
         return band.value, blocks.value
+
+    def set_tile(self, shape: int) -> None:
+        """Probe kernel tile: 0 = 64x104, 1 = 128x104, 2 = 64x232."""
+        _check(lib().sf_hip_plan_set_tile(self.handle, shape), "set_tile")
+
+    def tile(self) -> int:
+        return int(lib().sf_hip_plan_tile(self.handle))
 
     def set_timing(self, on: bool) -> None:
         _check(lib().sf_hip_plan_set_timing(self.handle, 1 if on else 0),

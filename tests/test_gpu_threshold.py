@@ -105,12 +105,13 @@ def test_random_frames_edges_and_both_paths(sf):
             assert same(got, expected), (shape, dtype, describe(got, expected))
 
 
-@pytest.mark.parametrize("form", [0, 2])
-def test_random_frames_device_resident(sf, form):
+@pytest.mark.parametrize("form,tile", [(0, 0), (0, 1), (0, 2), (2, 0)])
+def test_random_frames_device_resident(sf, form, tile):
     """The same edge cases through device-resident plans: the probe kernel
-    (form 0 at 16 bits) and the register-sweep kernel (form 2), including
-    widths that are not a multiple of 4 and an unaligned frame start."""
-    rng = np.random.default_rng(40 + form)
+    (form 0 at 16 bits) in each of its three tile shapes, and the
+    register-sweep kernel (form 2), including widths that are not a multiple
+    of 4 and an unaligned frame start."""
+    rng = np.random.default_rng(40 + form + 7 * tile)
     shapes = [(1, 1), (2, 3), (9, 1), (5, 5), (7, 300), (300, 7), (33, 65),
               (64, 64), (129, 257), (250, 232), (250, 233), (251, 464),
               (97, 105), (128, 104), (65, 209)]
@@ -124,6 +125,7 @@ def test_random_frames_device_resident(sf, form):
                 buf = sf.DeviceBuffer(frame.nbytes + 16)
                 buf.upload(frame, offset=offset)
                 plan = sf.Plan(pb, [shape], form=form)
+                plan.set_tile(tile)
                 plan.run([buf.ptr + offset])
                 plan.wait()
                 n = int(plan.counts()[0])
@@ -198,14 +200,16 @@ def test_full_size_model_frames_fused_and_plain(sf):
     buf, host = model_frames(sf, 2, h, w, seed=101)
     expected = [cpu.dext(host[f], capacity=h * w // 8)[1] for f in range(2)]
     assert all(len(e) > 1000 for e in expected)
-    for form in (0, 1, 2):
+    for form, tile in ((0, 0), (0, 1), (0, 2), (1, 0), (2, 0)):
         plan = sf.Plan(2, [(h, w)] * 2, form=form)
+        plan.set_tile(tile)
         plan.run([buf.ptr, buf.ptr + 2 * h * w])
         plan.wait()
         counts = plan.counts()
         for f in range(2):
             got = plan.output(f, int(counts[f]))
-            assert same(got, expected[f]), (form, f, describe(got, expected[f]))
+            assert same(got, expected[f]), (form, tile, f,
+                                             describe(got, expected[f]))
         plan.close()
 
 
