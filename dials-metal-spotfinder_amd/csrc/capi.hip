@@ -342,9 +342,12 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
     p->capacity.push_back((uint32_t)(px * (size_t)pixel_bytes / 16));
     staging += p->capacity.back();
     cells += (size_t)heights[f] * ((widths[f] + 63) / 64);
-    // enough band totals for the shortest band any form uses
-    const uint32_t shortest = (uint32_t)std::min(p->band_rows,
-                                                 sf::probe_tile_height(0));
+    // enough band totals for the shortest band any form or tile uses
+    int shortest_tile = sf::probe_tile_height(0);
+    for (int t = 1; t < sf::probe_shapes(); t++)
+      shortest_tile = std::min(shortest_tile, sf::probe_tile_height(t));
+    const uint32_t shortest =
+        (uint32_t)std::min(p->band_rows, shortest_tile);
     bands += (heights[f] + shortest - 1) / shortest;
     p->max_pixels = px > p->max_pixels ? px : p->max_pixels;
   }

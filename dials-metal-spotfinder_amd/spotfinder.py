@@ -358,7 +358,8 @@ class Plan:
         return band.value, blocks.value
 
     def set_tile(self, shape: int) -> None:
-        """Probe kernel tile: 0 = 64x104, 1 = 128x104, 2 = 64x232."""
+        """Probe kernel tile: 0 = 64x104, 1 = 128x104, 2 = 64x232,
+        3 = 32x104."""
         _check(lib().sf_hip_plan_set_tile(self.handle, shape), "set_tile")
 
     def tile(self) -> int:

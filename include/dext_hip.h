@@ -146,14 +146,15 @@ int sf_hip_plan_set_grid(sf_plan *plan, int blocks);
  * these frames (each `trials` times, after a warm-up run) and keeps the
  * fastest. For the sweep kernel the candidates are band heights (the tile's
  * rows: 44, 108, 236, 492) crossed with grids (0 and 4 / 8 / 16 resident
- * blocks a CU); for the probe kernel, its three tile shapes crossed with the
+ * blocks a CU); for the probe kernel, its four tile shapes crossed with the
  * same grids. Reports the chosen tile rows and grid through band_rows /
  * blocks (either may be NULL). Drops a captured graph: capture again. */
 int sf_hip_plan_autotune(sf_plan *plan, const void *const *device_frames,
                          int trials, int *band_rows, int *blocks);
 /* The probe kernel's tile shape: 0 = 64 x 104 (256 threads, the default),
- * 1 = 128 x 104 and 2 = 64 x 232 (512 threads each). The autotuner tries all
- * three; sf_hip_plan_tile reports the current one. Drops a captured graph. */
+ * 1 = 128 x 104 and 2 = 64 x 232 (512 threads each), 3 = 32 x 104 (128
+ * threads). The autotuner tries all of them; sf_hip_plan_tile reports the
+ * current one. Drops a captured graph. */
 int sf_hip_plan_set_tile(sf_plan *plan, int shape);
 int sf_hip_plan_tile(sf_plan *plan);
 int sf_hip_plan_run(sf_plan *plan, const void *const *device_frames);

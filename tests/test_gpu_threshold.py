@@ -105,7 +105,7 @@ def test_random_frames_edges_and_both_paths(sf):
             assert same(got, expected), (shape, dtype, describe(got, expected))
 
 
-@pytest.mark.parametrize("form,tile", [(0, 0), (0, 1), (0, 2), (2, 0)])
+@pytest.mark.parametrize("form,tile", [(0, 0), (0, 1), (0, 2), (0, 3), (2, 0)])
 def test_random_frames_device_resident(sf, form, tile):
     """The same edge cases through device-resident plans: the probe kernel
     (form 0 at 16 bits) in each of its three tile shapes, and the
@@ -200,7 +200,7 @@ def test_full_size_model_frames_fused_and_plain(sf):
     buf, host = model_frames(sf, 2, h, w, seed=101)
     expected = [cpu.dext(host[f], capacity=h * w // 8)[1] for f in range(2)]
     assert all(len(e) > 1000 for e in expected)
-    for form, tile in ((0, 0), (0, 1), (0, 2), (1, 0), (2, 0)):
+    for form, tile in ((0, 0), (0, 1), (0, 2), (0, 3), (1, 0), (2, 0)):
         plan = sf.Plan(2, [(h, w)] * 2, form=form)
         plan.set_tile(tile)
         plan.run([buf.ptr, buf.ptr + 2 * h * w])
