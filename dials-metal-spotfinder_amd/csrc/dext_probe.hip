@@ -139,7 +139,8 @@ __device__ __forceinline__ uint32_t scan_block(uint32_t v, uint32_t *total,
     all += tmp[k];
   }
   *total = all;
-  __syncthreads();
+  // No closing barrier: every caller passes another one before it scans
+  // again, so `tmp` is not overwritten while a wave still reads it.
   return before + x - v;
 }
 
@@ -448,34 +449,29 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
   __syncthreads();
 
   // -- candidates: E, unmasked, p > 0, inside the output tile ---------------
-  // output: rows [10, TH+10), columns [12, TW+12).
-  for (int k = tid; k < G::OUTWORDS; k += G::NT) {
-    const int lr = 10 + k / G::WORDS, j = k % G::WORDS;
-    uint32_t bits = sh.ebits[lr * G::WORDS + j];
-    uint32_t cand = 0;
-    while (bits) {
-      const int b = __builtin_ctz(bits);
-      bits &= bits - 1;
-      const int lc = 32 * j + b;
-      if (lc < 12 || lc >= G::TW + 12)
-        continue;
-      const uint32_t pair = sh.px[lr * G::PITCH + (lc >> 1)];
-      const uint32_t p = (lc & 1) ? pair >> 16 : pair & 0xFFFFu;
-      if (p)
-        cand |= 1u << b;
-    }
-    sh.work[lr * G::WORDS + j] = cand;
-  }
-  __syncthreads();
-
-  // The 11x11 sums leave out masked and eroded-set pixels. Masked ones are
-  // already 0 in LDS; zero the eroded-set ones too (candidates' own values
-  // are read from the frame), so a window sum is a plain sum.
+  // output: rows [10, TH+10), columns [12, TW+12). The same pass zeroes the
+  // eroded-set pixels in LDS: the 11x11 sums leave out masked and
+  // eroded-set pixels, masked ones are already 0, so each window sum becomes
+  // a plain sum (candidates' own values are read from the frame). A word's
+  // candidates are tested before its pixels are zeroed, by the same thread.
   {
     uint16_t *px16 = reinterpret_cast<uint16_t *>(sh.px);
     for (int k = tid; k < NW; k += G::NT) {
-      uint32_t bits = sh.ebits[k]; // set only in the E region
       const int lr = k / G::WORDS, j = k % G::WORDS;
+      uint32_t bits = sh.ebits[k]; // set only in the E region
+      if (lr >= 10 && lr < G::TH + 10) {
+        uint32_t cand = 0, x = bits;
+        while (x) {
+          const int b = __builtin_ctz(x);
+          x &= x - 1;
+          const int lc = 32 * j + b;
+          if (lc < 12 || lc >= G::TW + 12)
+            continue;
+          if (px16[lr * 2 * G::PITCH + lc])
+            cand |= 1u << b;
+        }
+        sh.work[k] = cand;
+      }
       while (bits) {
         const int b = __builtin_ctz(bits);
         bits &= bits - 1;
@@ -483,6 +479,7 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
       }
     }
   }
+  __syncthreads();
 
   // Row-major ranks of the candidates, and rounds of whole rows each at most
   // LIST candidates.
