@@ -257,40 +257,62 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
     sh.rowok[r] = 0;
 
   // -- stage the tile: values with masked pixels zeroed, masked bitmap -----
-  // A thread keeps one 4-pixel column group and steps down 8 rows at a time;
-  // every load of the tile is issued before any is used.
+  // A thread keeps one 4-pixel column group and steps down STEP rows at a
+  // time; every load of the tile is issued before any is used.
   {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) const u32x2 *global_u2;
+    typedef __attribute__((address_space(1))) const uint16_t *global_u16;
     constexpr int GROUPS = G::COLS / 4;         // 32 groups a row
     constexpr int STEP = G::NT / GROUPS;        // rows per iteration
     constexpr int ITER = (G::ROWS + STEP - 1) / STEP;
     const int grp = (int)tid % GROUPS, sub = (int)tid / GROUPS;
     const int c = c0 - 12 + 4 * grp;
-    const bool vec = fd.vec != 0 && c >= 0 && c + 4 <= width;
     uint32_t raw[ITER][2];
+    const bool interior = fd.vec != 0 && r0 >= 10 &&
+                          r0 + G::TH + 10 <= height && c0 >= 12 &&
+                          c0 + G::TW + 12 <= width;
+    if (interior) {
+      // every staged row in the frame, every group 8-byte aligned
+      const uint16_t *at = pix + ((size_t)(r0 - 10 + sub) * (size_t)width + c);
+      const size_t stride = (size_t)STEP * (size_t)width;
 #pragma unroll
-    for (int it = 0; it < ITER; it++) {
-      const int lr = it * STEP + sub;
-      const int y = r0 - 10 + lr;
-      raw[it][0] = 0xFFFFFFFFu;
-      raw[it][1] = 0xFFFFFFFFu;
-      if (lr < G::ROWS && y >= 0 && y < height) {
-        const uint16_t *row = pix + (size_t)y * (size_t)width;
-        if (vec) {
-          const uint2 d = *reinterpret_cast<const uint2 *>(row + c);
+      for (int it = 0; it < ITER; it++) {
+        raw[it][0] = raw[it][1] = 0xFFFFFFFFu;
+        if (it * STEP + sub < G::ROWS) {
+          const u32x2 d = *(global_u2)(at + (size_t)it * stride);
           raw[it][0] = d.x;
           raw[it][1] = d.y;
-        } else {
-          uint32_t v[4] = {0xFFFFu, 0xFFFFu, 0xFFFFu, 0xFFFFu};
+        }
+      }
+    } else {
+      const bool vec = fd.vec != 0 && c >= 0 && c + 4 <= width;
 #pragma unroll
-          for (int i = 0; i < 4; i++)
-            if (c + i >= 0 && c + i < width)
-              v[i] = row[c + i];
-          raw[it][0] = v[0] | (v[1] << 16);
-          raw[it][1] = v[2] | (v[3] << 16);
+      for (int it = 0; it < ITER; it++) {
+        const int lr = it * STEP + sub;
+        const int y = r0 - 10 + lr;
+        raw[it][0] = 0xFFFFFFFFu;
+        raw[it][1] = 0xFFFFFFFFu;
+        if (lr < G::ROWS && y >= 0 && y < height) {
+          const uint16_t *row = pix + (size_t)y * (size_t)width;
+          if (vec) {
+            const u32x2 d = *(global_u2)(row + c);
+            raw[it][0] = d.x;
+            raw[it][1] = d.y;
+          } else {
+            uint32_t v[4] = {0xFFFFu, 0xFFFFu, 0xFFFFu, 0xFFFFu};
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              if (c + i >= 0 && c + i < width)
+                v[i] = *(global_u16)(row + c + i);
+            raw[it][0] = v[0] | (v[1] << 16);
+            raw[it][1] = v[2] | (v[3] << 16);
+          }
         }
       }
     }
-    const us2 two = {2, 2};
+    const us2 two = {2, 2}, one = {1, 1};
+    const uint32_t place = 4u * (uint32_t)(grp & 7); // nibble slot in a word
 #pragma unroll
     for (int it = 0; it < ITER; it++) {
       const int lr = it * STEP + sub;
@@ -299,22 +321,26 @@ __device__ void probe_tile(const FrameDesc &fd, uint32_t tile_x,
       const us2 b = __builtin_bit_cast(us2, raw[it][1]) + two;
       const us2 pa = __builtin_elementwise_sub_sat(a, two);
       const us2 pb = __builtin_elementwise_sub_sat(b, two);
-      const uint32_t nib = (a.x < 2 ? 1u : 0u) | (a.y < 2 ? 2u : 0u) |
-                           (b.x < 2 ? 4u : 0u) | (b.y < 2 ? 8u : 0u);
+      // masked flags 1 / 0 a pixel: min(2 - a, 1) saturating
+      const us2 ma = __builtin_elementwise_min(
+          __builtin_elementwise_sub_sat(two, a), one);
+      const us2 mb = __builtin_elementwise_min(
+          __builtin_elementwise_sub_sat(two, b), one);
+      // the four flags as bits 0..3 (weights 1, 2 and 4, 8)
+      const uint32_t nib =
+          udot2(__builtin_bit_cast(uint32_t, mb), 0x00080004u,
+                udot2(__builtin_bit_cast(uint32_t, ma), 0x00020001u, 0u));
       if (lr < G::ROWS) {
         uint2 out;
         out.x = __builtin_bit_cast(uint32_t, pa);
         out.y = __builtin_bit_cast(uint32_t, pb);
         *reinterpret_cast<uint2 *>(&sh.px[lr * G::PITCH + 2 * grp]) = out;
       }
-      // gather eight lanes' nibbles into one 32-bit word at the first lane
-      uint32_t x = nib;
-      x |= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x101, 0xF, 0xF, true)
-           << 4;
-      x |= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x102, 0xF, 0xF, true)
-           << 8;
-      x |= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x104, 0xF, 0xF, true)
-           << 16;
+      // or eight lanes' nibbles, each in its slot, into the first lane
+      uint32_t x = nib << place;
+      x |= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x101, 0xF, 0xF, true);
+      x |= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x102, 0xF, 0xF, true);
+      x |= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x104, 0xF, 0xF, true);
       if (lr < G::ROWS && (grp & 7) == 0)
         sh.mbits[lr * G::WORDS + (grp >> 3)] = x;
     }
