@@ -28,6 +28,14 @@ hipError_t launch_order(const FrameDesc *, int, uint32_t, hipStream_t);
 hipError_t launch_plain(int, const FrameDesc *, int, uint32_t, uint32_t,
                         uint8_t *, uint8_t *, hipStream_t, hipEvent_t,
                         hipEvent_t);
+void sparse_layout(uint32_t, uint32_t, FrameDesc &);
+size_t sparse_bytes(uint32_t, uint32_t, uint32_t);
+void sparse_carve(uint8_t *, FrameDesc &);
+hipError_t launch_sparse_threshold(const FrameDesc *, int, uint32_t,
+                                   hipStream_t);
+hipError_t launch_sparse_order(const FrameDesc *, int, uint32_t, hipStream_t);
+hipError_t launch_sparse(const FrameDesc *, int, uint32_t, uint32_t,
+                         hipStream_t);
 hipError_t launch_probe(int, const FrameDesc *, int, uint32_t, int,
                         hipStream_t);
 int probe_shapes();
@@ -119,8 +127,8 @@ struct sf_plan {
   std::vector<uint32_t> heights, widths;
   std::vector<uint32_t> capacity;
   int band_rows = 236;
-  int form = 0;   // 0 fused (probe at 16 bits, sweep at 32), 1 plain,
-                  // 2 sweep
+  int form = 0;   // 0 fused (sparse passes at 16 bits, sweep at 32),
+                  // 1 plain, 2 sweep, 3 probe tiles (at 16 bits)
   int blocks = 0; // 0: one block per item
   int tile = 0;   // probe kernel tile shape (sf::probe_tile_width/height)
   hipStream_t stream = nullptr;
@@ -132,8 +140,9 @@ struct sf_plan {
   sf_signal_pixel *d_staging = nullptr, *d_out = nullptr;
   uint8_t *d_cell_count = nullptr;
   uint32_t *d_cell_base = nullptr, *d_band_total = nullptr;
-  uint32_t max_bands = 0;
+  uint32_t max_bands = 0, max_cell_rows = 0;
   size_t band_capacity = 0; // entries of d_band_total
+  uint8_t *d_sparse = nullptr; // the sparse form's tables, every frame
   uint32_t *d_counters = nullptr, *d_found = nullptr;
   uint8_t *d_mask0 = nullptr, *d_mask1 = nullptr;
   size_t max_pixels = 0;
@@ -152,12 +161,14 @@ struct sf_plan {
 namespace {
 
 // The kernel a form runs for this plan's pixel type.
-enum Kernel { kSweep, kPlain, kProbe };
+enum Kernel { kSweep, kPlain, kProbe, kSparse };
 
 Kernel kernel_for(const sf_plan *p, int form) {
   if (form == 1)
     return kPlain;
   if (form == 0 && p->pixel_bytes == 2)
+    return kSparse;
+  if (form == 3 && p->pixel_bytes == 2)
     return kProbe;
   return kSweep;
 }
@@ -196,6 +207,7 @@ void plan_free(sf_plan *p) {
   (void)hipFree(p->d_found);
   (void)hipFree(p->d_mask0);
   (void)hipFree(p->d_mask1);
+  (void)hipFree(p->d_sparse);
   if (p->h_desc)
     (void)hipHostFree(p->h_desc);
   if (p->h_found)
@@ -207,19 +219,27 @@ void plan_free(sf_plan *p) {
 // Lay out descriptors for the current form (strip width differs by form).
 void layout(sf_plan *p) {
   uint32_t item = 0;
-  size_t staging = 0, cells = 0, bands = 0;
+  size_t staging = 0, cells = 0, bands = 0, sparse = 0;
   p->any_vec = p->any_scalar = false;
-  p->max_bands = 0;
+  p->max_bands = p->max_cell_rows = 0;
+  const Kernel kernel = kernel_for(p, p->form);
   for (int f = 0; f < p->n; f++) {
     sf::FrameDesc &d = p->desc[f];
     d.height = p->heights[f];
     d.width = p->widths[f];
-    d.n_strips = strips_for(p, d.width, p->form);
-    d.band_rows = kernel_for(p, p->form) == kProbe
-                      ? (uint32_t)sf::probe_tile_height(p->tile)
-                      : (uint32_t)p->band_rows;
-    d.n_bands = (d.height + d.band_rows - 1) / d.band_rows;
     d.capacity = p->capacity[f];
+    if (kernel == kSparse) {
+      sf::sparse_layout(d.height, d.width, d);
+      sf::sparse_carve(p->d_sparse + sparse, d);
+      sparse += sf::sparse_bytes(d.height, d.width, d.capacity);
+      p->max_cell_rows = std::max(p->max_cell_rows, d.n_cell_rows);
+    } else {
+      d.n_strips = strips_for(p, d.width, p->form);
+      d.band_rows = kernel == kProbe
+                        ? (uint32_t)sf::probe_tile_height(p->tile)
+                        : (uint32_t)p->band_rows;
+      d.n_bands = (d.height + d.band_rows - 1) / d.band_rows;
+    }
     d.item_begin = item;
     item += d.n_strips * d.n_bands;
     d.staging = p->d_staging + staging;
@@ -268,10 +288,26 @@ int upload_desc(sf_plan *p, const void *const *frames) {
 }
 
 int enqueue(sf_plan *p, bool timing) {
+  const Kernel kernel = kernel_for(p, p->form);
+  if (kernel == kSparse) { // clears its own counters and orders its lists
+    if (timing)
+      SF_CHECK(hipEventRecord(p->ev[0], p->stream), "hipEventRecord");
+    SF_CHECK(sf::launch_sparse_threshold(p->d_desc, p->n, p->items,
+                                         p->stream),
+             "launching the sparse threshold");
+    if (timing)
+      for (int e = 1; e <= 3; e++)
+        SF_CHECK(hipEventRecord(p->ev[e], p->stream), "hipEventRecord");
+    SF_CHECK(sf::launch_sparse_order(p->d_desc, p->n, p->max_cell_rows,
+                                     p->stream),
+             "launching the sparse ordering");
+    if (timing)
+      SF_CHECK(hipEventRecord(p->ev[4], p->stream), "hipEventRecord");
+    return SF_OK;
+  }
   SF_CHECK(sf::launch_zero(p->d_desc, p->n, p->stream), "zeroing counters");
   if (timing)
     SF_CHECK(hipEventRecord(p->ev[0], p->stream), "hipEventRecord");
-  const Kernel kernel = kernel_for(p, p->form);
   if (kernel == kProbe) {
     SF_CHECK(sf::launch_probe(p->tile, p->d_desc, p->n, p->items, p->blocks,
                               p->stream),
@@ -334,13 +370,15 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
   p->widths.assign(widths, widths + n_frames);
   if (band_rows > 0)
     p->band_rows = ((band_rows + 20 + 15) & ~15) - 20;
-  size_t staging = 0, cells = 0, bands = 0;
+  size_t staging = 0, cells = 0, bands = 0, sparse = 0;
   for (int f = 0; f < n_frames; f++) {
     const size_t px = (size_t)heights[f] * widths[f];
     // The list holds one entry per 16 bytes of frame, as the reference's
     // device buffer does (src/dext_cuda.cu:614-616).
     p->capacity.push_back((uint32_t)(px * (size_t)pixel_bytes / 16));
     staging += p->capacity.back();
+    if (pixel_bytes == 2)
+      sparse += sf::sparse_bytes(heights[f], widths[f], p->capacity.back());
     cells += (size_t)heights[f] * ((widths[f] + 63) / 64);
     // enough band totals for the shortest band any form or tile uses
     int shortest_tile = sf::probe_tile_height(0);
@@ -363,6 +401,8 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
                  hipSuccess;
   ok = ok && hipMalloc(&p->d_out, sizeof(sf_signal_pixel) *
                                       (staging ? staging : 1)) == hipSuccess;
+  if (sparse)
+    ok = ok && hipMalloc(&p->d_sparse, sparse) == hipSuccess;
   ok = ok && hipMalloc(&p->d_cell_count, cells) == hipSuccess;
   ok = ok && hipMalloc(&p->d_cell_base, sizeof(uint32_t) * cells) ==
                  hipSuccess;
@@ -397,7 +437,7 @@ void sf_hip_plan_destroy(sf_plan *plan) {
 }
 
 int sf_hip_plan_set_form(sf_plan *p, int form) {
-  if (!p || form < 0 || form > 2)
+  if (!p || form < 0 || form > 3)
     return bad("sf_hip_plan_set_form: bad argument");
   if (form == p->form)
     return SF_OK;
@@ -427,6 +467,13 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
   const Kernel kernel = kernel_for(p, p->form);
   if (kernel == kPlain)
     return bad("sf_hip_plan_autotune: the plain form has nothing to tune");
+  if (kernel == kSparse) { // its launch shape follows the frames
+    if (band_rows)
+      *band_rows = (int)p->desc[0].band_rows;
+    if (blocks)
+      *blocks = 0;
+    return SF_OK;
+  }
   int cus = 256;
   {
     int device = 0;
@@ -929,26 +976,25 @@ int plan_lists(sf_plan *p, std::vector<const sf_signal_pixel *> &lists,
 } // namespace sf
 
 namespace sf {
-// The fused forms' layout of one frame (form 0: the probe kernel at 16 bits,
-// the sweep at 32): strips, bands and the list capacity. Pointers, item
-// offsets and the vector flag are the caller's to fill.
+// The fused forms' layout of one frame (form 0: the sparse passes at 16
+// bits, the sweep at 32): cell lattice or strips and bands, and the list
+// capacity. Pointers (the sparse tables too), item offsets and the vector
+// flag are the caller's to fill.
 void fused_layout(int pixel_bytes, uint32_t height, uint32_t width,
                   FrameDesc &d) {
   d.height = height;
   d.width = width;
-  if (pixel_bytes == 2) {
-    const uint32_t tw = (uint32_t)probe_tile_width(0);
-    d.n_strips = (width + tw - 1) / tw;
-    d.band_rows = (uint32_t)probe_tile_height(0);
-  } else {
-    // Short bands: a frame or two per launch still fills the chip (at 16
-    // frames 108-row bands are 5% faster, at one frame 44-row bands 35%).
-    const uint32_t ow = strip_width<uint32_t>();
-    d.n_strips = (width + ow - 1) / ow;
-    d.band_rows = 44;
-  }
-  d.n_bands = (height + d.band_rows - 1) / d.band_rows;
   d.capacity = (uint32_t)((size_t)height * width * (size_t)pixel_bytes / 16);
+  if (pixel_bytes == 2) {
+    sparse_layout(height, width, d);
+    return;
+  }
+  // Short bands: a frame or two per launch still fills the chip (at 16
+  // frames 108-row bands are 5% faster, at one frame 44-row bands 35%).
+  const uint32_t ow = strip_width<uint32_t>();
+  d.n_strips = (width + ow - 1) / ow;
+  d.band_rows = 44;
+  d.n_bands = (height + d.band_rows - 1) / d.band_rows;
 }
 
 // Cells of the per-row survivor table a frame needs, whatever the form.
@@ -960,14 +1006,14 @@ size_t cell_entries(uint32_t height, uint32_t width) {
 // lay each list out in index order: one batch on `stream`.
 hipError_t launch_batch(int pixel_bytes, bool any_vec, bool any_scalar,
                         const FrameDesc *d, int n, uint32_t items,
-                        uint32_t max_bands, hipStream_t stream) {
+                        uint32_t max_bands, uint32_t max_cell_rows,
+                        hipStream_t stream) {
+  if (pixel_bytes == 2)
+    return launch_sparse(d, n, items, max_cell_rows, stream);
   hipError_t e = launch_zero(d, n, stream);
   if (e != hipSuccess)
     return e;
-  e = pixel_bytes == 2
-          ? launch_probe(0, d, n, items, 0, stream)
-          : launch_fused(pixel_bytes, any_vec, any_scalar, d, n, items, 0,
-                         stream);
+  e = launch_fused(pixel_bytes, any_vec, any_scalar, d, n, items, 0, stream);
   if (e != hipSuccess)
     return e;
   return launch_order(d, n, max_bands, stream);

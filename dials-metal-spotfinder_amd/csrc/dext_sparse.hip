@@ -1,0 +1,857 @@
+// The u16 threshold as four sparse passes: a dense scout that tests one probe
+// pixel per 5x5 cell, then the erosion, the Poisson test and the ordering,
+// each only where the scout found something.
+//
+// Reference semantics: src/dext.cc:8-225, exactly (the parity tests compare
+// every SignalPixel field with the reference's own output).
+//
+// Why probes suffice. The output depends on the first mask (mask0: masked or
+// dispersion-positive) only through its 5x5 erosion E (src/dext.cc:117-142):
+// E(x) = 1 iff mask0 = 1 on every in-frame pixel of the 5x5 window N5(x),
+// and E matters only at unmasked pixels (a masked pixel is excluded from the
+// 11x11 sums and is never a candidate whatever E says). Cut the frame into
+// 5x5 cells on a fixed global lattice; cell (i, j) is rows 5i..5i+4 and
+// columns 5j..5j+4 clipped to the frame, and its probe is (5i+2, 5j+2), or
+// the frame's last row or column where that is off the frame (a cell cut to
+// one or two rows or columns is still within two of it). Every N5(x) of the
+// cell contains the probe, so where the probe has mask0 = 0, E = 0 on the
+// whole cell. A cell is active when its probe is masked or
+// dispersion-positive and the cell has an unmasked pixel; nothing else can
+// hold a candidate or an eroded pixel.
+//
+// The passes, each a kernel over every frame of a launch:
+//
+//   scout   one wave per 64 cell columns x 16 cell rows: each lane sums its
+//           probe's 7x7 window from one 16-byte load per pixel row (shared
+//           rows carried between windows), tests it, and the wave appends
+//           its active cells to the frame's list and writes their bits.
+//   erode   sixteen lanes per active cell, four cells a wave: mask0 on the
+//           9x9 pixels around the cell by separable window sums (columns in
+//           lanes, rows in registers), then E by bit operations on ballots.
+//           Cells with candidates (E, unmasked, p > 0) are listed.
+//   poisson sixteen lanes per candidate cell: the 11x11 sums of pixels
+//           neither masked nor eroded (E read from the neighbouring cells'
+//           tables), the test at the candidates, survivors to staging.
+//   order   one wave per cell row: survivors laid out in index order.
+#include "sf_device.hh"
+
+namespace sf {
+
+namespace {
+
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef u32x4 u32x4_a4 __attribute__((aligned(4))); // 16 bytes, 4-aligned
+typedef __attribute__((address_space(1))) const u32x4_a4 *global_q4;
+typedef __attribute__((address_space(1))) const uint16_t *global_u16;
+
+__device__ __forceinline__ uint32_t udot2(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(us2, a),
+                                __builtin_bit_cast(us2, b), c, false);
+}
+
+constexpr int kBandCells = 16; // cell rows a scout item covers
+constexpr int kSegment = 64 * kBandCells; // list entries an item owns
+
+// sqrt(2 (n - 1)) for n = 2..49 as the reference rounds it (src/dext.cc:108);
+// n = 0, 1 give +inf so that "lhs > rhs" is false there (0 * inf is NaN).
+__device__ __forceinline__ void fill_lut(float *lut) {
+  if (threadIdx.x < 64)
+    lut[threadIdx.x] = threadIdx.x < 2 ? __builtin_inff()
+                                       : sqrtf(2.0f * (float)(threadIdx.x - 1));
+}
+
+// One pixel row of a lane's probe window. The lane holds eight columns from
+// an even column `ls`; m[k] keeps the window's columns of word k (0xFFFF a
+// half) and w5[k] weighs the cell's own columns. With p = v + 1 at unmasked
+// pixels and 0 at masked ones (v + 2 wraps the two sentinels to 0 and 1):
+// s += sum p, q += sum p^2, n7 += window count, n5 += cell count.
+__device__ __forceinline__ void scout_row(const uint32_t (&d)[4],
+                                          const uint32_t (&m)[4],
+                                          const uint32_t (&w5)[4],
+                                          uint32_t &s, uint32_t &q,
+                                          uint32_t &n7, uint32_t &n5) {
+  const us2 two = {2, 2}, one = {1, 1};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const us2 a = __builtin_bit_cast(us2, d[k]) + two;
+    const uint32_t p =
+        __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(a, one)) &
+        m[k];
+    const uint32_t f = __builtin_bit_cast(
+        uint32_t, __builtin_elementwise_min(__builtin_bit_cast(us2, p), one));
+    s = udot2(p, 0x00010001u, s);
+    q = udot2(p, p, q);
+    n7 = udot2(f, 0x00010001u, n7);
+    n5 = udot2(f, w5[k], n5);
+  }
+}
+
+// Scout one item: cell columns 64 * strip + lane, cell rows
+// [16 * band, 16 * band + 16). VEC: the frame's rows are 4-byte aligned and
+// at least 8 pixels wide, so every lane reads its 8 columns as one load.
+template <bool VEC>
+__device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
+                           const float *lut, uint64_t *masks) {
+  const int height = (int)fd.height, width = (int)fd.width;
+  const int ncr = (int)fd.n_cell_rows, ncc = (int)fd.n_cell_cols;
+  const int lane = (int)lane_id();
+  const int j = 64 * (int)strip + lane;
+  const bool live = j < ncc;
+  const int pc = min(5 * j + 2, width - 1);
+  // window columns [pc-3, pc+3] and the cell's columns, in the frame
+  const int wlo = max(pc - 3, 0), whi = min(pc + 3, width - 1);
+  const int clo = 5 * j, chi = min(5 * j + 4, width - 1);
+  int ls = (pc - 3) & ~1;
+  if (VEC)
+    ls = min(max(ls, 0), width - 8);
+  uint32_t m[4], w5[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    m[k] = 0;
+    w5[k] = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int c = ls + 2 * k + h;
+      if (live && c >= wlo && c <= whi)
+        m[k] |= 0xFFFFu << (16 * h);
+      if (live && c >= clo && c <= chi)
+        w5[k] |= 1u << (16 * h);
+    }
+  }
+  const int probe_at = pc - ls; // 0..7 when the probe is in the frame
+  const uint16_t *pix = static_cast<const uint16_t *>(fd.pixels);
+
+  // a pixel row's eight columns; rows off the frame read as masked
+  const auto load_row = [&](int y, uint32_t (&d)[4]) {
+    if (y < 0 || y >= height) { // uniform
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        d[k] = 0xFFFFFFFFu;
+      return;
+    }
+    const uint16_t *row = pix + (size_t)y * (size_t)width;
+    if (VEC) {
+      const u32x4 v = *(global_q4)(row + ls);
+      d[0] = v.x;
+      d[1] = v.y;
+      d[2] = v.z;
+      d[3] = v.w;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        uint32_t lo = 0xFFFFu, hi = 0xFFFFu;
+        if (m[k] & 0xFFFFu)
+          lo = *(global_u16)(row + ls + 2 * k);
+        if (m[k] >> 16)
+          hi = *(global_u16)(row + ls + 2 * k + 1);
+        d[k] = lo | (hi << 16);
+      }
+    }
+  };
+
+  const int i0 = (int)band * kBandCells, i1 = min(i0 + kBandCells, ncr);
+  // rows 5 i0 - 1 and 5 i0: the first window's top, carried like a tail
+  uint32_t ts = 0, tq = 0, tn = 0, top5 = 0, unused = 0;
+  {
+    uint32_t d[4];
+    load_row(5 * i0 - 1, d);
+    scout_row(d, m, w5, ts, tq, tn, unused);
+    load_row(5 * i0, d);
+    scout_row(d, m, w5, ts, tq, tn, top5);
+  }
+  uint32_t cur[5][4];
+#pragma unroll
+  for (int r = 0; r < 5; r++)
+    load_row(5 * i0 + 1 + r, cur[r]);
+  for (int i = i0; i < i1; i++) {
+    uint32_t nxt[5][4];
+    if (i + 1 < i1) {
+#pragma unroll
+      for (int r = 0; r < 5; r++)
+        load_row(5 * i + 6 + r, nxt[r]);
+    }
+    // window rows 5i-1 .. 5i+5: the carried pair, three core rows, a new pair
+    uint32_t s = ts, q = tq, n7 = tn, n5 = top5;
+    uint32_t s2 = 0, q2 = 0, n2 = 0, n5n = 0;
+    scout_row(cur[0], m, w5, s, q, n7, n5);
+    scout_row(cur[1], m, w5, s, q, n7, n5);
+    scout_row(cur[2], m, w5, s, q, n7, n5);
+    scout_row(cur[3], m, w5, s2, q2, n2, n5);
+    scout_row(cur[4], m, w5, s2, q2, n2, n5n);
+    s += s2;
+    q += q2;
+    n7 += n2;
+    ts = s2;
+    tq = q2;
+    tn = n2;
+    top5 = n5n;
+    // the probe's own value, from row 5i+2
+    uint32_t probe_row[4] = {cur[1][0], cur[1][1], cur[1][2], cur[1][3]};
+    if (5 * i + 2 >= height) { // uniform: a last cell row of one or two rows
+      // probe on the last row instead, window rows height-4 .. height-1
+      s = q = n7 = 0;
+      uint32_t d[4];
+#pragma unroll
+      for (int r = 4; r >= 1; r--) {
+        load_row(height - r, d);
+        scout_row(d, m, w5, s, q, n7, unused);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        probe_row[k] = d[k];
+    }
+    const int pk = probe_at >> 1;
+    const uint32_t word = pk == 0   ? probe_row[0]
+                          : pk == 1 ? probe_row[1]
+                          : pk == 2 ? probe_row[2]
+                                    : probe_row[3];
+    const uint32_t pv = (word >> (16 * (probe_at & 1))) & 0xFFFFu;
+    const bool masked = pv >= 0xFFFEu;
+    // the dispersion test (src/dext.cc:97-109): sums of v from sums of v + 1
+    const uint32_t sv = s - n7;
+    const uint32_t qv = q - 2u * sv - n7;
+    const float fm = (float)n7, fs = (float)sv, fq = (float)qv;
+    const float lhs = fm * fq - fs * fs - fs * (fm - 1.0f);
+    const float rhs = fs * 6.0f * lut[n7];
+    const bool active = live && (masked || lhs > rhs) && n5 > 0;
+    const uint64_t bits = ballot(active);
+    if (lane == 0) {
+      fd.act[(size_t)i * fd.n_strips + strip] = bits;
+      masks[i - i0] = bits;
+    }
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        cur[r][k] = nxt[r][k];
+  }
+  // the item's active cells, in its own segment of the list
+  uint32_t total = 0;
+  const uint32_t local = band * fd.n_strips + strip;
+  uint32_t *list = fd.alist + (size_t)local * kSegment;
+  for (int i = i0; i < i1; i++) {
+    const uint64_t bits = masks[i - i0];
+    if ((bits >> lane) & 1u)
+      list[total + count_below(bits)] = ((uint32_t)i << 16) | (uint32_t)j;
+    total += (uint32_t)__builtin_popcountll(bits);
+  }
+  if (lane == 0)
+    fd.icount[local] = total;
+}
+
+// Work item `item` of a launch (4 a block, XCD-local runs of blocks): its
+// frame, copied into registers (the passes store through pointers the
+// compiler cannot tell apart from the descriptor), and its index within the
+// frame; false past the end.
+__device__ __forceinline__ bool block_item(const FrameDesc *frames,
+                                           int n_frames, uint32_t items,
+                                           FrameDesc &fd, uint32_t &local) {
+  // the wave's index, as a scalar: everything derived from it is uniform
+  const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane(
+      (int)(4 * xcd_local_item(blockIdx.x, gridDim.x) + (threadIdx.x >> 6)));
+  if (item >= items)
+    return false;
+  fd = frames[frame_of_item(frames, n_frames, item)];
+  local = item - fd.item_begin;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void scout_kernel(const FrameDesc *frames,
+                                                    int n_frames,
+                                                    uint32_t items) {
+  __shared__ float lut[64];
+  __shared__ uint64_t masks[4][kBandCells];
+  fill_lut(lut);
+  __syncthreads();
+  FrameDesc fd;
+  uint32_t local;
+  if (!block_item(frames, n_frames, items, fd, local))
+    return;
+  const uint32_t strip = local % fd.n_strips, band = local / fd.n_strips;
+  if (fd.vec && fd.width >= 8)
+    scout_item<true>(fd, strip, band, lut, masks[threadIdx.x >> 6]);
+  else
+    scout_item<false>(fd, strip, band, lut, masks[threadIdx.x >> 6]);
+}
+
+// ---------------------------------------------------------------------------
+// Sixteen lanes to a cell, four cells to a wave. Lane c of a cell's group
+// holds pixel column 5j - 5 + c (c = 15 idles: no result reads it) and the
+// cell's rows 5i - 5 .. 5i + 9 in registers. Row sums are prefix
+// differences; column sums move within the group's DPP row (16 lanes):
+// right<N> reads lane c + N, left<N> lane c - N, and lanes past the row's end
+// read 0.
+// ---------------------------------------------------------------------------
+template <int N> __device__ __forceinline__ uint32_t right(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x100 + N, 0xF, 0xF, true);
+}
+template <int N> __device__ __forceinline__ uint32_t left(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x110 + N, 0xF, 0xF, true);
+}
+// x over lanes c-3 .. c+3 (valid at c = 3..11)
+__device__ __forceinline__ uint32_t across7(uint32_t x) {
+  const uint32_t a = x + right<1>(x); // c .. c+1
+  const uint32_t b = a + right<2>(a); // c .. c+3
+  return left<3>(b) + right<1>(a) + right<3>(x);
+}
+// x over lanes c-5 .. c+5 (valid at c = 5..9)
+__device__ __forceinline__ uint32_t across11(uint32_t x) {
+  const uint32_t a = x + right<1>(x); // c .. c+1
+  const uint32_t b = a + right<2>(a); // c .. c+3
+  const uint32_t d = b + right<4>(b); // c .. c+7
+  return left<5>(d) + right<3>(a) + right<5>(x);
+}
+
+constexpr uint64_t kCellLanes = 0x03E003E003E003E0ull; // lanes 5..9 a group
+
+// The frame as a buffer resource: reads past its end return 0, never fault.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_buffer(
+    const FrameDesc &fd) {
+  return __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<void *>(fd.pixels), 0,
+      (int)(2u * fd.height * fd.width), 0x00020000);
+}
+
+// The 15 rows of a lane's column around `cell`; off the frame, or on an
+// invalid group, pixels read as masked.
+__device__ __forceinline__ void load_column(const FrameDesc &fd,
+                                            __amdgpu_buffer_rsrc_t rsrc,
+                                            bool valid, uint32_t cell, int c,
+                                            uint32_t (&v)[15]) {
+  const int height = (int)fd.height, width = (int)fd.width;
+  const int i = (int)(cell >> 16), j = (int)(cell & 0xFFFFu);
+  const int x = 5 * j - 5 + c, y0 = 5 * i - 5;
+  // byte offset of row y0 (wraps below the frame: then out of range)
+  const uint32_t at = 2u * (uint32_t)(y0 * width + x);
+  const uint32_t pitch = 2u * (uint32_t)width;
+#pragma unroll
+  for (int r = 0; r < 15; r++)
+    v[r] = (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(
+        rsrc, (int)(at + (uint32_t)r * pitch), 0, 0);
+  // the frame's edges and invalid groups (uniform test): mask what is off
+  const bool edge = !valid || (c < 15 && (x < 0 || x >= width)) || y0 < 0 ||
+                    y0 + 15 > height;
+  if (ballot(edge)) {
+    const bool col = valid && x >= 0 && x < width;
+#pragma unroll
+    for (int r = 0; r < 15; r++)
+      if (!col || (uint32_t)(y0 + r) >= (uint32_t)height)
+        v[r] = 0xFFFFu;
+  }
+}
+
+// mask0 on the 9x9 pixels around a cell, then E on the cell and its
+// candidates (E, unmasked, 0 < p): src/dext.cc:65-142. Writes the cell's
+// E and clears its survivors; returns its candidates.
+__device__ uint32_t erode_group(const FrameDesc &fd, bool valid, uint32_t cell,
+                                const uint32_t (&v)[15], const float *lut) {
+  const uint32_t lane = lane_id();
+  // count above bit 26 and sum, packed; squares; as prefix sums over rows
+  uint32_t pn[16], pq[16];
+  pn[0] = pq[0] = 0;
+#pragma unroll
+  for (int r = 0; r < 15; r++) {
+    const uint32_t x = v[r] < 0xFFFEu ? (v[r] | (1u << 26)) : 0u;
+    pn[r + 1] = pn[r] + x;
+    pq[r + 1] = pq[r] + __umul24(x, x); // low 24 bits: v
+  }
+  // mask0 or masked ("set") at pixel rows 5i-2+o, lanes 3..11: 7x7 windows
+  // (src/dext.cc:69-111)
+  uint64_t h5[9];
+#pragma unroll
+  for (int o = 0; o < 9; o++) {
+    const uint32_t ns = across7(pn[o + 7] - pn[o]);
+    const uint32_t sq = across7(pq[o + 7] - pq[o]);
+    const uint32_t n = ns >> 26, sum = ns & 0x3FFFFFFu;
+    const float fm = (float)n, fs = (float)sum, fq = (float)sq;
+    const float lhs = fm * fq - fs * fs - fs * (fm - 1.0f);
+    const float rhs = fs * 6.0f * lut[n];
+    const uint64_t b = ballot(v[o + 3] >= 0xFFFEu || lhs > rhs);
+    // set on all five columns c-2 .. c+2
+    h5[o] = b & (b << 1) & (b << 2) & (b >> 1) & (b >> 2);
+  }
+  const uint32_t at = 16u * (lane >> 4) + 5u;
+  uint32_t e25 = 0, c25 = 0;
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const uint32_t p = v[t + 5];
+    const uint64_t unmasked = ballot(p < 0xFFFEu);
+    const uint64_t positive = ballot(p - 1u < 0xFFFDu); // 0 < p < 0xFFFE
+    const uint64_t e = h5[t] & h5[t + 1] & h5[t + 2] & h5[t + 3] & h5[t + 4] &
+                       unmasked & kCellLanes;
+    e25 |= (uint32_t)((e >> at) & 31u) << (5 * t);
+    c25 |= (uint32_t)(((e & positive) >> at) & 31u) << (5 * t);
+  }
+  if (valid && (lane & 15u) == 0) {
+    const uint32_t ci = (cell >> 16) * fd.n_cell_cols + (cell & 0xFFFFu);
+    fd.ecell[ci] = e25;
+    fd.scell[ci] = 0;
+  }
+  return c25;
+}
+
+// One item's active cells, 64 at a time, four to a wave-step; the next four
+// cells' pixels are fetched while the current four are tested.
+__device__ __forceinline__ void erode_item(const FrameDesc &fd, uint32_t local,
+                           const float *lut) {
+  const uint32_t n = fd.icount[local];
+  const uint32_t lane = lane_id(), grp = lane >> 4;
+  const int c = (int)(lane & 15u);
+  const uint32_t *cells = fd.alist + (size_t)local * kSegment;
+  uint32_t *cand = fd.ccand + (size_t)local * kSegment;
+  const __amdgpu_buffer_rsrc_t rsrc = frame_buffer(fd);
+  for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+    const uint32_t m = min(64u, n - c0);
+    const uint32_t mine = lane < m ? cells[c0 + lane] : 0u;
+    const uint32_t steps = (m + 3) / 4;
+    bool valid = grp < m;
+    uint32_t cell = (uint32_t)__shfl((int)mine, (int)grp, 64);
+    uint32_t v[15];
+    load_column(fd, rsrc, valid, cell, c, v);
+    for (uint32_t g = 0; g < steps; g++) {
+      const uint32_t next = 4 * (g + 1) + grp;
+      const bool next_valid = next < m;
+      const uint32_t next_cell =
+          (uint32_t)__shfl((int)mine, (int)(next & 63u), 64);
+      uint32_t vn[15];
+      if (g + 1 < steps)
+        load_column(fd, rsrc, next_valid, next_cell, c, vn);
+      const uint32_t c25 = erode_group(fd, valid, cell, v, lut);
+      if (valid && c == 0)
+        cand[c0 + 4 * g + grp] = c25;
+#pragma unroll
+      for (int r = 0; r < 15; r++)
+        v[r] = vn[r];
+      cell = next_cell;
+      valid = next_valid;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void erode_kernel(const FrameDesc *frames,
+                                                    int n_frames,
+                                                    uint32_t items) {
+  __shared__ float lut[64];
+  fill_lut(lut);
+  __syncthreads();
+  FrameDesc fd;
+  uint32_t local;
+  if (block_item(frames, n_frames, items, fd, local))
+    erode_item(fd, local, lut);
+}
+
+// The Poisson test at a cell's candidates (src/dext.cc:146-215): 11x11 sums
+// of pixels neither masked nor eroded. The survivors, and for the lane's
+// pixel on each of the cell's rows: the mean, and count | value << 16.
+struct PoissonOut {
+  uint32_t s25;
+  float mean[5];
+  uint32_t pop[5];
+};
+
+__device__ void poisson_group(const FrameDesc &fd, __amdgpu_buffer_rsrc_t rsrc,
+                              bool valid, uint64_t entry, PoissonOut &res) {
+  const uint32_t lane = lane_id();
+  const int c = (int)(lane & 15u);
+  const uint32_t cell = (uint32_t)entry, c25 = (uint32_t)(entry >> 32);
+  const int i = (int)(cell >> 16), j = (int)(cell & 0xFFFFu);
+  const int width = (int)fd.width, ncr = (int)fd.n_cell_rows;
+  // E of the lane's column in the cells above, at and below: 0 where a cell
+  // is not active (E holds only in active cells) or off the frame
+  const int x = 5 * j - 5 + c;
+  const bool col = valid && c < 15 && x >= 0 && x < width;
+  const int jj = col ? x / 5 : 0, cc = col ? x - 5 * jj : 0;
+  uint64_t word[3];
+  uint32_t eroded[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int ii = min(max(i - 1 + a, 0), ncr - 1);
+    word[a] = fd.act[(uint32_t)ii * fd.n_strips + (uint32_t)(jj >> 6)];
+    eroded[a] = fd.ecell[(uint32_t)ii * fd.n_cell_cols + (uint32_t)jj];
+  }
+  uint32_t v[15];
+  load_column(fd, rsrc, valid, cell, c, v);
+  uint32_t e[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int ii = i - 1 + a;
+    e[a] = (col && ii >= 0 && ii < ncr && ((word[a] >> (jj & 63)) & 1u))
+               ? eroded[a]
+               : 0u;
+  }
+  // count above bit 23 and sum, packed, as prefix sums over rows
+  uint32_t pn[16];
+  pn[0] = 0;
+#pragma unroll
+  for (int r = 0; r < 15; r++) {
+    const uint32_t out = (e[r / 5] >> (5 * (r % 5) + cc)) & 1u;
+    const uint32_t xr = (v[r] < 0xFFFEu && !out) ? (v[r] | (1u << 23)) : 0u;
+    pn[r + 1] = pn[r] + xr;
+  }
+  const uint32_t at = 16u * (lane >> 4) + 5u;
+  res.s25 = 0;
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const uint32_t ns = across11(pn[t + 11] - pn[t]);
+    const uint32_t n = ns >> 23, sum = ns & 0x7FFFFFu;
+    const uint32_t p = v[t + 5];
+    // src/dext.cc:203-206, in the reference's order and rounding
+    const float m = n >= 2 ? (float)sum / (float)n : 0.0f;
+    const bool ok = (float)p >= m + 3.0f * sqrtf(m);
+    const bool cand = c >= 5 && c <= 9 && ((c25 >> (5 * t + c - 5)) & 1u);
+    const uint64_t b = ballot(valid && cand && ok);
+    res.s25 |= (uint32_t)((b >> at) & 31u) << (5 * t);
+    res.mean[t] = m;
+    res.pop[t] = n | (p << 16);
+  }
+}
+
+constexpr int kBatch = 4; // wave-steps of four cells per staging reservation
+
+// Survivors of the four cells of a wave-step: this group's count, the
+// groups' total, and the count of the groups before this one.
+__device__ __forceinline__ uint32_t step_counts(uint32_t s25, uint32_t grp,
+                                                uint32_t &before) {
+  const uint32_t k = (uint32_t)__builtin_popcount(s25);
+  const uint32_t k0 = (uint32_t)__shfl((int)k, 0, 64);
+  const uint32_t k1 = (uint32_t)__shfl((int)k, 16, 64);
+  const uint32_t k2 = (uint32_t)__shfl((int)k, 32, 64);
+  const uint32_t k3 = (uint32_t)__shfl((int)k, 48, 64);
+  before = (grp > 0 ? k0 : 0u) + (grp > 1 ? k1 : 0u) + (grp > 2 ? k2 : 0u);
+  return k0 + k1 + k2 + k3;
+}
+
+// One item's candidate cells: each 64 of its active cells are compacted to
+// their candidate cells (in LDS), tested four at a time, and every kBatch
+// steps' survivors go to staging under one atomic, each cell's run in its
+// pixel order.
+__device__ __forceinline__ void poisson_item(const FrameDesc &fd, uint32_t local,
+                             uint64_t *slots) {
+  const uint32_t n = fd.icount[local];
+  const uint32_t lane = lane_id(), grp = lane >> 4;
+  const int c = (int)(lane & 15u);
+  const uint32_t *cells = fd.alist + (size_t)local * kSegment;
+  const uint32_t *cand = fd.ccand + (size_t)local * kSegment;
+  const uint32_t width = fd.width;
+  const __amdgpu_buffer_rsrc_t rsrc = frame_buffer(fd);
+  for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+    const bool here = c0 + lane < n;
+    const uint32_t cell = here ? cells[c0 + lane] : 0u;
+    const uint32_t c25 = here ? cand[c0 + lane] : 0u;
+    const uint64_t has = ballot(c25 != 0);
+    if (c25)
+      slots[count_below(has)] = (uint64_t)cell | ((uint64_t)c25 << 32);
+    const uint32_t nc = (uint32_t)__builtin_popcountll(has);
+    const uint32_t steps = (nc + 3) / 4;
+    for (uint32_t g0 = 0; g0 < steps; g0 += kBatch) {
+      PoissonOut res[kBatch];
+      uint32_t cells_of[kBatch];
+      uint32_t total = 0;
+#pragma unroll
+      for (int b = 0; b < kBatch; b++) {
+        const uint32_t at = 4 * (g0 + b) + grp;
+        const bool valid = g0 + b < steps && at < nc;
+        const uint64_t entry = valid ? slots[at] : 0ull;
+        cells_of[b] = (uint32_t)entry;
+        res[b].s25 = 0;
+        if (g0 + b < steps) // uniform
+          poisson_group(fd, rsrc, valid, entry, res[b]);
+        uint32_t unused;
+        total += step_counts(res[b].s25, grp, unused);
+      }
+      if (total == 0)
+        continue;
+      uint32_t base = 0;
+      if (lane == 0)
+        base = atomicAdd(fd.counter, total);
+      base = (uint32_t)__shfl((int)base, 0, 64);
+#pragma unroll
+      for (int b = 0; b < kBatch; b++) {
+        uint32_t before;
+        const uint32_t sum = step_counts(res[b].s25, grp, before);
+        const uint32_t mine = base + before;
+        base += sum;
+        const uint32_t s25 = res[b].s25;
+        if (!s25)
+          continue;
+        const uint32_t i = cells_of[b] >> 16, j = cells_of[b] & 0xFFFFu;
+        if (c == 0) {
+          const uint32_t ci = i * fd.n_cell_cols + j;
+          fd.scell[ci] = s25;
+          fd.sbase[ci] = mine;
+#pragma unroll
+          for (int t = 0; t < 5; t++) {
+            const uint32_t row = (s25 >> (5 * t)) & 31u;
+            if (row)
+              atomicAdd(&fd.rowsurv[5 * i + t],
+                        (uint32_t)__builtin_popcount(row));
+          }
+        }
+        if (c < 5 || c > 9)
+          continue;
+#pragma unroll
+        for (int t = 0; t < 5; t++) {
+          const int bit = 5 * t + c - 5;
+          if (!((s25 >> bit) & 1u))
+            continue;
+          const uint32_t dst =
+              mine + (uint32_t)__builtin_popcount(s25 & ((1u << bit) - 1u));
+          if (dst < fd.capacity) {
+            sf_signal_pixel px;
+            px.index = (5 * i + t) * width + 5 * j + (uint32_t)c - 5;
+            px.value = res[b].pop[t] >> 16;
+            px.background = res[b].mean[t];
+            px.population = (uint16_t)(res[b].pop[t] & 0xFFFFu);
+            px.reserved = 0;
+            fd.staging[dst] = px;
+          }
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void poisson_kernel(const FrameDesc *frames,
+                                                      int n_frames,
+                                                      uint32_t items) {
+  __shared__ uint64_t slots[4][64];
+  FrameDesc fd;
+  uint32_t local;
+  if (block_item(frames, n_frames, items, fd, local))
+    poisson_item(fd, local, slots[threadIdx.x >> 6]);
+}
+
+// Inclusive sum of x over the lanes up to this one (DPP: within rows of 16,
+// then row 0's and row 1's totals broadcast upwards).
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);
+  return x;
+}
+
+// rowsurv, per pixel row, becomes its exclusive prefix (the row's first
+// place in the list), with the frame's total after the last row. One block a
+// frame.
+__global__ __launch_bounds__(1024) void rowscan_kernel(const FrameDesc *frames) {
+  __shared__ uint32_t sums[16];
+  __shared__ uint32_t carry;
+  const FrameDesc fd = frames[blockIdx.x];
+  const uint32_t n = 5 * fd.n_cell_rows;
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  uint32_t run = 0; // the carry, as thread 1023 knows it
+  if (threadIdx.x == 0)
+    carry = 0;
+  __syncthreads();
+  for (uint32_t b = 0; b < n; b += 4096) {
+    const uint32_t at = b + 4 * threadIdx.x;
+    uint32_t v[4], local = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      v[q] = at + q < n ? fd.rowsurv[at + q] : 0u;
+      local += v[q];
+    }
+    const uint32_t incl = wave_prefix(local);
+    if (lane == 63)
+      sums[wave] = incl;
+    __syncthreads();
+    uint32_t before = carry;
+    for (uint32_t w = 0; w < wave; w++)
+      before += sums[w];
+    uint32_t x = before + incl - local;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (at + q < n)
+        fd.rowsurv[at + q] = x;
+      x += v[q];
+    }
+    run = x;
+    __syncthreads();
+    if (threadIdx.x == 1023)
+      carry = run;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    fd.rowsurv[n] = carry;
+}
+
+constexpr int kTagWords = 16; // act words a cell row's pass holds at once
+
+// Places in index order for one cell row's survivors: pixel row 5i+t before
+// 5i+t+1, within a pixel row by cell, then by column. Each of the row's
+// staging entries gets its place; no entry is read.
+__device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i) {
+  const uint32_t lane = lane_id();
+  const uint32_t *first = fd.rowsurv + 5 * i; // places of the five rows
+  if (first[5] == first[0])
+    return; // no survivor on the cell row
+  uint32_t start[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++)
+    start[t] = first[t];
+  const uint64_t *act = fd.act + (size_t)i * fd.n_strips;
+  for (uint32_t w0 = 0; w0 < fd.n_strips; w0 += kTagWords) {
+    uint64_t bits[kTagWords];
+    uint32_t s[kTagWords], from[kTagWords];
+#pragma unroll
+    for (int u = 0; u < kTagWords; u++)
+      bits[u] = w0 + u < fd.n_strips ? act[w0 + u] : 0ull;
+#pragma unroll
+    for (int u = 0; u < kTagWords; u++) {
+      const uint32_t ci = i * fd.n_cell_cols + 64 * (w0 + u) + lane;
+      const bool on = (bits[u] >> lane) & 1u;
+      s[u] = on ? fd.scell[ci] : 0u;
+      from[u] = on ? fd.sbase[ci] : 0u; // meaningful where s != 0
+    }
+#pragma unroll
+    for (int u = 0; u < kTagWords; u++) {
+      if (!bits[u])
+        continue;
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+        const uint32_t k = (uint32_t)__builtin_popcount((s[u] >> (5 * t)) & 31u);
+        if (!ballot(k != 0))
+          continue;
+        const uint32_t incl = wave_prefix(k);
+        const uint32_t dst = start[t] + incl - k;
+        const uint32_t src =
+            from[u] + (uint32_t)__builtin_popcount(s[u] & ((1u << (5 * t)) - 1u));
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+          if ((uint32_t)q < k && src + q < fd.capacity)
+            fd.dstof[src + q] = dst + q;
+        start[t] += (uint32_t)__shfl((int)incl, 63, 64);
+      }
+    }
+  }
+}
+
+// Grid (cell-row quads, frames): a wave a cell row. Places are given only
+// when the list fits (the capacity contract of src/dext_gpu.cc).
+__global__ __launch_bounds__(256) void tag_kernel(const FrameDesc *frames) {
+  const FrameDesc fd = frames[blockIdx.y];
+  const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane(
+      (int)(4 * blockIdx.x + (threadIdx.x >> 6)));
+  if (i >= fd.n_cell_rows || *fd.counter > fd.capacity)
+    return;
+  tag_row(fd, i);
+}
+
+// Every staging entry to its place; the count goes out whether or not the
+// list fits. Grid (blocks, frames).
+__global__ __launch_bounds__(256) void place_kernel(const FrameDesc *frames) {
+  const FrameDesc fd = frames[blockIdx.y];
+  const uint32_t found = *fd.counter;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    *fd.out_count = found;
+  if (found > fd.capacity)
+    return;
+  const uint4 *from = reinterpret_cast<const uint4 *>(fd.staging);
+  uint4 *to = reinterpret_cast<uint4 *>(fd.out);
+  for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < found;
+       e += gridDim.x * 256) {
+    const uint32_t at = fd.dstof[e];
+    if (at < found)
+      to[at] = from[e];
+  }
+}
+
+__global__ void zero_kernel(const FrameDesc *frames) {
+  const FrameDesc &fd = frames[blockIdx.x];
+  if (threadIdx.x == 0)
+    *fd.counter = 0;
+  for (uint32_t r = threadIdx.x; r < 5 * fd.n_cell_rows; r += blockDim.x)
+    fd.rowsurv[r] = 0;
+}
+
+} // namespace
+
+// Items of a frame for the scout: 64-cell strips x 16-cell-row bands.
+void sparse_layout(uint32_t height, uint32_t width, FrameDesc &d) {
+  d.n_cell_rows = (height + 4) / 5;
+  d.n_cell_cols = (width + 4) / 5;
+  d.n_strips = (d.n_cell_cols + 63) / 64;
+  d.band_rows = 5 * kBandCells;
+  d.n_bands = (d.n_cell_rows + kBandCells - 1) / kBandCells;
+}
+
+// Bytes of the sparse tables of one frame, and their layout from `base`
+// (256-byte aligned pieces); `capacity`: the frame's list entries.
+static size_t sparse_piece(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t sparse_bytes(uint32_t height, uint32_t width, uint32_t capacity) {
+  FrameDesc d{};
+  sparse_layout(height, width, d);
+  const size_t cells = (size_t)d.n_cell_rows * d.n_cell_cols;
+  const size_t items = (size_t)d.n_strips * d.n_bands;
+  return sparse_piece(8 * (size_t)d.n_cell_rows * d.n_strips) +
+         sparse_piece(4 * items) + 2 * sparse_piece(4 * items * kSegment) +
+         3 * sparse_piece(4 * cells) +
+         sparse_piece(20 * (size_t)d.n_cell_rows + 4) +
+         sparse_piece(4 * (size_t)capacity);
+}
+
+void sparse_carve(uint8_t *base, FrameDesc &d) {
+  const size_t cells = (size_t)d.n_cell_rows * d.n_cell_cols;
+  const size_t items = (size_t)d.n_strips * d.n_bands;
+  size_t at = 0;
+  const auto take = [&](size_t bytes) {
+    uint8_t *p = base + at;
+    at += sparse_piece(bytes);
+    return p;
+  };
+  d.act = reinterpret_cast<uint64_t *>(take(8 * (size_t)d.n_cell_rows * d.n_strips));
+  d.icount = reinterpret_cast<uint32_t *>(take(4 * items));
+  d.alist = reinterpret_cast<uint32_t *>(take(4 * items * kSegment));
+  d.ccand = reinterpret_cast<uint32_t *>(take(4 * items * kSegment));
+  d.ecell = reinterpret_cast<uint32_t *>(take(4 * cells));
+  d.scell = reinterpret_cast<uint32_t *>(take(4 * cells));
+  d.sbase = reinterpret_cast<uint32_t *>(take(4 * cells));
+  d.rowsurv =
+      reinterpret_cast<uint32_t *>(take(20 * (size_t)d.n_cell_rows + 4));
+  d.dstof = reinterpret_cast<uint32_t *>(take(4 * (size_t)d.capacity));
+}
+
+// The sparse form on `stream` in two parts: the threshold (clear, scout,
+// erode, Poisson) and the ordering (row prefix, places, copy). `items`: the
+// launch's scout items; `max_cell_rows`: the most cell rows of any frame.
+hipError_t launch_sparse_threshold(const FrameDesc *frames, int n_frames,
+                                   uint32_t items, hipStream_t stream) {
+  if (n_frames <= 0)
+    return hipSuccess;
+  zero_kernel<<<n_frames, 256, 0, stream>>>(frames);
+  if (items) {
+    const uint32_t blocks = ((items + 3) / 4 + 7) & ~7u;
+    scout_kernel<<<blocks, 256, 0, stream>>>(frames, n_frames, items);
+    erode_kernel<<<blocks, 256, 0, stream>>>(frames, n_frames, items);
+    poisson_kernel<<<blocks, 256, 0, stream>>>(frames, n_frames, items);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse_order(const FrameDesc *frames, int n_frames,
+                               uint32_t max_cell_rows, hipStream_t stream) {
+  if (n_frames <= 0)
+    return hipSuccess;
+  rowscan_kernel<<<n_frames, 1024, 0, stream>>>(frames);
+  if (max_cell_rows)
+    tag_kernel<<<dim3((max_cell_rows + 3) / 4, n_frames), 256, 0, stream>>>(
+        frames);
+  place_kernel<<<dim3(64, n_frames), 256, 0, stream>>>(frames);
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse(const FrameDesc *frames, int n_frames, uint32_t items,
+                         uint32_t max_cell_rows, hipStream_t stream) {
+  const hipError_t e =
+      launch_sparse_threshold(frames, n_frames, items, stream);
+  if (e != hipSuccess)
+    return e;
+  return launch_sparse_order(frames, n_frames, max_cell_rows, stream);
+}
+
+} // namespace sf

@@ -47,7 +47,10 @@ void fused_layout(int pixel_bytes, uint32_t height, uint32_t width,
 size_t cell_entries(uint32_t height, uint32_t width);
 hipError_t launch_batch(int pixel_bytes, bool any_vec, bool any_scalar,
                         const FrameDesc *d, int n, uint32_t items,
-                        uint32_t max_bands, hipStream_t stream);
+                        uint32_t max_bands, uint32_t max_cell_rows,
+                        hipStream_t stream);
+size_t sparse_bytes(uint32_t height, uint32_t width, uint32_t capacity);
+void sparse_carve(uint8_t *base, FrameDesc &d);
 void ensure_device_tables();
 int labeller_add_lists(sf_labeller *l, int n, const int64_t *numbers,
                        const sf_signal_pixel *const *lists,
@@ -173,8 +176,15 @@ void free_slot(Slot *slot) {
   delete slot;
 }
 
+// Bytes of a 16-bit frame's sparse tables (0 at 32 bits).
+size_t sparse_part(int pixel_bytes, const sf::FrameDesc &d) {
+  return pixel_bytes == 2 ? sf::sparse_bytes(d.height, d.width, d.capacity)
+                          : 0;
+}
+
 // One frame's buffers for a series: staging and ordered list (capacity
-// entries each), cell counts and offsets, band totals.
+// entries each), cell counts and offsets, band totals, and at 16 bits the
+// sparse form's tables.
 hipError_t make_slot(Series *se, Slot **out) {
   Slot *slot = new Slot;
   slot->series = se;
@@ -194,6 +204,9 @@ hipError_t make_slot(Series *se, Slot **out) {
   at += ((size_t)d.n_bands + 1) * sizeof(uint32_t);
   const size_t count_at = at;
   at += cells;
+  at = (at + 255) & ~(size_t)255;
+  const size_t sparse_at = at;
+  at += sparse_part(se->pixel_bytes, d);
   const hipError_t e = hipMalloc(&slot->base, at);
   if (e != hipSuccess) {
     delete slot;
@@ -205,6 +218,8 @@ hipError_t make_slot(Series *se, Slot **out) {
   d.cell_base = reinterpret_cast<uint32_t *>(b + base_at);
   d.band_total = reinterpret_cast<uint32_t *>(b + band_at);
   d.cell_count = reinterpret_cast<uint8_t *>(b + count_at);
+  if (se->pixel_bytes == 2)
+    sf::sparse_carve(reinterpret_cast<uint8_t *>(b + sparse_at), d);
   *out = slot;
   return hipSuccess;
 }
@@ -214,8 +229,9 @@ size_t slot_bytes(int pixel_bytes, uint32_t height, uint32_t width) {
   sf::fused_layout(pixel_bytes, height, width, d);
   const size_t cap = std::max<size_t>(d.capacity, 1);
   const size_t cells = sf::cell_entries(height, width);
-  return 2 * cap * sizeof(sf_signal_pixel) + cells * (sizeof(uint32_t) + 1) +
-         ((size_t)d.n_bands + 1) * sizeof(uint32_t);
+  return 2 * cap * sizeof(sf_signal_pixel) + cells * sizeof(uint32_t) +
+         (((size_t)d.n_bands + 1) * sizeof(uint32_t)) +
+         cells + 255 + sparse_part(pixel_bytes, d);
 }
 
 hipError_t make_batch(int capacity, Batch **out) {
@@ -261,7 +277,7 @@ void free_batch(Batch *b) {
 // Enqueue a formed batch on its stream (no lock held).
 hipError_t launch(Batch *b) {
   const int n = (int)b->requests.size();
-  uint32_t items = 0, max_bands = 0;
+  uint32_t items = 0, max_bands = 0, max_cell_rows = 0;
   bool any_vec = false, any_scalar = false;
   const uint32_t k = b->pixel_bytes == 2 ? 4 : 2;
   for (int f = 0; f < n; f++) {
@@ -272,6 +288,7 @@ hipError_t launch(Batch *b) {
     d.item_begin = items;
     items += d.n_strips * d.n_bands;
     max_bands = std::max(max_bands, d.n_bands);
+    max_cell_rows = std::max(max_cell_rows, d.n_cell_rows);
     const bool aligned =
         (reinterpret_cast<uintptr_t>(d.pixels) % 8) == 0;
     d.vec = (aligned && d.width % k == 0) ? 1u : 0u;
@@ -284,7 +301,7 @@ hipError_t launch(Batch *b) {
                      hipMemcpyHostToDevice, b->stream);
   if (e == hipSuccess)
     e = sf::launch_batch(b->pixel_bytes, any_vec, any_scalar, b->d_desc, n,
-                         items, max_bands, b->stream);
+                         items, max_bands, max_cell_rows, b->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->h_found, b->d_found, sizeof(uint32_t) * n,
                        hipMemcpyDeviceToHost, b->stream);
@@ -844,6 +861,14 @@ int sf_hip_server_stats(sf_server *s, sf_server_stats *stats) {
   stats->peak_bytes = s->peak;
   stats->budget = s->budget;
   return SF_OK;
+}
+
+size_t sf_hip_server_slot_bytes(int pixel_bytes, size_t height,
+                                size_t width) {
+  if ((pixel_bytes != 2 && pixel_bytes != 4) || !height || !width ||
+      height > 0x7fffffffu / width)
+    return 0;
+  return slot_bytes(pixel_bytes, (uint32_t)height, (uint32_t)width);
 }
 
 int sf_hip_server_play(sf_server *s, int n_series, const int *series,

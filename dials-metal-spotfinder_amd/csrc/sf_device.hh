@@ -74,6 +74,22 @@ struct FrameDesc {
   uint32_t capacity;
   uint32_t item_begin;         // first work item of this frame
   uint32_t vec;                // 1 if rows may be read K pixels at a time
+  // The sparse u16 form's per-frame tables (dext_sparse.hip): 5x5 cells on
+  // a global lattice, n_cell_rows x n_cell_cols of them, 64 cell columns to
+  // a word of `act`. Per-cell tables are indexed row * n_cell_cols + column
+  // and hold a value only where the cell's `act` bit is set.
+  // A scout item (16 cell rows x 64 cell columns, n_strips x n_bands of them)
+  // lists its active cells in its own segment of 1024 entries.
+  uint64_t *act;               // n_cell_rows * n_strips: active cells
+  uint32_t *icount;            // per item: active cells it listed
+  uint32_t *alist;             // per item segment: cells, row << 16 | column
+  uint32_t *ccand;             // beside alist: each cell's candidates
+  uint32_t *ecell;             // per cell: its eroded pixels (25 bits)
+  uint32_t *scell;             // per cell: its surviving pixels
+  uint32_t *sbase;             // per cell: staging offset of its survivors
+  uint32_t *rowsurv;           // per pixel row: survivors, then first place
+  uint32_t *dstof;             // per staging entry: its place in `out`
+  uint32_t n_cell_rows, n_cell_cols;
 };
 
 // ---------------------------------------------------------------------------

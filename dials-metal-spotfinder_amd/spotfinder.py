@@ -133,6 +133,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_labeller_pixel_count", _sz, _p)
     fn("sf_hip_labeller_results", _i, _p, _p, _p, _p)
     fn("sf_hip_labeller_last_error", ctypes.c_char_p)
+    fn("sf_hip_server_slot_bytes", _sz, _i, _sz, _sz)
     fn("sf_hip_server_create", _p, _sz, _i, _i)
     fn("sf_hip_server_destroy", None, _p)
     fn("sf_hip_server_open", _i, _p, _i, _sz, _sz, ctypes.POINTER(Options))
@@ -320,8 +321,10 @@ def synth_frames(buffer: DeviceBuffer, pixel_bytes: int, height: int,
 class Plan:
     """Fixed frame shapes thresholded together on the device.
 
-    form 0 is the fused single-pass kernel, form 1 the plain three-kernel
-    per-frame form. Frames are device pointers (ints).
+    form 0 is the fused default (the sparse passes at 16 bits, the register
+    sweep at 32), form 1 the plain three-kernel per-frame form, form 2 the
+    register sweep, form 3 the probe-lattice tile kernel (16 bits). Frames
+    are device pointers (ints).
     """
 
     def __init__(self, pixel_bytes: int, shapes, band_rows: int = 0,
@@ -358,7 +361,7 @@ class Plan:
         return band.value, blocks.value
 
     def set_tile(self, shape: int) -> None:
-        """Probe kernel tile: 0 = 64x104, 1 = 128x104, 2 = 64x232,
+        """Probe kernel (form 3) tile: 0 = 64x104, 1 = 128x104, 2 = 64x232,
         3 = 32x104."""
         _check(lib().sf_hip_plan_set_tile(self.handle, shape), "set_tile")
 
@@ -503,6 +506,14 @@ def _labeller_results(handle, status):
 
 
 # --- the serving loop ---------------------------------------------------------
+
+def server_slot_bytes(pixel_bytes: int, height: int, width: int) -> int:
+    """Bytes of budget one frame of this shape takes in a Server."""
+    n = int(lib().sf_hip_server_slot_bytes(pixel_bytes, height, width))
+    if n == 0:
+        raise ValueError("server_slot_bytes: bad shape or pixel type")
+    return n
+
 
 class Server:
     """Frames of several concurrent series, batched as they arrive, grouped

@@ -128,11 +128,13 @@ int sf_hip_take_found(sf_signal_pixel *out, size_t n);
  * launches one frame at a time, dext_cuda.cu:603-656) ----------------------
  * A plan fixes a list of frame shapes of one pixel type (2 or 4 bytes) and
  * owns the staging, cell tables and ordered output lists for them. run()
- * thresholds one device-resident frame per shape in one threshold launch
- * plus one ordering launch, asynchronously on the plan's stream.
- * form 0 = fused (default: the sparse probe-lattice tile kernel for 16-bit
- * frames, the register-sweep kernel for 32-bit ones), 1 = plain three-kernel
- * form, one frame per launch sequence, 2 = the register-sweep kernel. */
+ * thresholds every frame of the plan in one sequence of launches (threshold,
+ * then ordering), asynchronously on the plan's stream.
+ * form 0 = fused (default: the sparse passes for 16-bit frames -- a probe
+ * per 5x5 cell, then erosion, Poisson test and ordering only around active
+ * cells -- the register-sweep kernel for 32-bit ones), 1 = plain
+ * three-kernel form, one frame per launch sequence, 2 = the register-sweep
+ * kernel, 3 = the probe-lattice tile kernel (16-bit; the sweep at 32). */
 typedef struct sf_plan sf_plan;
 sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
                             const uint32_t *heights, const uint32_t *widths,
@@ -147,11 +149,14 @@ int sf_hip_plan_set_grid(sf_plan *plan, int blocks);
  * fastest. For the sweep kernel the candidates are band heights (the tile's
  * rows: 44, 108, 236, 492) crossed with grids (0 and 4 / 8 / 16 resident
  * blocks a CU); for the probe kernel, its four tile shapes crossed with the
- * same grids. Reports the chosen tile rows and grid through band_rows /
- * blocks (either may be NULL). Drops a captured graph: capture again. */
+ * same grids; the sparse passes' launch shape follows the frames, so they
+ * have nothing to time (band_rows reports their 80-row scout band). Reports
+ * the chosen tile rows and grid through band_rows / blocks (either may be
+ * NULL). Drops a captured graph: capture again. */
 int sf_hip_plan_autotune(sf_plan *plan, const void *const *device_frames,
                          int trials, int *band_rows, int *blocks);
-/* The probe kernel's tile shape: 0 = 64 x 104 (256 threads, the default),
+/* The probe kernel's (form 3) tile shape: 0 = 64 x 104 (256 threads, the
+ * default),
  * 1 = 128 x 104 and 2 = 64 x 232 (512 threads each), 3 = 32 x 104 (128
  * threads). The autotuner tries all of them; sf_hip_plan_tile reports the
  * current one. Drops a captured graph. */
@@ -273,6 +278,8 @@ int sf_hip_server_frame_list(sf_server *server, int series, size_t k,
                              sf_signal_pixel *out, size_t capacity,
                              size_t *n);
 int sf_hip_server_stats(sf_server *server, sf_server_stats *stats);
+/* Bytes of budget one frame's slot of this shape takes. */
+size_t sf_hip_server_slot_bytes(int pixel_bytes, size_t height, size_t width);
 /* Plays an arrival schedule and drains: frame f of series i (f < n_frames[i])
  * arrives at t0 + f / rate_hz[i] and is numbered first_frame + f; its pixels
  * are rings[sum(ring_lengths[:i]) + f % ring_lengths[i]]. Latencies are

@@ -10,8 +10,6 @@ tolerance: the outputs are integers and reference-order FP64.
 
 from __future__ import annotations
 
-import math
-
 import numpy as np
 import pytest
 
@@ -27,12 +25,6 @@ SERIES = [
     (4, 200, 260, 12, 33, 50.0, {}),
     (2, 150, 260, 20, 34, 0.3, {"two_d": True}),
 ]
-
-
-def slot_bytes(pixel_bytes, h, w):
-    """An upper bound of one frame's device buffers in the server."""
-    cap = h * w * pixel_bytes // 16
-    return 2 * 16 * cap + h * math.ceil(w / 64) * 5 + (h // 16 + 2) * 4
 
 
 def make_series(sf):
@@ -81,7 +73,7 @@ def same_tables(a, b):
 def test_alone_equals_mix_equals_oracle(sf):
     data = make_series(sf)
     big = 1 << 30
-    tight = sum(slot_bytes(*s[:3]) for s in SERIES) + 4096
+    tight = sum(sf.server_slot_bytes(*s[:3]) for s in SERIES) + 4096
     mix, stats = run(sf, data, [0, 1, 2, 3], tight, max_batch=5,
                      rates=[3000.0, 2000.0, 1500.0, 2500.0])
     assert stats["batched_frames"] == sum(s[3] for s in SERIES)

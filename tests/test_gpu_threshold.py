@@ -105,12 +105,13 @@ def test_random_frames_edges_and_both_paths(sf):
             assert same(got, expected), (shape, dtype, describe(got, expected))
 
 
-@pytest.mark.parametrize("form,tile", [(0, 0), (0, 1), (0, 2), (0, 3), (2, 0)])
+@pytest.mark.parametrize("form,tile", [(0, 0), (3, 0), (3, 1), (3, 2), (3, 3),
+                                       (2, 0)])
 def test_random_frames_device_resident(sf, form, tile):
-    """The same edge cases through device-resident plans: the probe kernel
-    (form 0 at 16 bits) in each of its three tile shapes, and the
-    register-sweep kernel (form 2), including widths that are not a multiple
-    of 4 and an unaligned frame start."""
+    """The same edge cases through device-resident plans: the sparse passes
+    (form 0 at 16 bits), the probe tile kernel (form 3) in each of its four
+    tile shapes, and the register-sweep kernel (form 2), including widths
+    that are not a multiple of 4 and an unaligned frame start."""
     rng = np.random.default_rng(40 + form + 7 * tile)
     shapes = [(1, 1), (2, 3), (9, 1), (5, 5), (7, 300), (300, 7), (33, 65),
               (64, 64), (129, 257), (250, 232), (250, 233), (251, 464),
@@ -200,7 +201,7 @@ def test_full_size_model_frames_fused_and_plain(sf):
     buf, host = model_frames(sf, 2, h, w, seed=101)
     expected = [cpu.dext(host[f], capacity=h * w // 8)[1] for f in range(2)]
     assert all(len(e) > 1000 for e in expected)
-    for form, tile in ((0, 0), (0, 1), (0, 2), (0, 3), (1, 0), (2, 0)):
+    for form, tile in ((0, 0), (3, 0), (3, 1), (3, 2), (3, 3), (1, 0), (2, 0)):
         plan = sf.Plan(2, [(h, w)] * 2, form=form)
         plan.set_tile(tile)
         plan.run([buf.ptr, buf.ptr + 2 * h * w])
@@ -217,7 +218,8 @@ def test_batched_uneven_sizes_graph_and_persistent(sf):
     """The uneven-chunk shape mix (2162x2068, 3269x3110, 4362x4148, scaled
     to keep the oracle quick) in one launch, replayed from a captured graph
     with eager work between replays, through a persistent grid, and after
-    the autotuner has picked its launch shape (probe and sweep kernels)."""
+    the autotuner has picked its launch shape (sparse, sweep and probe
+    kernels)."""
     shapes = [(216, 206), (327, 311), (436, 414), (216, 206), (101, 3)]
     bufs, hosts = [], []
     for k, (h, w) in enumerate(shapes):
@@ -257,7 +259,7 @@ def test_batched_uneven_sizes_graph_and_persistent(sf):
     counts = plan.counts()
     for f in range(len(shapes)):
         assert same(plan.output(f, int(counts[f])), expected[f]), ("grid", f)
-    for form in (0, 2):
+    for form in (0, 2, 3):
         plan.set_form(form)
         band, blocks = plan.autotune(ptrs, 1)
         assert blocks >= 0 and band > 0
