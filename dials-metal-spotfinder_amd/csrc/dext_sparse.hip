@@ -61,17 +61,26 @@ __device__ __forceinline__ void fill_lut(float *lut) {
                                        : sqrtf(2.0f * (float)(threadIdx.x - 1));
 }
 
+// The frame as a buffer resource: reads past its end return 0, never fault.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_buffer(
+    const FrameDesc &fd) {
+  return __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<void *>(fd.pixels), 0,
+      (int)(2u * fd.height * fd.width), 0x00020000);
+}
+
 // One pixel row of a lane's probe window. The lane holds eight columns from
 // an even column `ls`; m[k] keeps the window's columns of word k (0xFFFF a
 // half) and w5[k] weighs the cell's own columns. With p = v + 1 at unmasked
 // pixels and 0 at masked ones (v + 2 wraps the two sentinels to 0 and 1):
-// s += sum p, q += sum p^2, n7 += window count, n5 += cell count.
+// s += sum p, q += sum p^2, n7 += window count, n5 += cell count. `one` is
+// {1, 1} held in a register, so that min(p, 1) stays one packed minimum.
 __device__ __forceinline__ void scout_row(const uint32_t (&d)[4],
                                           const uint32_t (&m)[4],
-                                          const uint32_t (&w5)[4],
+                                          const uint32_t (&w5)[4], us2 one,
                                           uint32_t &s, uint32_t &q,
                                           uint32_t &n7, uint32_t &n5) {
-  const us2 two = {2, 2}, one = {1, 1};
+  const us2 two = {2, 2};
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const us2 a = __builtin_bit_cast(us2, d[k]) + two;
@@ -89,7 +98,8 @@ __device__ __forceinline__ void scout_row(const uint32_t (&d)[4],
 
 // Scout one item: cell columns 64 * strip + lane, cell rows
 // [16 * band, 16 * band + 16). VEC: the frame's rows are 4-byte aligned and
-// at least 8 pixels wide, so every lane reads its 8 columns as one load.
+// at least 8 pixels wide, so every lane reads its 8 columns as one 16-byte
+// load (a buffer load: the row's offset is a scalar).
 template <bool VEC>
 __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
                            const float *lut, uint64_t *masks) {
@@ -119,25 +129,32 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
         w5[k] |= 1u << (16 * h);
     }
   }
+  us2 one = {1, 1};
+  __asm__ volatile("" : "+v"(one));
   const int probe_at = pc - ls; // 0..7 when the probe is in the frame
   const uint16_t *pix = static_cast<const uint16_t *>(fd.pixels);
+  const __amdgpu_buffer_rsrc_t rsrc = frame_buffer(fd);
+  const uint32_t lane_at = 2u * (uint32_t)ls;
 
-  // a pixel row's eight columns; rows off the frame read as masked
-  const auto load_row = [&](int y, uint32_t (&d)[4]) {
-    if (y < 0 || y >= height) { // uniform
+  // a pixel row's eight columns; rows off the frame read as masked (CHECK:
+  // the item reaches the frame's top or bottom)
+  const auto load_row = [&]<bool CHECK>(int y, uint32_t (&d)[4]) {
+    if (CHECK && (y < 0 || y >= height)) { // uniform
 #pragma unroll
       for (int k = 0; k < 4; k++)
         d[k] = 0xFFFFFFFFu;
       return;
     }
-    const uint16_t *row = pix + (size_t)y * (size_t)width;
     if (VEC) {
-      const u32x4 v = *(global_q4)(row + ls);
-      d[0] = v.x;
-      d[1] = v.y;
-      d[2] = v.z;
-      d[3] = v.w;
+      typedef int i32x4 __attribute__((ext_vector_type(4)));
+      const i32x4 v = __builtin_amdgcn_raw_buffer_load_b128(
+          rsrc, (int)lane_at, (int)(2u * (uint32_t)y * (uint32_t)width), 0);
+      d[0] = (uint32_t)v.x;
+      d[1] = (uint32_t)v.y;
+      d[2] = (uint32_t)v.z;
+      d[3] = (uint32_t)v.w;
     } else {
+      const uint16_t *row = pix + (size_t)y * (size_t)width;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         uint32_t lo = 0xFFFFu, hi = 0xFFFFu;
@@ -149,36 +166,31 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
       }
     }
   };
+  const auto load_rows = [&]<bool CHECK>(int y, uint32_t (&rows)[5][4]) {
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+      load_row.template operator()<CHECK>(y + r, rows[r]);
+  };
 
   const int i0 = (int)band * kBandCells, i1 = min(i0 + kBandCells, ncr);
   // rows 5 i0 - 1 and 5 i0: the first window's top, carried like a tail
   uint32_t ts = 0, tq = 0, tn = 0, top5 = 0, unused = 0;
   {
     uint32_t d[4];
-    load_row(5 * i0 - 1, d);
-    scout_row(d, m, w5, ts, tq, tn, unused);
-    load_row(5 * i0, d);
-    scout_row(d, m, w5, ts, tq, tn, top5);
+    load_row.template operator()<true>(5 * i0 - 1, d);
+    scout_row(d, m, w5, one, ts, tq, tn, unused);
+    load_row.template operator()<true>(5 * i0, d);
+    scout_row(d, m, w5, one, ts, tq, tn, top5);
   }
-  uint32_t cur[5][4];
-#pragma unroll
-  for (int r = 0; r < 5; r++)
-    load_row(5 * i0 + 1 + r, cur[r]);
-  for (int i = i0; i < i1; i++) {
-    uint32_t nxt[5][4];
-    if (i + 1 < i1) {
-#pragma unroll
-      for (int r = 0; r < 5; r++)
-        load_row(5 * i + 6 + r, nxt[r]);
-    }
-    // window rows 5i-1 .. 5i+5: the carried pair, three core rows, a new pair
+  // cell row i from its rows 5i+1 .. 5i+5 and the carried pair 5i-1, 5i
+  const auto cell_row = [&](int i, const uint32_t (&rows)[5][4]) {
     uint32_t s = ts, q = tq, n7 = tn, n5 = top5;
     uint32_t s2 = 0, q2 = 0, n2 = 0, n5n = 0;
-    scout_row(cur[0], m, w5, s, q, n7, n5);
-    scout_row(cur[1], m, w5, s, q, n7, n5);
-    scout_row(cur[2], m, w5, s, q, n7, n5);
-    scout_row(cur[3], m, w5, s2, q2, n2, n5);
-    scout_row(cur[4], m, w5, s2, q2, n2, n5n);
+    scout_row(rows[0], m, w5, one, s, q, n7, n5);
+    scout_row(rows[1], m, w5, one, s, q, n7, n5);
+    scout_row(rows[2], m, w5, one, s, q, n7, n5);
+    scout_row(rows[3], m, w5, one, s2, q2, n2, n5);
+    scout_row(rows[4], m, w5, one, s2, q2, n2, n5n);
     s += s2;
     q += q2;
     n7 += n2;
@@ -187,15 +199,15 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
     tn = n2;
     top5 = n5n;
     // the probe's own value, from row 5i+2
-    uint32_t probe_row[4] = {cur[1][0], cur[1][1], cur[1][2], cur[1][3]};
+    uint32_t probe_row[4] = {rows[1][0], rows[1][1], rows[1][2], rows[1][3]};
     if (5 * i + 2 >= height) { // uniform: a last cell row of one or two rows
       // probe on the last row instead, window rows height-4 .. height-1
       s = q = n7 = 0;
       uint32_t d[4];
 #pragma unroll
       for (int r = 4; r >= 1; r--) {
-        load_row(height - r, d);
-        scout_row(d, m, w5, s, q, n7, unused);
+        load_row.template operator()<true>(height - r, d);
+        scout_row(d, m, w5, one, s, q, n7, unused);
       }
 #pragma unroll
       for (int k = 0; k < 4; k++)
@@ -220,12 +232,26 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
       fd.act[(size_t)i * fd.n_strips + strip] = bits;
       masks[i - i0] = bits;
     }
-#pragma unroll
-    for (int r = 0; r < 5; r++)
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        cur[r][k] = nxt[r][k];
-  }
+  };
+  // two row sets in turn: one loads while the other is summed
+  const auto sweep = [&]<bool CHECK>() {
+    uint32_t ra[5][4], rb[5][4];
+    load_rows.template operator()<CHECK>(5 * i0 + 1, ra);
+    for (int i = i0; i < i1; i += 2) {
+      if (i + 1 < i1)
+        load_rows.template operator()<CHECK>(5 * i + 6, rb);
+      cell_row(i, ra);
+      if (i + 1 >= i1)
+        break;
+      if (i + 2 < i1)
+        load_rows.template operator()<CHECK>(5 * i + 11, ra);
+      cell_row(i + 1, rb);
+    }
+  };
+  if (5 * i1 + 5 <= height) // rows 5 i0 + 1 .. 5 i1 + 4 all in the frame
+    sweep.template operator()<false>();
+  else
+    sweep.template operator()<true>();
   // the item's active cells, in its own segment of the list
   uint32_t total = 0;
   const uint32_t local = band * fd.n_strips + strip;
@@ -289,29 +315,26 @@ template <int N> __device__ __forceinline__ uint32_t right(uint32_t x) {
 template <int N> __device__ __forceinline__ uint32_t left(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x110 + N, 0xF, 0xF, true);
 }
-// x over lanes c-3 .. c+3 (valid at c = 3..11)
-__device__ __forceinline__ uint32_t across7(uint32_t x) {
+// x over lanes c .. c+6: the 7-wide window centred on lane c+3 (valid at
+// c = 0..8). Every step folds its lane move into an add (additions only: a
+// subtraction of a moved operand, v_subrev_u32_dpp, has been seen to take
+// the move from the other operand on this hardware).
+__device__ __forceinline__ uint32_t window7(uint32_t x) {
   const uint32_t a = x + right<1>(x); // c .. c+1
   const uint32_t b = a + right<2>(a); // c .. c+3
-  return left<3>(b) + right<1>(a) + right<3>(x);
+  return (b + right<4>(a)) + right<6>(x);
 }
-// x over lanes c-5 .. c+5 (valid at c = 5..9)
-__device__ __forceinline__ uint32_t across11(uint32_t x) {
+// x over lanes c .. c+10: the 11-wide window centred on lane c+5 (valid at
+// c = 0..4).
+__device__ __forceinline__ uint32_t window11(uint32_t x) {
   const uint32_t a = x + right<1>(x); // c .. c+1
   const uint32_t b = a + right<2>(a); // c .. c+3
   const uint32_t d = b + right<4>(b); // c .. c+7
-  return left<5>(d) + right<3>(a) + right<5>(x);
+  return (d + right<8>(a)) + right<10>(x);
 }
 
 constexpr uint64_t kCellLanes = 0x03E003E003E003E0ull; // lanes 5..9 a group
 
-// The frame as a buffer resource: reads past its end return 0, never fault.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_buffer(
-    const FrameDesc &fd) {
-  return __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<void *>(fd.pixels), 0,
-      (int)(2u * fd.height * fd.width), 0x00020000);
-}
 
 // The 15 rows of a lane's column around `cell`; off the frame, or on an
 // invalid group, pixels read as masked.
@@ -356,19 +379,20 @@ __device__ uint32_t erode_group(const FrameDesc &fd, bool valid, uint32_t cell,
     pn[r + 1] = pn[r] + x;
     pq[r + 1] = pq[r] + __umul24(x, x); // low 24 bits: v
   }
-  // mask0 or masked ("set") at pixel rows 5i-2+o, lanes 3..11: 7x7 windows
-  // (src/dext.cc:69-111)
+  // mask0 or masked ("set") at pixel rows 5i-2+o, columns 3..11: 7x7
+  // windows (src/dext.cc:69-111), tested on lane c for column c+3
   uint64_t h5[9];
 #pragma unroll
   for (int o = 0; o < 9; o++) {
-    const uint32_t ns = across7(pn[o + 7] - pn[o]);
-    const uint32_t sq = across7(pq[o + 7] - pq[o]);
+    const uint32_t ns = window7(pn[o + 7] - pn[o]);
+    const uint32_t sq = window7(pq[o + 7] - pq[o]);
     const uint32_t n = ns >> 26, sum = ns & 0x3FFFFFFu;
     const float fm = (float)n, fs = (float)sum, fq = (float)sq;
     const float lhs = fm * fq - fs * fs - fs * (fm - 1.0f);
     const float rhs = fs * 6.0f * lut[n];
-    const uint64_t b = ballot(v[o + 3] >= 0xFFFEu || lhs > rhs);
-    // set on all five columns c-2 .. c+2
+    const uint64_t b =
+        (ballot(lhs > rhs) << 3) | ballot(v[o + 3] >= 0xFFFEu);
+    // set on all five columns c-2 .. c+2 (bit c: column c of its group)
     h5[o] = b & (b << 1) & (b << 2) & (b >> 1) & (b >> 2);
   }
   const uint32_t at = 16u * (lane >> 4) + 5u;
@@ -441,43 +465,56 @@ __global__ __launch_bounds__(256) void erode_kernel(const FrameDesc *frames,
     erode_item(fd, local, lut);
 }
 
-// The Poisson test at a cell's candidates (src/dext.cc:146-215): 11x11 sums
-// of pixels neither masked nor eroded. The survivors, and for the lane's
-// pixel on each of the cell's rows: the mean, and count | value << 16.
-struct PoissonOut {
-  uint32_t s25;
-  float mean[5];
-  uint32_t pop[5];
+// What a lane needs for one cell's Poisson sums: its column's 15 pixels and
+// the E words of the cells above, at and below in its cell column.
+struct PoissonIn {
+  uint32_t v[15];
+  uint64_t word[3];
+  uint32_t eroded[3];
 };
 
-__device__ void poisson_group(const FrameDesc &fd, __amdgpu_buffer_rsrc_t rsrc,
-                              bool valid, uint64_t entry, PoissonOut &res) {
+__device__ __forceinline__ void poisson_load(const FrameDesc &fd,
+                                             __amdgpu_buffer_rsrc_t rsrc,
+                                             bool valid, uint32_t cell,
+                                             PoissonIn &in) {
+  const int c = (int)(lane_id() & 15u);
+  const int i = (int)(cell >> 16), j = (int)(cell & 0xFFFFu);
+  const int x = 5 * j - 5 + c, ncr = (int)fd.n_cell_rows;
+  const bool col = valid && c < 15 && x >= 0 && x < (int)fd.width;
+  const uint32_t jj = col ? (uint32_t)x / 5u : 0u;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const uint32_t ii = (uint32_t)min(max(i - 1 + a, 0), ncr - 1);
+    in.word[a] = fd.act[ii * fd.n_strips + (jj >> 6)];
+    in.eroded[a] = fd.ecell[ii * fd.n_cell_cols + jj];
+  }
+  load_column(fd, rsrc, valid, cell, c, in.v);
+}
+
+// The Poisson test at a cell's candidates (src/dext.cc:146-215): 11x11 sums
+// of pixels neither masked nor eroded. Returns the group's survivors; for
+// the lane's pixel on each of the cell's rows: the mean, and count | value
+// << 16.
+__device__ __forceinline__ uint32_t poisson_test(const FrameDesc &fd,
+                                                 bool valid, uint64_t entry,
+                                                 const PoissonIn &in,
+                                                 float (&mean)[5],
+                                                 uint32_t (&pop)[5]) {
   const uint32_t lane = lane_id();
   const int c = (int)(lane & 15u);
   const uint32_t cell = (uint32_t)entry, c25 = (uint32_t)(entry >> 32);
   const int i = (int)(cell >> 16), j = (int)(cell & 0xFFFFu);
-  const int width = (int)fd.width, ncr = (int)fd.n_cell_rows;
-  // E of the lane's column in the cells above, at and below: 0 where a cell
-  // is not active (E holds only in active cells) or off the frame
-  const int x = 5 * j - 5 + c;
-  const bool col = valid && c < 15 && x >= 0 && x < width;
-  const int jj = col ? x / 5 : 0, cc = col ? x - 5 * jj : 0;
-  uint64_t word[3];
-  uint32_t eroded[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    const int ii = min(max(i - 1 + a, 0), ncr - 1);
-    word[a] = fd.act[(uint32_t)ii * fd.n_strips + (uint32_t)(jj >> 6)];
-    eroded[a] = fd.ecell[(uint32_t)ii * fd.n_cell_cols + (uint32_t)jj];
-  }
-  uint32_t v[15];
-  load_column(fd, rsrc, valid, cell, c, v);
+  const int x = 5 * j - 5 + c, ncr = (int)fd.n_cell_rows;
+  const bool col = valid && c < 15 && x >= 0 && x < (int)fd.width;
+  const uint32_t jj = col ? (uint32_t)x / 5u : 0u;
+  const uint32_t cc = col ? (uint32_t)x - 5u * jj : 0u;
+  // E where the cell is active (E holds only in active cells) and in frame
   uint32_t e[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     const int ii = i - 1 + a;
-    e[a] = (col && ii >= 0 && ii < ncr && ((word[a] >> (jj & 63)) & 1u))
-               ? eroded[a]
+    e[a] = (col && ii >= 0 && ii < ncr && ((in.word[a] >> (jj & 63u)) & 1u))
+               ? in.eroded[a]
                : 0u;
   }
   // count above bit 23 and sum, packed, as prefix sums over rows
@@ -485,56 +522,96 @@ __device__ void poisson_group(const FrameDesc &fd, __amdgpu_buffer_rsrc_t rsrc,
   pn[0] = 0;
 #pragma unroll
   for (int r = 0; r < 15; r++) {
-    const uint32_t out = (e[r / 5] >> (5 * (r % 5) + cc)) & 1u;
-    const uint32_t xr = (v[r] < 0xFFFEu && !out) ? (v[r] | (1u << 23)) : 0u;
+    const uint32_t out = (e[r / 5] >> (5u * (r % 5) + cc)) & 1u;
+    const uint32_t xr =
+        (in.v[r] < 0xFFFEu && !out) ? (in.v[r] | (1u << 23)) : 0u;
     pn[r + 1] = pn[r] + xr;
   }
-  const uint32_t at = 16u * (lane >> 4) + 5u;
-  res.s25 = 0;
+  // tested on lane c (0..4) for the cell's column c, i.e. column c+5
+  const uint32_t at = 16u * (lane >> 4);
+  uint32_t s25 = 0;
 #pragma unroll
   for (int t = 0; t < 5; t++) {
-    const uint32_t ns = across11(pn[t + 11] - pn[t]);
+    const uint32_t ns = window11(pn[t + 11] - pn[t]);
     const uint32_t n = ns >> 23, sum = ns & 0x7FFFFFu;
-    const uint32_t p = v[t + 5];
+    const uint32_t p = right<5>(in.v[t + 5]);
     // src/dext.cc:203-206, in the reference's order and rounding
     const float m = n >= 2 ? (float)sum / (float)n : 0.0f;
     const bool ok = (float)p >= m + 3.0f * sqrtf(m);
-    const bool cand = c >= 5 && c <= 9 && ((c25 >> (5 * t + c - 5)) & 1u);
+    const bool cand = c <= 4 && ((c25 >> (5 * t + c)) & 1u);
     const uint64_t b = ballot(valid && cand && ok);
-    res.s25 |= (uint32_t)((b >> at) & 31u) << (5 * t);
-    res.mean[t] = m;
-    res.pop[t] = n | (p << 16);
+    s25 |= (uint32_t)((b >> at) & 31u) << (5 * t);
+    mean[t] = m;
+    pop[t] = n | (p << 16);
   }
+  return s25;
 }
 
-constexpr int kBatch = 4; // wave-steps of four cells per staging reservation
+// A wave's survivors wait here until a whole run of them takes its place in
+// staging under one atomic: each cell's run in its pixel order.
+constexpr int kStash = 512;
+constexpr int kStashCells = 64;
+struct Stash {
+  float mean[kStash];
+  uint32_t pop[kStash];   // count | value << 16
+  uint32_t index[kStash]; // pixel index in the frame
+  uint32_t cell[kStashCells];
+  uint32_t s25[kStashCells];
+  uint32_t at[kStashCells]; // the cell's run in the stash
+};
 
-// Survivors of the four cells of a wave-step: this group's count, the
-// groups' total, and the count of the groups before this one.
-__device__ __forceinline__ uint32_t step_counts(uint32_t s25, uint32_t grp,
-                                                uint32_t &before) {
-  const uint32_t k = (uint32_t)__builtin_popcount(s25);
-  const uint32_t k0 = (uint32_t)__shfl((int)k, 0, 64);
-  const uint32_t k1 = (uint32_t)__shfl((int)k, 16, 64);
-  const uint32_t k2 = (uint32_t)__shfl((int)k, 32, 64);
-  const uint32_t k3 = (uint32_t)__shfl((int)k, 48, 64);
-  before = (grp > 0 ? k0 : 0u) + (grp > 1 ? k1 : 0u) + (grp > 2 ? k2 : 0u);
-  return k0 + k1 + k2 + k3;
+__device__ __forceinline__ void flush(const FrameDesc &fd, Stash &st,
+                                      uint32_t &n, uint32_t &cells) {
+  const uint32_t lane = lane_id();
+  if (n) {
+    uint32_t base = 0;
+    if (lane == 0)
+      base = atomicAdd(fd.counter, n);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    for (uint32_t k = lane; k < cells; k += 64) {
+      const uint32_t cell = st.cell[k], s25 = st.s25[k];
+      const uint32_t i = cell >> 16;
+      const uint32_t ci = i * fd.n_cell_cols + (cell & 0xFFFFu);
+      fd.scell[ci] = s25;
+      fd.sbase[ci] = base + st.at[k];
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+        const uint32_t row = (s25 >> (5 * t)) & 31u;
+        if (row)
+          atomicAdd(&fd.rowsurv[5 * i + t], (uint32_t)__builtin_popcount(row));
+      }
+    }
+    for (uint32_t e = lane; e < n; e += 64) {
+      const uint32_t dst = base + e;
+      if (dst < fd.capacity) {
+        sf_signal_pixel px;
+        px.index = st.index[e];
+        px.value = st.pop[e] >> 16;
+        px.background = st.mean[e];
+        px.population = (uint16_t)(st.pop[e] & 0xFFFFu);
+        px.reserved = 0;
+        fd.staging[dst] = px;
+      }
+    }
+  }
+  n = 0;
+  cells = 0;
 }
 
 // One item's candidate cells: each 64 of its active cells are compacted to
-// their candidate cells (in LDS), tested four at a time, and every kBatch
-// steps' survivors go to staging under one atomic, each cell's run in its
-// pixel order.
-__device__ __forceinline__ void poisson_item(const FrameDesc &fd, uint32_t local,
-                             uint64_t *slots) {
+// their candidate cells (in LDS) and tested four at a time, the next four
+// cells' inputs fetched while the current four are tested; survivors gather
+// in the stash.
+__device__ __forceinline__ void poisson_item(const FrameDesc &fd,
+                                             uint32_t local, uint64_t *slots,
+                                             Stash &st) {
   const uint32_t n = fd.icount[local];
   const uint32_t lane = lane_id(), grp = lane >> 4;
   const int c = (int)(lane & 15u);
   const uint32_t *cells = fd.alist + (size_t)local * kSegment;
   const uint32_t *cand = fd.ccand + (size_t)local * kSegment;
-  const uint32_t width = fd.width;
   const __amdgpu_buffer_rsrc_t rsrc = frame_buffer(fd);
+  uint32_t stashed = 0, stashed_cells = 0;
   for (uint32_t c0 = 0; c0 < n; c0 += 64) {
     const bool here = c0 + lane < n;
     const uint32_t cell = here ? cells[c0 + lane] : 0u;
@@ -544,82 +621,80 @@ __device__ __forceinline__ void poisson_item(const FrameDesc &fd, uint32_t local
       slots[count_below(has)] = (uint64_t)cell | ((uint64_t)c25 << 32);
     const uint32_t nc = (uint32_t)__builtin_popcountll(has);
     const uint32_t steps = (nc + 3) / 4;
-    for (uint32_t g0 = 0; g0 < steps; g0 += kBatch) {
-      PoissonOut res[kBatch];
-      uint32_t cells_of[kBatch];
-      uint32_t total = 0;
-#pragma unroll
-      for (int b = 0; b < kBatch; b++) {
-        const uint32_t at = 4 * (g0 + b) + grp;
-        const bool valid = g0 + b < steps && at < nc;
-        const uint64_t entry = valid ? slots[at] : 0ull;
-        cells_of[b] = (uint32_t)entry;
-        res[b].s25 = 0;
-        if (g0 + b < steps) // uniform
-          poisson_group(fd, rsrc, valid, entry, res[b]);
-        uint32_t unused;
-        total += step_counts(res[b].s25, grp, unused);
-      }
-      if (total == 0)
-        continue;
-      uint32_t base = 0;
-      if (lane == 0)
-        base = atomicAdd(fd.counter, total);
-      base = (uint32_t)__shfl((int)base, 0, 64);
-#pragma unroll
-      for (int b = 0; b < kBatch; b++) {
-        uint32_t before;
-        const uint32_t sum = step_counts(res[b].s25, grp, before);
-        const uint32_t mine = base + before;
-        base += sum;
-        const uint32_t s25 = res[b].s25;
-        if (!s25)
-          continue;
-        const uint32_t i = cells_of[b] >> 16, j = cells_of[b] & 0xFFFFu;
-        if (c == 0) {
-          const uint32_t ci = i * fd.n_cell_cols + j;
-          fd.scell[ci] = s25;
-          fd.sbase[ci] = mine;
+    if (!steps)
+      continue;
+    bool valid = grp < nc;
+    uint64_t entry = valid ? slots[grp] : 0ull;
+    PoissonIn in;
+    poisson_load(fd, rsrc, valid, (uint32_t)entry, in);
+    for (uint32_t g = 0; g < steps; g++) {
+      const uint32_t next = 4 * (g + 1) + grp;
+      const bool next_valid = next < nc;
+      const uint64_t next_entry = next_valid ? slots[next] : 0ull;
+      PoissonIn ahead;
+      if (g + 1 < steps)
+        poisson_load(fd, rsrc, next_valid, (uint32_t)next_entry, ahead);
+      float mean[5];
+      uint32_t pop[5];
+      const uint32_t s25 = poisson_test(fd, valid, entry, in, mean, pop);
+      // this step's survivors: counts of the four cells, in group order
+      const uint32_t k = (uint32_t)__builtin_popcount(s25);
+      const uint32_t k0 = (uint32_t)__shfl((int)k, 0, 64);
+      const uint32_t k1 = (uint32_t)__shfl((int)k, 16, 64);
+      const uint32_t k2 = (uint32_t)__shfl((int)k, 32, 64);
+      const uint32_t k3 = (uint32_t)__shfl((int)k, 48, 64);
+      const uint32_t total = k0 + k1 + k2 + k3;
+      if (total) {
+        if (stashed + total > (uint32_t)kStash ||
+            stashed_cells + 4 > (uint32_t)kStashCells)
+          flush(fd, st, stashed, stashed_cells);
+        const uint32_t before =
+            (grp > 0 ? k0 : 0u) + (grp > 1 ? k1 : 0u) + (grp > 2 ? k2 : 0u);
+        const uint32_t run = stashed + before;
+        // the cell's record, by its group leader
+        const uint64_t lead = ballot(c == 0 && k != 0);
+        if (c == 0 && k) {
+          const uint32_t slot = stashed_cells + count_below(lead);
+          st.cell[slot] = (uint32_t)entry;
+          st.s25[slot] = s25;
+          st.at[slot] = run;
+        }
+        if (c <= 4) {
+          const uint32_t cell_i = (uint32_t)entry >> 16;
+          const uint32_t cell_j = (uint32_t)entry & 0xFFFFu;
 #pragma unroll
           for (int t = 0; t < 5; t++) {
-            const uint32_t row = (s25 >> (5 * t)) & 31u;
-            if (row)
-              atomicAdd(&fd.rowsurv[5 * i + t],
-                        (uint32_t)__builtin_popcount(row));
+            const int bit = 5 * t + c;
+            if (!((s25 >> bit) & 1u))
+              continue;
+            const uint32_t e =
+                run + (uint32_t)__builtin_popcount(s25 & ((1u << bit) - 1u));
+            st.mean[e] = mean[t];
+            st.pop[e] = pop[t];
+            st.index[e] = (5 * cell_i + t) * fd.width + 5 * cell_j + (uint32_t)c;
           }
         }
-        if (c < 5 || c > 9)
-          continue;
-#pragma unroll
-        for (int t = 0; t < 5; t++) {
-          const int bit = 5 * t + c - 5;
-          if (!((s25 >> bit) & 1u))
-            continue;
-          const uint32_t dst =
-              mine + (uint32_t)__builtin_popcount(s25 & ((1u << bit) - 1u));
-          if (dst < fd.capacity) {
-            sf_signal_pixel px;
-            px.index = (5 * i + t) * width + 5 * j + (uint32_t)c - 5;
-            px.value = res[b].pop[t] >> 16;
-            px.background = res[b].mean[t];
-            px.population = (uint16_t)(res[b].pop[t] & 0xFFFFu);
-            px.reserved = 0;
-            fd.staging[dst] = px;
-          }
-        }
+        stashed += total;
+        stashed_cells += (uint32_t)__builtin_popcountll(lead);
       }
+      in = ahead;
+      entry = next_entry;
+      valid = next_valid;
     }
   }
+  flush(fd, st, stashed, stashed_cells);
 }
 
 __global__ __launch_bounds__(256) void poisson_kernel(const FrameDesc *frames,
                                                       int n_frames,
                                                       uint32_t items) {
   __shared__ uint64_t slots[4][64];
+  __shared__ Stash stash[4];
   FrameDesc fd;
   uint32_t local;
+  const uint32_t wave = threadIdx.x >> 6;
   if (block_item(frames, n_frames, items, fd, local))
-    poisson_item(fd, local, slots[threadIdx.x >> 6]);
+    poisson_item(fd, local, slots[wave], stash[wave]);
 }
 
 // Inclusive sum of x over the lanes up to this one (DPP: within rows of 16,
