@@ -32,7 +32,7 @@ void sparse_layout(uint32_t, uint32_t, FrameDesc &);
 size_t sparse_bytes(uint32_t, uint32_t, uint32_t);
 void sparse_carve(uint8_t *, FrameDesc &);
 hipError_t launch_sparse_threshold(const FrameDesc *, int, uint32_t,
-                                   hipStream_t);
+                                   hipStream_t, hipEvent_t *);
 hipError_t launch_sparse_order(const FrameDesc *, int, uint32_t, hipStream_t);
 hipError_t launch_sparse(const FrameDesc *, int, uint32_t, uint32_t,
                          hipStream_t);
@@ -290,13 +290,14 @@ int upload_desc(sf_plan *p, const void *const *frames) {
 int enqueue(sf_plan *p, bool timing) {
   const Kernel kernel = kernel_for(p, p->form);
   if (kernel == kSparse) { // clears its own counters and orders its lists
-    if (timing)
-      SF_CHECK(hipEventRecord(p->ev[0], p->stream), "hipEventRecord");
+    // ev[0] after the clearing kernel, ev[1] after the scout, ev[2] = ev[3]
+    // after the Poisson pass, ev[4] after the ordering
     SF_CHECK(sf::launch_sparse_threshold(p->d_desc, p->n, p->items,
-                                         p->stream),
+                                         p->stream,
+                                         timing ? p->ev : nullptr),
              "launching the sparse threshold");
     if (timing)
-      for (int e = 1; e <= 3; e++)
+      for (int e = 2; e <= 3; e++)
         SF_CHECK(hipEventRecord(p->ev[e], p->stream), "hipEventRecord");
     SF_CHECK(sf::launch_sparse_order(p->d_desc, p->n, p->max_cell_rows,
                                      p->stream),
@@ -692,6 +693,23 @@ int sf_hip_plan_last_times(sf_plan *p, double *threshold_ms,
     *threshold_ms = a;
   if (order_ms)
     *order_ms = b;
+  return SF_OK;
+}
+
+int sf_hip_plan_last_pass_times(sf_plan *p, double *first_ms,
+                                double *rest_ms, double *order_ms) {
+  if (!p || !p->timing)
+    return bad("sf_hip_plan_last_pass_times: timing is off");
+  float a = 0.0f, b = 0.0f, c = 0.0f;
+  SF_CHECK(hipEventElapsedTime(&a, p->ev[0], p->ev[1]), "event time");
+  SF_CHECK(hipEventElapsedTime(&b, p->ev[1], p->ev[3]), "event time");
+  SF_CHECK(hipEventElapsedTime(&c, p->ev[3], p->ev[4]), "event time");
+  if (first_ms)
+    *first_ms = a;
+  if (rest_ms)
+    *rest_ms = b;
+  if (order_ms)
+    *order_ms = c;
   return SF_OK;
 }
 

@@ -894,14 +894,23 @@ void sparse_carve(uint8_t *base, FrameDesc &d) {
 // The sparse form on `stream` in two parts: the threshold (clear, scout,
 // erode, Poisson) and the ordering (row prefix, places, copy). `items`: the
 // launch's scout items; `max_cell_rows`: the most cell rows of any frame.
+// `marks` (optional, two events): recorded after the clearing kernel and
+// after the scout, to time the scout on its own.
 hipError_t launch_sparse_threshold(const FrameDesc *frames, int n_frames,
-                                   uint32_t items, hipStream_t stream) {
+                                   uint32_t items, hipStream_t stream,
+                                   hipEvent_t *marks) {
   if (n_frames <= 0)
     return hipSuccess;
   zero_kernel<<<n_frames, 256, 0, stream>>>(frames);
-  if (items) {
-    const uint32_t blocks = ((items + 3) / 4 + 7) & ~7u;
+  hipError_t e;
+  if (marks && (e = hipEventRecord(marks[0], stream)) != hipSuccess)
+    return e;
+  const uint32_t blocks = ((items + 3) / 4 + 7) & ~7u;
+  if (items)
     scout_kernel<<<blocks, 256, 0, stream>>>(frames, n_frames, items);
+  if (marks && (e = hipEventRecord(marks[1], stream)) != hipSuccess)
+    return e;
+  if (items) {
     erode_kernel<<<blocks, 256, 0, stream>>>(frames, n_frames, items);
     poisson_kernel<<<blocks, 256, 0, stream>>>(frames, n_frames, items);
   }
@@ -923,7 +932,7 @@ hipError_t launch_sparse_order(const FrameDesc *frames, int n_frames,
 hipError_t launch_sparse(const FrameDesc *frames, int n_frames, uint32_t items,
                          uint32_t max_cell_rows, hipStream_t stream) {
   const hipError_t e =
-      launch_sparse_threshold(frames, n_frames, items, stream);
+      launch_sparse_threshold(frames, n_frames, items, stream, nullptr);
   if (e != hipSuccess)
     return e;
   return launch_sparse_order(frames, n_frames, max_cell_rows, stream);

@@ -116,6 +116,9 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_plan_set_timing", _i, _p, _i)
     fn("sf_hip_plan_last_times", _i, _p, ctypes.POINTER(ctypes.c_double),
        ctypes.POINTER(ctypes.c_double))
+    fn("sf_hip_plan_last_pass_times", _i, _p,
+       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+       ctypes.POINTER(ctypes.c_double))
     fn("sf_hip_plan_stream", _p, _p)
     fn("sf_hip_synth_frames", _i, _p, _i, _u32, _u32, _u32, ctypes.c_uint64,
        ctypes.c_double, _i, ctypes.c_double, _p, _u32, ctypes.c_double)
@@ -411,6 +414,15 @@ class Plan:
         _check(lib().sf_hip_plan_last_times(self.handle, ctypes.byref(a),
                                             ctypes.byref(b)), "last_times")
         return a.value, b.value
+
+    def last_pass_times(self):
+        """(first threshold kernel, the threshold's other kernels, ordering)
+        of the last timed run, in ms."""
+        a, b, c = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+        _check(lib().sf_hip_plan_last_pass_times(
+            self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+            "last_pass_times")
+        return a.value, b.value, c.value
 
     def close(self) -> None:
         if self.handle:

@@ -182,6 +182,12 @@ uint32_t sf_hip_plan_capacity(sf_plan *plan, int frame);
 int sf_hip_plan_set_timing(sf_plan *plan, int on);
 int sf_hip_plan_last_times(sf_plan *plan, double *threshold_ms,
                            double *order_ms);
+/* The last run's threshold split by pass (ms): the first kernel (the sparse
+ * form's scout, or the single threshold kernel of the other fused forms),
+ * the threshold's remaining kernels (the sparse form's erosion and Poisson
+ * passes), and the ordering. Any output may be NULL. */
+int sf_hip_plan_last_pass_times(sf_plan *plan, double *first_ms,
+                                double *rest_ms, double *order_ms);
 /* The plan's stream, as an opaque handle (hipStream_t). */
 void *sf_hip_plan_stream(sf_plan *plan);
 
