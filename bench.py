@@ -7,9 +7,10 @@ frames, Poisson background 0.3, ~1500 Gaussian spots a frame (sigma 0.8-2
 px, 1-5 frames, peaks up to 20,000), masked module gaps at the 514 x 1030
 pitch and 0.05% hot pixels -- generated on the device from a seed. A step is
 one chunk of C frames through the hot path: the threshold with ordered
-compaction (one graph replay), then the device 3-D labelling and centroid
-pass for the chunk. `value` is frames/s over the timed steps with the inputs
-already in HBM.
+compaction (one graph replay), and the device 3-D labelling and centroid
+pass for the chunk; two plans alternate, so a chunk's labelling overlaps the
+next chunk's threshold. `value` is frames/s over the timed steps (all of
+them thresholded and labelled) with the inputs already in HBM.
 
 Also reported: the roofline of the dominant kernel, the sparse threshold's
 scout (it reads every pixel of the chunk once: algorithmic bytes per launch
@@ -371,31 +372,51 @@ def main() -> int:
     path = ("threshold + ordered compaction (one graph replay)"
             + (" + 3-D labelling and centroids" if labeller else ""))
 
-    # One captured graph (zero counters, threshold, ordering) replayed per
-    # chunk; binding a chunk only rewrites the frame descriptors it reads.
-    plan.capture(ptr[:chunk])
+    # Two plans in turn, each with one captured graph (threshold, ordering)
+    # replayed per chunk; binding a chunk only rewrites the frame descriptors
+    # it reads. Step s replays chunk s, then hands chunk s-1 (done or nearly)
+    # to the labeller, so the host never waits on the threshold in flight
+    # and chunk s-1's labelling overlaps chunk s's threshold. The labeller's
+    # gather runs on the producing plan's stream, ahead of that plan's next
+    # replay.
+    plans = [plan, sf.Plan(2, [(HEIGHT, WIDTH)] * chunk, band_rows=args.band)]
+    for p in plans:
+        p.capture(ptr[:chunk])
+    pending = []  # the replayed chunk not yet labelled: (plan, first frame)
 
     def step(s):
         c = s % n_chunks
-        plan.bind(ptr[c * chunk:(c + 1) * chunk])
-        plan.replay()
-        if labeller is not None:
+        p = plans[s % 2]
+        p.bind(ptr[c * chunk:(c + 1) * chunk])
+        p.replay()
+        if labeller is not None and pending:
             # frame numbers keep ascending as the series is replayed
-            labeller.add_plan(plan, first_frame=s * chunk)
+            done, first = pending.pop()
+            labeller.add_plan(done, first_frame=first)
+        pending.append((p, s * chunk))
+
+    def drain():
+        if labeller is not None and pending:
+            done, first = pending.pop()
+            labeller.add_plan(done, first_frame=first)
+        pending.clear()
+        for p in plans:
+            p.wait()
+        if labeller is not None:
+            labeller.wait()
 
     for s in range(args.warmup):
         step(s)
-    plan.wait()
+    drain()
     sf.lib().sf_hip_synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         step(args.warmup + s)
-    plan.wait()
-    if labeller is not None:
-        labeller.wait()
+    drain()
     sf.lib().sf_hip_synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps * chunk / elapsed
+    plans[1].close()
 
     # The scout alone, and the threshold's passes together, from HIP events
     # on the plan's stream.

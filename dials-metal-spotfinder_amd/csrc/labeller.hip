@@ -367,10 +367,9 @@ __device__ int finish_spot(const LabelArgs &a, sf_spot &spot, double sum,
 // One component by one thread: the reference's emit() (src/dials_spots.cc:
 // 221-360) operation for operation, every sum in member order.
 __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
-                               const uint32_t *order,
-                               const int32_t *node_frame,
-                               const uint32_t *node_index,
-                               const uint32_t *node_value, sf_spot &spot) {
+                               const int32_t *mem_frame,
+                               const uint32_t *mem_index,
+                               const uint32_t *mem_value, sf_spot &spot) {
   spot.n_signal = n;
   double sum = 0.0, sum_sq = 0.0;
   double weighted[3] = {0.0, 0.0, 0.0};
@@ -378,11 +377,11 @@ __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
   uint32_t peak_value = 0;
   double peak[3] = {0.0, 0.0, 0.0};
   for (uint32_t k = 0; k < n; k++) {
-    const uint32_t g = order[s0 + k];
-    const uint32_t idx = node_index[g];
+    const uint32_t g = s0 + k;
+    const uint32_t idx = mem_index[g];
     const int32_t x = (int32_t)(idx % a.width);
     const int32_t y = (int32_t)(idx / a.width);
-    const int32_t z = node_frame[g];
+    const int32_t z = mem_frame[g];
     if (k == 0) {
       x0 = x;
       x1 = x + 1;
@@ -398,15 +397,15 @@ __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
       z0 = z < z0 ? z : z0;
       z1 = z + 1 > z1 ? z + 1 : z1;
     }
-    const double value = (double)node_value[g];
+    const double value = (double)mem_value[g];
     const double position[3] = {(double)x + 0.5, (double)y + 0.5,
                                 (double)z + 0.5};
     sum += value;
     sum_sq += value * value;
     for (int axis = 0; axis < 3; axis++)
       weighted[axis] += value * position[axis];
-    if (node_value[g] > peak_value) {
-      peak_value = node_value[g];
+    if (mem_value[g] > peak_value) {
+      peak_value = mem_value[g];
       peak[0] = position[0];
       peak[1] = position[1];
       peak[2] = position[2];
@@ -421,13 +420,13 @@ __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
   return finish_spot(a, spot, sum, sum_sq, weighted, peak,
                      [&](const double (&centre)[3], double (&delta_sq)[3]) {
                        for (uint32_t k = 0; k < n; k++) {
-                         const uint32_t g = order[s0 + k];
-                         const uint32_t idx = node_index[g];
-                         const double value = (double)node_value[g];
+                         const uint32_t g = s0 + k;
+                         const uint32_t idx = mem_index[g];
+                         const double value = (double)mem_value[g];
                          const double position[3] = {
                              (double)(idx % a.width) + 0.5,
                              (double)(idx / a.width) + 0.5,
-                             (double)node_frame[g] + 0.5};
+                             (double)mem_frame[g] + 0.5};
                          for (int axis = 0; axis < 3; axis++) {
                            const double delta = position[axis] - centre[axis];
                            delta_sq[axis] += value * delta * delta;
@@ -459,6 +458,24 @@ template <typename T> __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
+// The closed members' frame, index and value in emission order (the sorted
+// order), gathered once: the finalise and pixel-table passes then read them
+// in order instead of through `order`.
+__global__ void gather_members(const uint32_t *n_closed, const uint32_t *order,
+                               const int32_t *node_frame,
+                               const uint32_t *node_index,
+                               const uint32_t *node_value, int32_t *mem_frame,
+                               uint32_t *mem_index, uint32_t *mem_value) {
+  const uint32_t end = *n_closed;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t g = order[i];
+    mem_frame[i] = node_frame[g];
+    mem_index[i] = node_index[g];
+    mem_value[i] = node_value[g];
+  }
+}
+
 // One wave per closed component, in emission order, with the result the
 // sequential emit() gives bit for bit:
 //  * bbox, peak (first strict maximum in member order) and the sums are
@@ -471,9 +488,9 @@ template <typename T> __device__ __forceinline__ T wave_max(T v) {
 //    member order, as src/dials_spots.cc:304-316 adds them.
 __global__ __launch_bounds__(256) void finalise(
     LabelArgs a, const uint32_t *starts, const uint32_t *n_starts,
-    const uint32_t *n_closed, const uint32_t *order,
-    const int32_t *node_frame, const uint32_t *node_index,
-    const uint32_t *node_value, SpotOut *out, unsigned long long *counts) {
+    const uint32_t *n_closed, const int32_t *mem_frame,
+    const uint32_t *mem_index, const uint32_t *mem_value, SpotOut *out,
+    unsigned long long *counts) {
   __shared__ double terms[4][3 * 64]; // a wave's variance terms, by axis
   const uint32_t comps = *n_starts;
   const uint32_t end_all = *n_closed;
@@ -505,10 +522,9 @@ __global__ __launch_bounds__(256) void finalise(
         m_val[j] = 0;
         m_z[j] = 0;
         if (held && k < n) {
-          const uint32_t g = order[s0 + k];
-          m_idx[j] = node_index[g];
-          m_val[j] = node_value[g];
-          m_z[j] = node_frame[g];
+          m_idx[j] = mem_index[s0 + k];
+          m_val[j] = mem_value[s0 + k];
+          m_z[j] = mem_frame[s0 + k];
         }
       }
       // pass 1: bbox, peak, integer sums, bounds
@@ -546,10 +562,8 @@ __global__ __launch_bounds__(256) void finalise(
             visit(k, m_idx[j], m_val[j], m_z[j]);
         }
       } else {
-        for (uint32_t k = lane; k < n; k += 64) {
-          const uint32_t g = order[s0 + k];
-          visit(k, node_index[g], node_value[g], node_frame[g]);
-        }
+        for (uint32_t k = lane; k < n; k += 64)
+          visit(k, mem_index[s0 + k], mem_value[s0 + k], mem_frame[s0 + k]);
       }
       x0 = wave_min(x0);
       x1 = wave_max(x1);
@@ -570,8 +584,8 @@ __global__ __launch_bounds__(256) void finalise(
       const bool exact = cnt * top * top < bound && cnt * top * reach < bound;
       if (!exact) {
         if (lane == 0)
-          kind = emit_sequential(a, s0, n, order, node_frame, node_index,
-                                 node_value, spot);
+          kind = emit_sequential(a, s0, n, mem_frame, mem_index, mem_value,
+                                 spot);
         kind = __shfl(kind, 0, 64);
       } else {
         sv = wave_sum(sv);
@@ -591,11 +605,10 @@ __global__ __launch_bounds__(256) void finalise(
                                     (double)sz * 0.5};
         double peak[3] = {0.0, 0.0, 0.0};
         if (vtop > 0) {
-          const uint32_t g = order[s0 + ktop];
-          const uint32_t idx = node_index[g];
+          const uint32_t idx = mem_index[s0 + ktop];
           peak[0] = (double)(idx % a.width) + 0.5;
           peak[1] = (double)(idx / a.width) + 0.5;
-          peak[2] = (double)node_frame[g] + 0.5;
+          peak[2] = (double)mem_frame[s0 + ktop] + 0.5;
         }
         // the variance terms of one 64-member chunk, added in member order
         auto chunk_terms = [&](uint32_t k0, uint32_t idx, uint32_t v,
@@ -642,10 +655,9 @@ __global__ __launch_bounds__(256) void finalise(
                   uint32_t idx = 0, v = 0;
                   int32_t z = 0;
                   if (k0 + lane < n) {
-                    const uint32_t g = order[s0 + k0 + lane];
-                    idx = node_index[g];
-                    v = node_value[g];
-                    z = node_frame[g];
+                    idx = mem_index[s0 + k0 + lane];
+                    v = mem_value[s0 + k0 + lane];
+                    z = mem_frame[s0 + k0 + lane];
                   }
                   chunk_terms(k0, idx, v, z, centre, delta_sq);
                 }
@@ -683,20 +695,20 @@ __global__ __launch_bounds__(256) void finalise(
   }
 }
 
+// Per component: accepted << 32 | kept members, so that one exclusive scan
+// gives both the spot rank (high half) and the pixel rank (low half; the
+// members of a chunk stay below 2^32).
 __global__ void spot_flags(const SpotOut *res, const uint32_t *n_comps,
-                           uint32_t *accepted, uint32_t *members) {
+                           unsigned long long *packed) {
   const uint32_t comps = *n_comps;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
-       c += gridDim.x * blockDim.x) {
-    accepted[c] = res[c].accepted;
-    members[c] = res[c].members;
-  }
+       c += gridDim.x * blockDim.x)
+    packed[c] = ((unsigned long long)res[c].accepted << 32) | res[c].members;
 }
 
 // Appends the kept spots after what earlier chunks wrote.
 __global__ void write_spots(const SpotOut *res, const uint32_t *n_comps,
-                            const uint32_t *spot_rank,
-                            const uint32_t *pixel_rank,
+                            const unsigned long long *rank,
                             const unsigned long long *totals,
                             sf_spot *spots) {
   const uint32_t comps = *n_comps;
@@ -706,8 +718,8 @@ __global__ void write_spots(const SpotOut *res, const uint32_t *n_comps,
     if (!res[c].accepted)
       continue;
     sf_spot s = res[c].spot;
-    s.first = pixel_base + pixel_rank[c];
-    spots[spot_base + spot_rank[c]] = s;
+    s.first = pixel_base + (uint32_t)rank[c];
+    spots[spot_base + (rank[c] >> 32)] = s;
   }
 }
 
@@ -717,11 +729,11 @@ __global__ void write_spots(const SpotOut *res, const uint32_t *n_comps,
 __global__ void write_pixels(const SpotOut *res, const uint32_t *n_closed,
                              const uint32_t *start_flag,
                              const uint32_t *start_rank,
-                             const uint32_t *starts, const uint32_t *order,
-                             const uint32_t *pixel_rank,
-                             const int32_t *node_frame,
-                             const uint32_t *node_index,
-                             const uint32_t *node_value,
+                             const uint32_t *starts,
+                             const unsigned long long *rank,
+                             const int32_t *mem_frame,
+                             const uint32_t *mem_index,
+                             const uint32_t *mem_value,
                              const unsigned long long *totals,
                              sf_pixel *pixels) {
   const uint32_t end = *n_closed;
@@ -731,25 +743,23 @@ __global__ void write_pixels(const SpotOut *res, const uint32_t *n_closed,
     const uint32_t c = start_rank[i] + start_flag[i] - 1;
     if (!res[c].accepted)
       continue;
-    const uint32_t g = order[i];
     sf_pixel p;
-    p.frame = node_frame[g];
-    p.index = node_index[g];
-    p.value = node_value[g];
-    pixels[pixel_base + pixel_rank[c] + (i - starts[c])] = p;
+    p.frame = mem_frame[i];
+    p.index = mem_index[i];
+    p.value = mem_value[i];
+    pixels[pixel_base + (uint32_t)rank[c] + (i - starts[c])] = p;
   }
 }
 
 __global__ void advance_totals(const uint32_t *n_comps,
-                               const uint32_t *spot_rank,
-                               const uint32_t *pixel_rank,
+                               const unsigned long long *rank,
                                const SpotOut *res,
                                unsigned long long *totals) {
   const uint32_t comps = *n_comps;
   if (comps) {
     const uint32_t last = comps - 1;
-    totals[0] += spot_rank[last] + res[last].accepted;
-    totals[1] += pixel_rank[last] + res[last].members;
+    totals[0] += (rank[last] >> 32) + res[last].accepted;
+    totals[1] += (uint32_t)rank[last] + res[last].members;
   }
 }
 
@@ -875,7 +885,7 @@ struct sf_labeller {
   sf::DevArray<uint32_t> values, order, open_flag, open_rank, start_flag,
       start_rank, starts;
   sf::DevArray<sf::SpotOut> results;
-  sf::DevArray<uint32_t> acc_flag, acc_rank, mem_count, mem_rank;
+  sf::DevArray<unsigned long long> spot_pack, spot_rank;
   sf::DevArray<uint8_t> temp;
   sf::DevArray<sf::ChunkDesc> chunk;
   uint32_t *d_scalars = nullptr; // n_starts, n_closed
@@ -911,6 +921,26 @@ int lab_fail(const char *what, hipError_t e) {
 uint32_t grid_for(size_t n) {
   const size_t blocks = (n + 255) / 256;
   return (uint32_t)std::min<size_t>(std::max<size_t>(blocks, 1), 65535);
+}
+
+// The component sort on 32-bit keys: keys of 25 to 30 bits go in three
+// 10-bit passes instead of four 8-bit ones.
+using Sort10 = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<
+        rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 10,
+        rocprim::block_radix_rank_algorithm::match>>;
+
+hipError_t sort32(void *temp, size_t &bytes, const uint32_t *keys_in,
+                  uint32_t *keys_out, const uint32_t *values_in,
+                  uint32_t *values_out, uint32_t n, int bits,
+                  hipStream_t s) {
+  if (bits > 24 && bits <= 30)
+    return rocprim::radix_sort_pairs<Sort10>(temp, bytes, keys_in, keys_out,
+                                             values_in, values_out, n, 0, bits,
+                                             s);
+  return rocprim::radix_sort_pairs(temp, bytes, keys_in, keys_out, values_in,
+                                   values_out, n, 0, bits, s);
 }
 
 // Label one chunk whose frame lists are device-resident. `frames` holds
@@ -1005,10 +1035,8 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(l->start_rank.ensure(n), "alloc");
   LAB_CHECK(l->starts.ensure(n), "alloc");
   LAB_CHECK(l->results.ensure(n), "alloc");
-  LAB_CHECK(l->acc_flag.ensure(n), "alloc");
-  LAB_CHECK(l->acc_rank.ensure(n), "alloc");
-  LAB_CHECK(l->mem_count.ensure(n), "alloc");
-  LAB_CHECK(l->mem_rank.ensure(n), "alloc");
+  LAB_CHECK(l->spot_pack.ensure(n), "alloc");
+  LAB_CHECK(l->spot_rank.ensure(n), "alloc");
   const int in = l->carry_side, outside = 1 - l->carry_side;
   LAB_CHECK(l->carry_frame[outside].ensure(n), "alloc");
   LAB_CHECK(l->carry_index[outside].ensure(n), "alloc");
@@ -1027,9 +1055,8 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   uint32_t *keys32 = reinterpret_cast<uint32_t *>(l->keys.ptr);
   uint32_t *keys32_sorted = reinterpret_cast<uint32_t *>(l->keys_sorted.ptr);
   if (narrow)
-    LAB_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, keys32, keys32_sorted,
-                                        l->values.ptr, l->order.ptr, n, 0,
-                                        root_bits + slot_bits, s),
+    LAB_CHECK(sort32(nullptr, bytes, keys32, keys32_sorted, l->values.ptr,
+                     l->order.ptr, n, root_bits + slot_bits, s),
               "sort size");
   else
     LAB_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, l->keys.ptr,
@@ -1040,6 +1067,11 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(rocprim::exclusive_scan(nullptr, bytes, l->open_flag.ptr,
                                     l->open_rank.ptr, 0u, (size_t)n,
                                     rocprim::plus<uint32_t>(), s),
+            "scan size");
+  need = std::max(need, bytes);
+  LAB_CHECK(rocprim::exclusive_scan(nullptr, bytes, l->spot_pack.ptr,
+                                    l->spot_rank.ptr, 0ull, (size_t)n,
+                                    rocprim::plus<unsigned long long>(), s),
             "scan size");
   need = std::max(need, bytes);
   LAB_CHECK(l->temp.ensure(need + 256), "alloc temp");
@@ -1100,10 +1132,9 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(hipGetLastError(), "label kernels");
   bytes = l->temp.cap;
   if (narrow) {
-    LAB_CHECK(rocprim::radix_sort_pairs(l->temp.ptr, bytes, keys32,
-                                        keys32_sorted, l->values.ptr,
-                                        l->order.ptr, n, 0,
-                                        root_bits + slot_bits, s),
+    LAB_CHECK(sort32(l->temp.ptr, bytes, keys32, keys32_sorted,
+                     l->values.ptr, l->order.ptr, n, root_bits + slot_bits,
+                     s),
               "sorting components");
     // closed keys stay below 2^(root_bits + slot_bits) - 1, so the open key
     // (all ones) sorts last on those bits too
@@ -1134,36 +1165,36 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
         l->keys_sorted.ptr, ~0ull, l->d_scalars + 1);
   sf::count_components<<<1, 1, 0, s>>>(l->start_rank.ptr, l->start_flag.ptr,
                                        n, l->d_scalars);
+  // the members in emission order, in the sort's input arrays (dead now)
+  int32_t *mem_frame = reinterpret_cast<int32_t *>(l->values.ptr);
+  uint32_t *mem_index = reinterpret_cast<uint32_t *>(l->keys.ptr);
+  uint32_t *mem_value = mem_index + n;
+  sf::gather_members<<<grid, 256, 0, s>>>(l->d_scalars + 1, l->order.ptr,
+                                          l->node_frame.ptr,
+                                          l->node_index.ptr,
+                                          l->node_value.ptr, mem_frame,
+                                          mem_index, mem_value);
   // a bounded grid: each wave loops over components, each block adds its
   // tallies once
   sf::finalise<<<std::min<uint32_t>(grid, 1024), 256, 0, s>>>(
-      a, l->starts.ptr, l->d_scalars,
-                                    l->d_scalars + 1, l->order.ptr,
-                                    l->node_frame.ptr, l->node_index.ptr,
-                                    l->node_value.ptr, l->results.ptr,
-                                    l->d_counts);
+      a, l->starts.ptr, l->d_scalars, l->d_scalars + 1, mem_frame, mem_index,
+      mem_value, l->results.ptr, l->d_counts);
   sf::spot_flags<<<grid, 256, 0, s>>>(l->results.ptr, l->d_scalars,
-                                      l->acc_flag.ptr, l->mem_count.ptr);
+                                      l->spot_pack.ptr);
   bytes = l->temp.cap;
-  LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->acc_flag.ptr,
-                                    l->acc_rank.ptr, 0u, (size_t)n,
-                                    rocprim::plus<uint32_t>(), s),
-            "ranking spots");
-  bytes = l->temp.cap;
-  LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->mem_count.ptr,
-                                    l->mem_rank.ptr, 0u, (size_t)n,
-                                    rocprim::plus<uint32_t>(), s),
-            "placing pixels");
+  LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->spot_pack.ptr,
+                                    l->spot_rank.ptr, 0ull, (size_t)n,
+                                    rocprim::plus<unsigned long long>(), s),
+            "ranking spots and their pixels");
   sf::write_spots<<<grid, 256, 0, s>>>(l->results.ptr, l->d_scalars,
-                                       l->acc_rank.ptr, l->mem_rank.ptr,
-                                       l->d_totals, l->spots.ptr);
+                                       l->spot_rank.ptr, l->d_totals,
+                                       l->spots.ptr);
   sf::write_pixels<<<grid, 256, 0, s>>>(
       l->results.ptr, l->d_scalars + 1, l->start_flag.ptr, l->start_rank.ptr,
-      l->starts.ptr, l->order.ptr, l->mem_rank.ptr, l->node_frame.ptr,
-      l->node_index.ptr, l->node_value.ptr, l->d_totals, l->pixels.ptr);
-  sf::advance_totals<<<1, 1, 0, s>>>(l->d_scalars, l->acc_rank.ptr,
-                                     l->mem_rank.ptr, l->results.ptr,
-                                     l->d_totals);
+      l->starts.ptr, l->spot_rank.ptr, mem_frame, mem_index, mem_value,
+      l->d_totals, l->pixels.ptr);
+  sf::advance_totals<<<1, 1, 0, s>>>(l->d_scalars, l->spot_rank.ptr,
+                                     l->results.ptr, l->d_totals);
   bytes = l->temp.cap;
   LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->open_flag.ptr,
                                     l->open_rank.ptr, 0u, (size_t)n,
@@ -1369,10 +1400,8 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   l->start_rank.release();
   l->starts.release();
   l->results.release();
-  l->acc_flag.release();
-  l->acc_rank.release();
-  l->mem_count.release();
-  l->mem_rank.release();
+  l->spot_pack.release();
+  l->spot_rank.release();
   l->temp.release();
   l->chunk.release();
   l->spots.release();
