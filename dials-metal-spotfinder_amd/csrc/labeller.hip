@@ -7,19 +7,25 @@
 //
 //   nodes    the pixels still open from earlier chunks (the frontier), then
 //            every signal pixel of the chunk, in (frame, index) order;
-//   unite    each node hooks to the pixel on its left (same row and frame),
-//            the pixel above, and the same pixel on the previous frame when
-//            the frame numbers are consecutive and not two_d
-//            (src/dials_spots.cc:102-137); roots always point to the lower
-//            node, so a component's root is its first (frame, index) pixel;
+//   runs     horizontal runs of nodes (consecutive indices on one row of one
+//            frame; each carried pixel is a run of its own), numbered in
+//            node order. The grouping works on runs: a chunk of 16-bit frames
+//            has about seven pixels to a run;
+//   unite    each run joins the runs above it on its frame and the runs on
+//            the same row of the previous frame when the frame numbers are
+//            consecutive and not two_d (src/dials_spots.cc:102-137); roots
+//            always point to the lower run, so a component's root is its
+//            first run and that run's first pixel its first pixel;
 //   emit     a component is closed when it has no pixel on the chunk's last
 //            frame (or at finish, or always in two_d), and closed components
 //            are emitted ordered by (last frame, first pixel) -- the order
-//            the streaming Labeller flushes them (src/dials_spots.cc:153-194);
-//   finalise one thread per component walks its members in (frame, index)
-//            order and repeats the reference's FP64 arithmetic operation for
-//            operation (src/dials_spots.cc:221-360), so sums, centroid,
-//            variance and the separation test are bit-identical;
+//            the streaming Labeller flushes them (src/dials_spots.cc:153-194):
+//            a radix sort of the runs on (last frame, root run) keeps each
+//            component's runs, and so its pixels, in (frame, index) order;
+//   finalise the reference's emit() arithmetic (src/dials_spots.cc:221-360)
+//            per component, bit for bit: small components one to a lane,
+//            larger ones one to a wave with the variance terms added in
+//            member order;
 //   carry    the open components' pixels, still in (frame, index) order,
 //            become the next chunk's frontier, each pointing at its
 //            component's first pixel.
@@ -55,6 +61,7 @@ struct LabelArgs {
   int32_t two_d;
   uint64_t min_size, max_size;
   double max_separation;
+  double inv_width;                      // 1 / width, rounded
 };
 
 __device__ __forceinline__ uint32_t find_root(uint32_t *parent, uint32_t i) {
@@ -101,57 +108,224 @@ __device__ __forceinline__ int chunk_frame_of(const ChunkDesc *frames,
   return lo;
 }
 
-// Position of `index` among nodes [from, to), or -1.
-__device__ __forceinline__ long find_index(const uint32_t *index,
-                                           uint32_t from, uint32_t to,
-                                           uint32_t want) {
-  uint32_t lo = from, hi = to;
-  while (lo < hi) {
-    const uint32_t mid = lo + (hi - lo) / 2;
-    if (index[mid] < want)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return (lo < to && index[lo] == want) ? (long)lo : -1;
+// Row of pixel `idx`. The product with the rounded reciprocal is within
+// 2^-20 of idx / width (idx < 2^32), so for widths below 2^20 its truncation
+// is the quotient or one less, and one comparison settles it.
+__device__ __forceinline__ uint32_t row_of(const LabelArgs &a, uint32_t idx) {
+  if (a.width >= (1u << 20))
+    return idx / a.width;
+  uint32_t q = (uint32_t)((double)idx * a.inv_width);
+  if (idx - q * a.width >= a.width)
+    q++;
+  return q;
 }
 
-__global__ void gather_nodes(LabelArgs a, const ChunkDesc *frames,
-                             const int32_t *carry_frame,
-                             const uint32_t *carry_index,
-                             const uint32_t *carry_value,
-                             const uint32_t *carry_root, int32_t *node_frame,
-                             uint32_t *node_index, uint32_t *node_value,
-                             uint32_t *node_slot, uint32_t *parent,
-                             uint32_t *last_slot) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
-       g += gridDim.x * blockDim.x) {
-    last_slot[g] = 0;
+// Whole-wave reductions through DPP, with all 64 lanes active: a scan within
+// rows of 16 lanes (row_shr 1, 2, 4, 8), then the last lane of row 0 and of
+// row 2 broadcast into rows 1 and 3 (row_bcast 15), and the last lane of
+// row 1 into rows 2 and 3 (row_bcast 31); lane 63 ends with the result.
+// Lanes with no source read the identity.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_take(uint32_t x, uint32_t identity) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)x, CTRL,
+                                               ROWS, 0xF, false);
+}
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t x, uint32_t identity,
+                                                Op op) {
+  x = op(x, dpp_take<0x111, 0xF>(x, identity));
+  x = op(x, dpp_take<0x112, 0xF>(x, identity));
+  x = op(x, dpp_take<0x114, 0xF>(x, identity));
+  x = op(x, dpp_take<0x118, 0xF>(x, identity));
+  x = op(x, dpp_take<0x142, 0xA>(x, identity));
+  x = op(x, dpp_take<0x143, 0xC>(x, identity));
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+  return wave_reduce(x, 0xFFFFFFFFu,
+                     [](uint32_t p, uint32_t q) { return p < q ? p : q; });
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+  return wave_reduce(x, 0u,
+                     [](uint32_t p, uint32_t q) { return p > q ? p : q; });
+}
+__device__ __forceinline__ int32_t wave_min_i32(int32_t x) {
+  return (int32_t)wave_reduce((uint32_t)x, 0x7FFFFFFFu, [](uint32_t p,
+                                                           uint32_t q) {
+    return (int32_t)p < (int32_t)q ? p : q;
+  });
+}
+__device__ __forceinline__ int32_t wave_max_i32(int32_t x) {
+  return (int32_t)wave_reduce((uint32_t)x, 0x80000000u, [](uint32_t p,
+                                                           uint32_t q) {
+    return (int32_t)p > (int32_t)q ? p : q;
+  });
+}
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_take_f64(double x) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const uint32_t lo = dpp_take<CTRL, ROWS>((uint32_t)b, 0u);
+  const uint32_t hi = dpp_take<CTRL, ROWS>((uint32_t)(b >> 32), 0u);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// The sum of integer-valued doubles: exact in any order while every partial
+// sum stays below 2^53.
+__device__ __forceinline__ double wave_sum_f64(double x) {
+  x += dpp_take_f64<0x111, 0xF>(x);
+  x += dpp_take_f64<0x112, 0xF>(x);
+  x += dpp_take_f64<0x114, 0xF>(x);
+  x += dpp_take_f64<0x118, 0xF>(x);
+  x += dpp_take_f64<0x142, 0xA>(x);
+  x += dpp_take_f64<0x143, 0xC>(x);
+  return readlane_f64(x, 63);
+}
+
+// Nodes per tile of gather_nodes and number_runs.
+constexpr uint32_t kTile = 2048;
+
+// Nodes from the frontier and the chunk's lists, in (frame, index) order,
+// and which of them begin a run: a horizontal run is consecutive indices on
+// one row of one frame, and every carried node is a run of its own. Bit
+// (g & 63) of head_bits[g >> 6] marks node g a run head; tile t (nodes
+// [t, t + 1) * kTile) counts its heads in head_count[t].
+__global__ __launch_bounds__(256) void gather_nodes(
+    LabelArgs a, const ChunkDesc *frames, const int32_t *carry_frame,
+    const uint32_t *carry_index, const uint32_t *carry_value,
+    int32_t *node_frame, uint32_t *node_index, uint32_t *node_value,
+    uint32_t *node_slot, uint64_t *head_bits, uint32_t *head_count) {
+  __shared__ uint32_t listed;
+  const uint32_t lane = lane_id();
+  const uint32_t tile = blockIdx.x * kTile;
+  if (threadIdx.x == 0)
+    listed = 0;
+  __syncthreads();
+  uint32_t mine = 0;
+  int k = -1; // this thread's chunk frame, found once, then stepped
+  for (uint32_t g = tile + threadIdx.x;
+       g < min(tile + kTile, a.n_nodes); g += blockDim.x) {
+    bool head = true;
     if (g < a.n_carry) {
       node_frame[g] = carry_frame[g];
       node_index[g] = carry_index[g];
       node_value[g] = carry_value[g];
       node_slot[g] = 0;
-      parent[g] = carry_root[g];
     } else {
-      const int k = chunk_frame_of(frames, a.n_frames, g);
-      const sf_signal_pixel e = frames[k].list[g - frames[k].offset];
+      if (k < 0)
+        k = chunk_frame_of(frames, a.n_frames, g);
+      while (k + 1 < (int)a.n_frames && frames[k + 1].offset <= g)
+        k++;
+      const sf_signal_pixel *list = frames[k].list;
+      const uint32_t at = g - frames[k].offset;
+      const sf_signal_pixel e = list[at];
       node_frame[g] = frames[k].number;
       node_index[g] = e.index;
       node_value[g] = e.value;
       node_slot[g] = (uint32_t)k + 1;
-      parent[g] = g;
+      head = at == 0 || e.index - row_of(a, e.index) * a.width == 0 ||
+             list[at - 1].index != e.index - 1;
     }
+    const uint64_t mask = ballot(head);
+    if (lane == 0)
+      head_bits[g >> 6] = mask; // the wave's 64 nodes start at g
+    mine += head;
+  }
+  if (mine)
+    atomicAdd(&listed, mine);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    head_count[blockIdx.x] = listed;
+}
+
+// Runs numbered in node order: run r is the r-th head. Tile t's first run
+// number is the heads of tiles 0 .. t - 1, which each block adds up itself;
+// each wave takes 512 consecutive nodes of its tile. Writes run_of per
+// node; per run its head node, first and last pixel index, slot and initial
+// parent -- itself, or for a carried node its carried root: carried nodes
+// are all heads and come first, so their run numbers are their node
+// numbers -- and a zero last_slot. The block of the last tile writes the run
+// count.
+__global__ __launch_bounds__(256) void number_runs(
+    LabelArgs a, const uint64_t *head_bits, const uint32_t *head_count,
+    const uint32_t *node_index, const uint32_t *node_slot,
+    const uint32_t *carry_root, uint32_t *run_of, uint32_t *run_head,
+    uint32_t *run_idx, uint32_t *run_last, uint32_t *run_slot,
+    uint32_t *parent, uint32_t *last_slot, uint32_t *n_runs) {
+  // part[w]: wave w's share of the earlier tiles' heads; part[4 + w]: the
+  // heads among wave w's own 512 nodes
+  __shared__ uint32_t part[8];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t tile = blockIdx.x * kTile;
+  uint32_t before = 0;
+  for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x)
+    before += head_count[i];
+  before = wave_reduce(before, 0u,
+                       [](uint32_t p, uint32_t q) { return p + q; });
+  const uint32_t first = tile + 512 * wave;
+  uint32_t own = 0;
+  if (lane < 8 && first + 64 * lane < a.n_nodes)
+    own = (uint32_t)__builtin_popcountll(head_bits[(first >> 6) + lane]);
+  own = wave_reduce(own, 0u, [](uint32_t p, uint32_t q) { return p + q; });
+  if (lane == 0) {
+    part[wave] = before;
+    part[4 + wave] = own;
+  }
+  __syncthreads();
+  const uint32_t tiles_before = part[0] + part[1] + part[2] + part[3];
+  uint32_t run = tiles_before; // heads before this wave's nodes
+  for (uint32_t w = 0; w < wave; w++)
+    run += part[4 + w];
+  if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0)
+    *n_runs = tiles_before + part[4] + part[5] + part[6] + part[7];
+  const uint64_t upto = ~0ull >> (63 - lane);
+  for (uint32_t j = 0; j < 8; j++) {
+    const uint32_t g = first + 64 * j + lane;
+    if (first + 64 * j >= a.n_nodes)
+      break;
+    const uint64_t mask = head_bits[(first >> 6) + j];
+    // the next group's heads, for lane 63's tail test
+    const uint64_t next = first + 64 * (j + 1) < a.n_nodes
+                              ? head_bits[(first >> 6) + j + 1]
+                              : 1ull;
+    if (g < a.n_nodes) {
+      const uint32_t r = run + (uint32_t)__builtin_popcountll(mask & upto) - 1;
+      run_of[g] = r;
+      const bool head = (mask >> lane) & 1;
+      const bool tail = g + 1 == a.n_nodes ||
+                        (lane < 63 ? (mask >> (lane + 1)) & 1 : next & 1);
+      if (head || tail) {
+        const uint32_t idx = node_index[g];
+        if (head) {
+          run_head[r] = g;
+          run_idx[r] = idx;
+          run_slot[r] = node_slot[g];
+          parent[r] = g < a.n_carry ? carry_root[g] : r;
+          last_slot[r] = 0;
+        }
+        if (tail)
+          run_last[r] = idx;
+      }
+    }
+    run += (uint32_t)__builtin_popcountll(mask);
   }
 }
 
-// row_start[slot * (height + 1) + y] = the first node of `slot` whose index
-// is at least y * width (slot 0: the carried nodes of the last carried
-// frame; slot k + 1: chunk frame k). One thread per (slot, row).
-__global__ void fill_rows(LabelArgs a, const ChunkDesc *frames,
-                          const uint32_t *node_index, uint32_t *row_start) {
+// run_rows[slot * (height + 1) + y] = the first run of `slot` whose first
+// index is at least y * width (slot 0: the carried runs -- single carried
+// pixels -- of the last carried frame; slot k + 1: chunk frame k, whose runs
+// start at its first node's run). One thread per (slot, row).
+__global__ void fill_run_rows(LabelArgs a, const ChunkDesc *frames,
+                              const uint32_t *run_of, const uint32_t *run_idx,
+                              const uint32_t *n_runs, uint32_t *run_rows) {
   const uint32_t height = a.pixels / a.width;
   const uint32_t total = (a.n_frames + 1) * (height + 1);
+  const uint32_t runs = *n_runs;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += gridDim.x * blockDim.x) {
     const uint32_t slot = t / (height + 1), y = t % (height + 1);
@@ -160,8 +334,11 @@ __global__ void fill_rows(LabelArgs a, const ChunkDesc *frames,
       from = a.carry_tail;
       to = a.n_carry;
     } else {
-      from = frames[slot - 1].offset;
-      to = slot < a.n_frames ? frames[slot].offset : a.n_nodes;
+      const uint32_t f = frames[slot - 1].offset;
+      const uint32_t e =
+          slot < a.n_frames ? frames[slot].offset : a.n_nodes;
+      from = f < a.n_nodes ? run_of[f] : runs;
+      to = e < a.n_nodes ? run_of[e] : runs;
     }
     const uint32_t want = y * a.width; // y == height: past every index
     uint32_t lo = from, hi = to;
@@ -170,147 +347,193 @@ __global__ void fill_rows(LabelArgs a, const ChunkDesc *frames,
     } else {
       while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;
-        if (node_index[mid] < want)
+        if (run_idx[mid] < want)
           lo = mid + 1;
         else
           hi = mid;
       }
     }
-    row_start[t] = lo;
+    run_rows[t] = lo;
   }
 }
 
-// Horizontal runs without atomics: every chunk node points at the first
-// node of its run (consecutive indices on one row of one frame), which is
-// lower, so roots still point down.
-__global__ void link_runs(LabelArgs a, const ChunkDesc *frames,
-                          const uint32_t *node_index, const uint32_t *slot,
-                          uint32_t *parent, uint32_t *run) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
-       g += gridDim.x * blockDim.x) {
-    if (g < a.n_carry) { // carried nodes already point at their roots
-      run[g] = g;
-      continue;
+// First entry in [lo, hi) of the sorted `key` that is at least `want` (hi
+// if none). Four independent probes a step cut the range five ways.
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t *key,
+                                                uint32_t lo, uint32_t hi,
+                                                uint32_t want) {
+  while (hi - lo > 4) {
+    const uint32_t step = (hi - lo) / 5;
+    const uint32_t p0 = lo + step, p1 = p0 + step, p2 = p1 + step,
+                   p3 = p2 + step;
+    const uint32_t v0 = key[p0], v1 = key[p1], v2 = key[p2], v3 = key[p3];
+    if (v3 < want) {
+      lo = p3 + 1;
+    } else if (v2 < want) {
+      lo = p2 + 1;
+      hi = p3;
+    } else if (v1 < want) {
+      lo = p1 + 1;
+      hi = p2;
+    } else if (v0 < want) {
+      lo = p0 + 1;
+      hi = p1;
+    } else {
+      hi = p0;
     }
-    const int k = (int)slot[g] - 1; // chunk frame, from the gather
-    const uint32_t from = frames[k].offset;
-    uint32_t r = g, idx = node_index[g];
-    while (r > from && idx % a.width != 0 && node_index[r - 1] == idx - 1) {
-      r--;
-      idx--;
+  }
+  uint32_t below = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < 4; q++)
+    below += lo + q < hi && key[lo + q] < want;
+  return lo + below;
+}
+
+// Joins run `r` to every run of [from, to) -- the runs of one row, in index
+// order, disjoint -- that covers one of indices lo .. hi. Long rows start
+// with a search on the runs' last indices; runs are then read four at a
+// time.
+__device__ __forceinline__ void unite_row(uint32_t *parent,
+                                          const uint32_t *run_idx,
+                                          const uint32_t *run_last,
+                                          uint32_t r, uint32_t from,
+                                          uint32_t to, uint32_t lo,
+                                          uint32_t hi) {
+  if (to - from > 8)
+    from = lower_bound(run_last, from, to, lo);
+  for (uint32_t u = from; u < to; u += 4) {
+    uint32_t first[4], last[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const bool in = u + q < to;
+      first[q] = in ? run_idx[u + q] : 0xFFFFFFFFu;
+      last[q] = in ? run_last[u + q] : 0u;
     }
-    parent[g] = r;
-    run[g] = r;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (u + q >= to || first[q] > hi)
+        return;
+      if (last[q] >= lo)
+        unite(parent, r, u + q);
+    }
   }
 }
 
-// The up and previous-frame edges (src/dials_spots.cc:106-137). An edge is
-// skipped when the left neighbour has the same edge one column over: the two
-// targets are then neighbours on their own row and already joined, as are
-// the node and its left neighbour.
-__global__ void unite_edges(LabelArgs a, const ChunkDesc *frames,
-                            const uint32_t *node_index, const uint32_t *slot,
-                            const uint32_t *row_start, uint32_t *parent) {
+// The up and previous-frame edges (src/dials_spots.cc:106-137), a run at a
+// time: a run joins each run above it on its frame, and each run on the same
+// row of the frame before when the frame numbers are consecutive and not
+// two_d. Runs in run order, a contiguous range of them to each XCD, so that
+// the run tables they read are mostly in that XCD's L2.
+__global__ __launch_bounds__(256) void unite_runs(
+    LabelArgs a, const ChunkDesc *frames, const uint32_t *run_idx,
+    const uint32_t *run_last, const uint32_t *run_slot,
+    const uint32_t *run_rows, const uint32_t *n_runs, uint32_t *parent) {
   const uint32_t stride = a.pixels / a.width + 1;
-  for (uint32_t g = a.n_carry + blockIdx.x * blockDim.x + threadIdx.x;
-       g < a.n_nodes; g += gridDim.x * blockDim.x) {
-    const int k = (int)slot[g] - 1; // chunk frame, from the gather
-    const uint32_t from = frames[k].offset;
-    const uint32_t idx = node_index[g];
-    const uint32_t y = idx / a.width;
-    const uint32_t *rows = row_start + (size_t)(k + 1) * stride;
-    const bool left =
-        g > from && idx % a.width != 0 && node_index[g - 1] == idx - 1;
-    if (y > 0) {
-      const uint32_t lo = rows[y - 1];
-      const long up = find_index(node_index, lo, rows[y], idx - a.width);
-      if (up >= 0 && !(left && (uint32_t)up > lo &&
-                       node_index[up - 1] == idx - a.width - 1))
-        unite(parent, g, (uint32_t)up);
-    }
+  const uint32_t runs = *n_runs;
+  for (uint32_t r = xcd_local_item(blockIdx.x, gridDim.x) * blockDim.x +
+                    threadIdx.x + a.n_carry;
+       r < runs; r += gridDim.x * blockDim.x) {
+    const int k = (int)run_slot[r] - 1;
+    const uint32_t idx = run_idx[r], last = run_last[r];
+    const uint32_t y = row_of(a, idx);
+    const uint32_t *rows = run_rows + (size_t)(k + 1) * stride;
+    if (y > 0)
+      unite_row(parent, run_idx, run_last, r, rows[y - 1], rows[y],
+                idx - a.width, last - a.width);
     if (!a.two_d && frames[k].linked) {
-      const uint32_t *prev = row_start + (size_t)k * stride;
-      const uint32_t lo = prev[y];
-      const long back = find_index(node_index, lo, prev[y + 1], idx);
-      if (back >= 0 &&
-          !(left && (uint32_t)back > lo && node_index[back - 1] == idx - 1))
-        unite(parent, g, (uint32_t)back);
+      const uint32_t *prev = run_rows + (size_t)k * stride;
+      unite_row(parent, run_idx, run_last, r, prev[y], prev[y + 1], idx,
+                last);
     }
   }
 }
 
-// Every union is done: resolve each node's root with a read-only walk into
-// its own array. (Writing roots back into `parent` while other threads still
-// walk it would let a path-halving write replace a finished root with an
-// intermediate ancestor.)
-__global__ void resolve_roots(LabelArgs a, const uint32_t *parent,
-                              const uint32_t *run, const uint32_t *slot,
-                              uint32_t *root, uint32_t *last_slot) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
-       g += gridDim.x * blockDim.x) {
-    if (run[g] != g)
-      continue; // resolved from its run's first node below
-    uint32_t r = g;
-    while (parent[r] != r)
-      r = parent[r];
-    root[g] = r;
-    if (slot[g]) // a run lies on one frame: one update per run
-      atomicMax(&last_slot[r], slot[g]);
+// Every union is done: resolve each run's root (its component's first run,
+// whose first pixel is the component's first pixel) with a read-only walk.
+// (Writing roots back into `parent` while other threads still walk it would
+// let a path-halving write replace a finished root with an intermediate
+// ancestor.) A run lies on one frame: one last_slot update per run.
+__global__ void resolve_runs(const uint32_t *parent, const uint32_t *run_slot,
+                             const uint32_t *n_runs, uint32_t *root,
+                             uint32_t *last_slot) {
+  const uint32_t runs = *n_runs;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < runs;
+       r += gridDim.x * blockDim.x) {
+    uint32_t p = r;
+    while (parent[p] != p)
+      p = parent[p];
+    root[r] = p;
+    if (run_slot[r])
+      atomicMax(&last_slot[p], run_slot[r]);
   }
 }
 
-__global__ void resolve_runs(LabelArgs a, const uint32_t *run,
-                             uint32_t *root) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
-       g += gridDim.x * blockDim.x) {
-    const uint32_t r = run[g];
-    if (r != g)
-      root[g] = root[r];
-  }
-}
-
-// Sort keys: closed components by (last slot, root), the slot shifted left
-// by `shift`; open ones last (all bits set). Also the flags of the nodes that
-// stay open.
+// Sort keys per run: closed components by (last slot, root run), the slot
+// shifted left by `shift`; open ones last (all bits set). Also each run's
+// pixels if it stays open (else 0), for the frontier.
 template <typename K>
 __global__ void make_keys(LabelArgs a, const uint32_t *root,
-                          const uint32_t *last_slot, K *keys,
-                          uint32_t *values, uint32_t *open_flag, int shift) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
-       g += gridDim.x * blockDim.x) {
-    const uint32_t r = root[g];
-    const uint32_t last = last_slot[r];
+                          const uint32_t *last_slot, const uint32_t *run_head,
+                          const uint32_t *n_runs, K *keys, uint32_t *values,
+                          uint32_t *open_len, int shift) {
+  const uint32_t runs = *n_runs;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < runs;
+       r += gridDim.x * blockDim.x) {
+    const uint32_t p = root[r];
+    const uint32_t last = last_slot[p];
     const bool open = !a.closing && !a.two_d && a.n_frames > 0 &&
                       last == a.n_frames;
-    keys[g] = open ? ~(K)0 : (K)(((K)last << shift) | r);
-    values[g] = g;
-    open_flag[g] = open ? 1u : 0u;
+    keys[r] = open ? ~(K)0 : (K)(((K)last << shift) | p);
+    values[r] = r;
+    open_len[r] =
+        open ? (r + 1 < runs ? run_head[r + 1] : a.n_nodes) - run_head[r] : 0;
   }
 }
 
-// start_flag[i] = sorted entry i begins a closed component.
+// Per sorted run i: start << 32 | pixels, where start = 1 if the run begins
+// a closed component and pixels = its pixel count if closed (else 0). One
+// exclusive scan of these gives each closed run's component (high half) and
+// its first place among the closed members (low half). Also each run's
+// sorted place (pos_of) and the number of closed runs (d_closed_runs).
 template <typename K>
-__global__ void mark_starts(LabelArgs a, const K *keys, K open_key,
-                            uint32_t *start_flag) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_nodes;
-       i += gridDim.x * blockDim.x)
-    start_flag[i] =
-        (keys[i] != open_key && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
+__global__ void mark_runs(LabelArgs a, const K *keys, K open_key,
+                          const uint32_t *order, const uint32_t *run_head,
+                          uint32_t runs, unsigned long long *pack,
+                          uint32_t *pos_of, uint32_t *d_closed_runs) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < runs;
+       i += gridDim.x * blockDim.x) {
+    const K key = keys[i];
+    const bool closed = key != open_key;
+    const bool start = closed && (i == 0 || key != keys[i - 1]);
+    const uint32_t r = order[i];
+    pos_of[r] = i;
+    uint32_t len = 0;
+    if (closed)
+      len = (r + 1 < runs ? run_head[r + 1] : a.n_nodes) - run_head[r];
+    pack[i] = ((unsigned long long)start << 32) | len;
+    if (closed && (i + 1 == runs || keys[i + 1] == open_key))
+      *d_closed_runs = i + 1;
+  }
 }
 
-template <typename K>
-__global__ void list_starts(LabelArgs a, const uint32_t *start_flag,
-                            const uint32_t *start_rank, uint32_t *starts,
-                            const K *keys, K open_key,
-                            uint32_t *n_closed_nodes) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_nodes;
+// Component c's first run (starts) and first member (comp_mem); the number
+// of components (d_scalars[0]) and of closed members (d_scalars[1]).
+__global__ void list_starts(const unsigned long long *pack,
+                            const unsigned long long *rank, uint32_t runs,
+                            const uint32_t *d_closed_runs, uint32_t *starts,
+                            uint32_t *comp_mem, uint32_t *d_scalars) {
+  const uint32_t closed = *d_closed_runs;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < closed;
        i += gridDim.x * blockDim.x) {
-    if (start_flag[i])
-      starts[start_rank[i]] = i;
-    const bool closed = keys[i] != open_key;
-    const bool next_open = i + 1 == a.n_nodes || keys[i + 1] == open_key;
-    if (closed && next_open)
-      *n_closed_nodes = i + 1;
+    const unsigned long long p = pack[i], q = rank[i];
+    if (p >> 32) {
+      starts[q >> 32] = i;
+      comp_mem[q >> 32] = (uint32_t)q;
+    }
+    if (i + 1 == closed) {
+      d_scalars[0] = (uint32_t)(q >> 32) + (uint32_t)(p >> 32);
+      d_scalars[1] = (uint32_t)q + (uint32_t)p;
+    }
   }
 }
 
@@ -379,8 +602,9 @@ __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
   for (uint32_t k = 0; k < n; k++) {
     const uint32_t g = s0 + k;
     const uint32_t idx = mem_index[g];
-    const int32_t x = (int32_t)(idx % a.width);
-    const int32_t y = (int32_t)(idx / a.width);
+    const uint32_t row = row_of(a, idx);
+    const int32_t x = (int32_t)(idx - row * a.width);
+    const int32_t y = (int32_t)row;
     const int32_t z = mem_frame[g];
     if (k == 0) {
       x0 = x;
@@ -422,11 +646,11 @@ __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
                        for (uint32_t k = 0; k < n; k++) {
                          const uint32_t g = s0 + k;
                          const uint32_t idx = mem_index[g];
+                         const uint32_t row = row_of(a, idx);
                          const double value = (double)mem_value[g];
                          const double position[3] = {
-                             (double)(idx % a.width) + 0.5,
-                             (double)(idx / a.width) + 0.5,
-                             (double)mem_frame[g] + 0.5};
+                             (double)(idx - row * a.width) + 0.5,
+                             (double)row + 0.5, (double)mem_frame[g] + 0.5};
                          for (int axis = 0; axis < 3; axis++) {
                            const double delta = position[axis] - centre[axis];
                            delta_sq[axis] += value * delta * delta;
@@ -435,235 +659,287 @@ __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
                      });
 }
 
-template <typename T> __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1)
-    v += __shfl_xor(v, d, 64);
-  return v;
-}
-template <typename T> __device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T o = __shfl_xor(v, d, 64);
-    v = o < v ? o : v;
-  }
-  return v;
-}
-template <typename T> __device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T o = __shfl_xor(v, d, 64);
-    v = o > v ? o : v;
-  }
-  return v;
-}
-
-// The closed members' frame, index and value in emission order (the sorted
-// order), gathered once: the finalise and pixel-table passes then read them
-// in order instead of through `order`.
-__global__ void gather_members(const uint32_t *n_closed, const uint32_t *order,
+// The closed members' frame, index and value in emission order, gathered
+// once, one thread per node: node g of run r is member
+// (first member of r's sorted place) + (g - r's head).
+__global__ void gather_members(LabelArgs a, const uint32_t *d_closed_runs,
+                               const uint32_t *run_of, const uint32_t *pos_of,
+                               const uint32_t *run_head,
+                               const unsigned long long *rank,
                                const int32_t *node_frame,
                                const uint32_t *node_index,
                                const uint32_t *node_value, int32_t *mem_frame,
                                uint32_t *mem_index, uint32_t *mem_value) {
-  const uint32_t end = *n_closed;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end;
-       i += gridDim.x * blockDim.x) {
-    const uint32_t g = order[i];
-    mem_frame[i] = node_frame[g];
-    mem_index[i] = node_index[g];
-    mem_value[i] = node_value[g];
+  const uint32_t closed = *d_closed_runs;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
+       g += gridDim.x * blockDim.x) {
+    const uint32_t r = run_of[g];
+    const uint32_t i = pos_of[r];
+    if (i >= closed)
+      continue;
+    const uint32_t at = (uint32_t)rank[i] + (g - run_head[r]);
+    mem_frame[at] = node_frame[g];
+    mem_index[at] = node_index[g];
+    mem_value[at] = node_value[g];
   }
 }
 
-// One wave per closed component, in emission order, with the result the
-// sequential emit() gives bit for bit:
+// Adds a block's tallies of emitted kinds into `counts` (groups,
+// too_small, too_large, separated, accepted, at 1..5), one atomic per
+// counter and block.
+__device__ void add_tallies(const uint32_t (&tally)[4],
+                            unsigned long long *counts) {
+  __shared__ uint32_t block_tally[4];
+  if (threadIdx.x < 4)
+    block_tally[threadIdx.x] = 0;
+  __syncthreads();
+  for (int t = 0; t < 4; t++)
+    if (tally[t])
+      atomicAdd(&block_tally[t], tally[t]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int slot[4] = {5, 2, 3, 4}; // kAccepted, kTooSmall, ...
+    for (int t = 0; t < 4; t++)
+      if (block_tally[t])
+        atomicAdd(&counts[slot[t]], (unsigned long long)block_tally[t]);
+  }
+}
+
+// Components of up to this many members are emitted one to a lane.
+constexpr uint32_t kLaneMembers = 32;
+
+// The closed components, one to a lane, in emission order: too small, too
+// large, or at most kLaneMembers members and emitted by the reference's own
+// arithmetic (emit_sequential). Larger ones are listed for finalise_wide.
+__global__ __launch_bounds__(256) void finalise_narrow(
+    LabelArgs a, const uint32_t *starts, const uint32_t *n_starts,
+    const uint32_t *n_closed, const int32_t *mem_frame,
+    const uint32_t *mem_index, const uint32_t *mem_value, SpotOut *out,
+    uint32_t *wide, uint32_t *n_wide, unsigned long long *counts) {
+  const uint32_t comps = *n_starts;
+  const uint32_t end_all = *n_closed;
+  const uint32_t lane = lane_id();
+  uint32_t tally[4] = {0, 0, 0, 0};
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
+       c += gridDim.x * blockDim.x) {
+    const uint32_t s0 = starts[c];
+    const uint32_t n = (c + 1 < comps ? starts[c + 1] : end_all) - s0;
+    const bool later =
+        n >= a.min_size && n <= a.max_size && n > kLaneMembers;
+    const uint64_t mask = ballot(later);
+    if (mask) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+      uint32_t base = 0;
+      if (lane == leader)
+        base = atomicAdd(n_wide, (uint32_t)__builtin_popcountll(mask));
+      base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
+      if (later)
+        wide[base + count_below(mask)] = c;
+    }
+    if (later)
+      continue;
+    SpotOut r;
+    memset(&r, 0, sizeof(r));
+    int kind;
+    if (n < a.min_size)
+      kind = kTooSmall;
+    else if (n > a.max_size)
+      kind = kTooLarge;
+    else
+      kind = emit_sequential(a, s0, n, mem_frame, mem_index, mem_value,
+                             r.spot);
+    if (kind == kAccepted) {
+      r.accepted = 1;
+      r.members = n;
+    }
+    out[c] = r;
+    tally[kind]++;
+  }
+  add_tallies(tally, counts);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd(&counts[1], (unsigned long long)comps);
+}
+
+// Variance terms staged per round, and the stride of each axis's row (two
+// doubles over, so the three chains read distinct banks).
+constexpr uint32_t kTermRound = 256;
+constexpr uint32_t kTermStride = kTermRound + 2;
+
+// One wave per listed component, with the result the sequential emit()
+// gives bit for bit:
 //  * bbox, peak (first strict maximum in member order) and the sums are
 //    taken across the wave. The reference sums in double in member order;
 //    when every term and partial sum is an integer (or half-integer) below
 //    2^53 -- always so for 16-bit frames -- those double sums are exact and
-//    equal the integer sums computed here in any order. Otherwise one lane
-//    runs the sequential emit();
-//  * the variance terms v (p - c)^2 are formed in parallel, then added in
-//    member order, as src/dials_spots.cc:304-316 adds them.
-__global__ __launch_bounds__(256) void finalise(
+//    equal the sums of the same integers taken in any order, which is what
+//    the lanes and the DPP reduction add here. Otherwise one lane runs the
+//    sequential emit();
+//  * the variance terms v (p - c)^2 are formed in parallel and staged in
+//    LDS, then lanes 0, 1 and 2 each add one axis's terms in member order,
+//    as src/dials_spots.cc:304-316 adds them.
+__global__ __launch_bounds__(256) void finalise_wide(
     LabelArgs a, const uint32_t *starts, const uint32_t *n_starts,
     const uint32_t *n_closed, const int32_t *mem_frame,
-    const uint32_t *mem_index, const uint32_t *mem_value, SpotOut *out,
+    const uint32_t *mem_index, const uint32_t *mem_value,
+    const uint32_t *wide, const uint32_t *n_wide, SpotOut *out,
     unsigned long long *counts) {
-  __shared__ double terms[4][3 * 64]; // a wave's variance terms, by axis
+  __shared__ double terms[4][3 * kTermStride];
+  const uint32_t count = *n_wide;
   const uint32_t comps = *n_starts;
   const uint32_t end_all = *n_closed;
-  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lane = lane_id();
+  double *mine = terms[threadIdx.x >> 6];
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   uint32_t tally[4] = {0, 0, 0, 0};
-  for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-       c < comps; c += waves) {
+  for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       w < count; w += waves) {
+    const uint32_t c = wide[w];
     const uint32_t s0 = starts[c];
-    const uint32_t s1 = c + 1 < comps ? starts[c + 1] : end_all;
-    const uint32_t n = s1 - s0;
+    const uint32_t n = (c + 1 < comps ? starts[c + 1] : end_all) - s0;
     SpotOut r;
     memset(&r, 0, sizeof(r));
-    int kind = kAccepted;
-    if (n < a.min_size) {
-      kind = kTooSmall;
-    } else if (n > a.max_size) {
-      kind = kTooLarge;
+    // the first round's members in registers: lane + 64 j, j < 4
+    uint32_t m_idx[4], m_val[4];
+    int32_t m_z[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t k = lane + 64u * j;
+      m_idx[j] = k < n ? mem_index[s0 + k] : 0;
+      m_val[j] = k < n ? mem_value[s0 + k] : 0;
+      m_z[j] = k < n ? mem_frame[s0 + k] : 0;
+    }
+    // pass 1: bbox, peak, sums, bounds
+    uint32_t x0 = 0xFFFFFFFFu, x1 = 0, y0 = 0xFFFFFFFFu, y1 = 0;
+    int32_t z0 = INT32_MAX, z1 = INT32_MIN;
+    double sv = 0.0, svv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    uint32_t vmax = 0, kmax = 0xFFFFFFFFu, zabs = 0;
+    auto visit = [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
+      const uint32_t y = row_of(a, idx);
+      const uint32_t x = idx - y * a.width;
+      x0 = min(x0, x);
+      x1 = max(x1, x + 1);
+      y0 = min(y0, y);
+      y1 = max(y1, y + 1);
+      z0 = min(z0, z);
+      z1 = max(z1, z + 1);
+      const double value = (double)v;
+      sv += value;
+      svv += value * value;
+      sx += value * (double)(2 * x + 1);
+      sy += value * (double)(2 * y + 1);
+      sz += value * (double)(2 * (int64_t)z + 1);
+      zabs = max(zabs, (uint32_t)(z < 0 ? -(int64_t)z : (int64_t)z));
+      if (v > vmax) { // lanes see members in order, so first per lane
+        vmax = v;
+        kmax = k;
+      }
+    };
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t k = lane + 64u * j;
+      if (k < n)
+        visit(k, m_idx[j], m_val[j], m_z[j]);
+    }
+    for (uint32_t k = lane + kTermRound; k < n; k += 64)
+      visit(k, mem_index[s0 + k], mem_value[s0 + k], mem_frame[s0 + k]);
+    x0 = wave_min_u32(x0);
+    x1 = wave_max_u32(x1);
+    y0 = wave_min_u32(y0);
+    y1 = wave_max_u32(y1);
+    z0 = wave_min_i32(z0);
+    z1 = wave_max_i32(z1);
+    zabs = wave_max_u32(zabs);
+    const uint32_t vtop = wave_max_u32(vmax);
+    const uint32_t ktop = wave_min_u32(vmax == vtop ? kmax : 0xFFFFFFFFu);
+    sf_spot &spot = r.spot;
+    // Exactness of the reference's double sums: every term and partial
+    // sum below 2^52 (with margin) in magnitude.
+    const double bound = 4503599627370496.0; // 2^52
+    const double top = (double)vtop, cnt = (double)n;
+    const double reach =
+        (double)max(max(a.width, a.pixels / a.width), zabs) * 2.0 + 1.0;
+    const bool exact = cnt * top * top < bound && cnt * top * reach < bound;
+    int kind;
+    if (!exact) {
+      kind = kAccepted;
+      if (lane == 0)
+        kind = emit_sequential(a, s0, n, mem_frame, mem_index, mem_value,
+                               spot);
+      kind = __builtin_amdgcn_readlane(kind, 0);
     } else {
-      // Members in registers: lane holds members lane + 64 j, j < KMAX.
-      constexpr int KMAX = 16;
-      const bool held = n <= 64u * KMAX;
-      uint32_t m_idx[KMAX], m_val[KMAX];
-      int32_t m_z[KMAX];
-#pragma unroll
-      for (int j = 0; j < KMAX; j++) {
-        const uint32_t k = lane + 64u * j;
-        m_idx[j] = 0;
-        m_val[j] = 0;
-        m_z[j] = 0;
-        if (held && k < n) {
-          m_idx[j] = mem_index[s0 + k];
-          m_val[j] = mem_value[s0 + k];
-          m_z[j] = mem_frame[s0 + k];
-        }
+      const double sum = wave_sum_f64(sv);
+      const double sum_sq = wave_sum_f64(svv);
+      const double weighted[3] = {wave_sum_f64(sx) * 0.5,
+                                  wave_sum_f64(sy) * 0.5,
+                                  wave_sum_f64(sz) * 0.5};
+      spot.n_signal = n;
+      spot.bbox[0] = (int32_t)x0;
+      spot.bbox[1] = (int32_t)x1;
+      spot.bbox[2] = (int32_t)y0;
+      spot.bbox[3] = (int32_t)y1;
+      spot.bbox[4] = z0;
+      spot.bbox[5] = z1;
+      double peak[3] = {0.0, 0.0, 0.0};
+      if (vtop > 0) {
+        const uint32_t idx = mem_index[s0 + ktop];
+        const uint32_t y = row_of(a, idx);
+        peak[0] = (double)(idx - y * a.width) + 0.5;
+        peak[1] = (double)y + 0.5;
+        peak[2] = (double)mem_frame[s0 + ktop] + 0.5;
       }
-      // pass 1: bbox, peak, integer sums, bounds
-      int32_t x0 = INT32_MAX, x1 = INT32_MIN, y0 = INT32_MAX, y1 = INT32_MIN,
-              z0 = INT32_MAX, z1 = INT32_MIN;
-      uint64_t sv = 0, svv = 0, sx = 0, sy = 0;
-      int64_t sz = 0;
-      uint32_t vmax = 0, kmax = 0xFFFFFFFFu;
-      int32_t zabs = 0;
-      auto visit = [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
-        const int32_t x = (int32_t)(idx % a.width);
-        const int32_t y = (int32_t)(idx / a.width);
-        x0 = min(x0, x);
-        x1 = max(x1, x + 1);
-        y0 = min(y0, y);
-        y1 = max(y1, y + 1);
-        z0 = min(z0, z);
-        z1 = max(z1, z + 1);
-        sv += v;
-        svv += (uint64_t)v * v; // exact while v < 2^32
-        sx += (uint64_t)v * (uint64_t)(2 * x + 1);
-        sy += (uint64_t)v * (uint64_t)(2 * y + 1);
-        sz += (int64_t)v * (int64_t)(2 * (int64_t)z + 1);
-        zabs = max(zabs, z < 0 ? -z : z);
-        if (v > vmax) { // lanes see members in order, so first per lane
-          vmax = v;
-          kmax = k;
-        }
-      };
-      if (held) {
+      kind = finish_spot(
+          a, spot, sum, sum_sq, weighted, peak,
+          [&](const double (&centre)[3], double (&delta_sq)[3]) {
+            double acc = 0.0; // lane a < 3: axis a, in member order
+            for (uint32_t k0 = 0; k0 < n; k0 += kTermRound) {
 #pragma unroll
-        for (int j = 0; j < KMAX; j++) {
-          const uint32_t k = lane + 64u * j;
-          if (k < n)
-            visit(k, m_idx[j], m_val[j], m_z[j]);
-        }
-      } else {
-        for (uint32_t k = lane; k < n; k += 64)
-          visit(k, mem_index[s0 + k], mem_value[s0 + k], mem_frame[s0 + k]);
-      }
-      x0 = wave_min(x0);
-      x1 = wave_max(x1);
-      y0 = wave_min(y0);
-      y1 = wave_max(y1);
-      z0 = wave_min(z0);
-      z1 = wave_max(z1);
-      zabs = wave_max(zabs);
-      const uint32_t vtop = wave_max(vmax);
-      const uint32_t ktop = wave_min(vmax == vtop ? kmax : 0xFFFFFFFFu);
-      sf_spot &spot = r.spot;
-      // Exactness of the reference's double sums: every term and partial
-      // sum below 2^52 (with margin) in magnitude.
-      const double bound = 4503599627370496.0; // 2^52
-      const double top = (double)vtop, cnt = (double)n;
-      const double reach = (double)max(max(a.width, a.pixels / a.width),
-                                       (uint32_t)zabs) * 2.0 + 1.0;
-      const bool exact = cnt * top * top < bound && cnt * top * reach < bound;
-      if (!exact) {
-        if (lane == 0)
-          kind = emit_sequential(a, s0, n, mem_frame, mem_index, mem_value,
-                                 spot);
-        kind = __shfl(kind, 0, 64);
-      } else {
-        sv = wave_sum(sv);
-        svv = wave_sum(svv);
-        sx = wave_sum(sx);
-        sy = wave_sum(sy);
-        sz = wave_sum(sz);
-        spot.n_signal = n;
-        spot.bbox[0] = x0;
-        spot.bbox[1] = x1;
-        spot.bbox[2] = y0;
-        spot.bbox[3] = y1;
-        spot.bbox[4] = z0;
-        spot.bbox[5] = z1;
-        const double sum = (double)sv, sum_sq = (double)svv;
-        const double weighted[3] = {(double)sx * 0.5, (double)sy * 0.5,
-                                    (double)sz * 0.5};
-        double peak[3] = {0.0, 0.0, 0.0};
-        if (vtop > 0) {
-          const uint32_t idx = mem_index[s0 + ktop];
-          peak[0] = (double)(idx % a.width) + 0.5;
-          peak[1] = (double)(idx / a.width) + 0.5;
-          peak[2] = (double)mem_frame[s0 + ktop] + 0.5;
-        }
-        // the variance terms of one 64-member chunk, added in member order
-        auto chunk_terms = [&](uint32_t k0, uint32_t idx, uint32_t v,
-                               int32_t z, const double (&centre)[3],
-                               double (&delta_sq)[3]) {
-          double t[3] = {0.0, 0.0, 0.0};
-          if (k0 + lane < n) {
-            const double value = (double)v;
-            const double position[3] = {(double)(idx % a.width) + 0.5,
-                                        (double)(idx / a.width) + 0.5,
-                                        (double)z + 0.5};
-            for (int axis = 0; axis < 3; axis++) {
-              const double delta = position[axis] - centre[axis];
-              t[axis] = value * delta * delta;
-            }
-          }
-          // terms through LDS, then every lane adds them in member order
-          // (the same sums in every lane; the reads are broadcasts)
-          double *mine = terms[threadIdx.x >> 6];
-          for (int axis = 0; axis < 3; axis++)
-            mine[axis * 64 + lane] = t[axis];
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          const uint32_t m = min(64u, n - k0);
-          for (uint32_t i = 0; i < m; i++)
-            for (int axis = 0; axis < 3; axis++)
-              delta_sq[axis] += mine[axis * 64 + i];
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        };
-        kind = finish_spot(
-            a, spot, sum, sum_sq, weighted, peak,
-            [&](const double (&centre)[3], double (&delta_sq)[3]) {
-              if (held) {
-#pragma unroll
-                for (int j = 0; j < KMAX; j++)
-                  if (64u * j < n)
-                    chunk_terms(64u * j, m_idx[j], m_val[j], m_z[j], centre,
-                                delta_sq);
-              } else {
-                for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-                  uint32_t idx = 0, v = 0;
-                  int32_t z = 0;
-                  if (k0 + lane < n) {
-                    idx = mem_index[s0 + k0 + lane];
-                    v = mem_value[s0 + k0 + lane];
-                    z = mem_frame[s0 + k0 + lane];
-                  }
-                  chunk_terms(k0, idx, v, z, centre, delta_sq);
+              for (int j = 0; j < 4; j++) {
+                const uint32_t k = k0 + lane + 64u * j;
+                uint32_t idx = m_idx[j], v = m_val[j];
+                int32_t z = m_z[j];
+                if (k0 > 0) {
+                  idx = k < n ? mem_index[s0 + k] : 0;
+                  v = k < n ? mem_value[s0 + k] : 0;
+                  z = k < n ? mem_frame[s0 + k] : 0;
+                }
+                const uint32_t y = row_of(a, idx);
+                const double value = (double)v;
+                const double position[3] = {
+                    (double)(idx - y * a.width) + 0.5, (double)y + 0.5,
+                    (double)z + 0.5};
+                for (int axis = 0; axis < 3; axis++) {
+                  const double delta = position[axis] - centre[axis];
+                  mine[axis * kTermStride + lane + 64u * j] =
+                      value * delta * delta;
                 }
               }
-            });
-      }
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+              if (lane < 3) {
+                // eight terms read ahead of their eight dependent adds
+                const uint32_t m = min(kTermRound, n - k0);
+                const double *row = mine + lane * kTermStride;
+                uint32_t i = 0;
+                for (; i + 8 <= m; i += 8) {
+                  double r8[8];
+#pragma unroll
+                  for (int q = 0; q < 8; q++)
+                    r8[q] = row[i + q];
+#pragma unroll
+                  for (int q = 0; q < 8; q++)
+                    acc += r8[q];
+                }
+                for (; i < m; i++)
+                  acc += row[i];
+              }
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            for (int axis = 0; axis < 3; axis++)
+              delta_sq[axis] = readlane_f64(acc, axis);
+          });
     }
     if (kind == kAccepted) {
       r.accepted = 1;
@@ -674,25 +950,7 @@ __global__ __launch_bounds__(256) void finalise(
       tally[kind]++;
     }
   }
-  // counts: groups, too_small, too_large, separated, accepted -- summed
-  // over the block first, one atomic per counter and block
-  __shared__ uint32_t block_tally[4];
-  if (threadIdx.x < 4)
-    block_tally[threadIdx.x] = 0;
-  __syncthreads();
-  if (lane == 0)
-    for (int t = 0; t < 4; t++)
-      if (tally[t])
-        atomicAdd(&block_tally[t], tally[t]);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int slot[4] = {5, 2, 3, 4}; // kAccepted, kTooSmall, ...
-    for (int t = 0; t < 4; t++)
-      if (block_tally[t])
-        atomicAdd(&counts[slot[t]], (unsigned long long)block_tally[t]);
-    if (blockIdx.x == 0)
-      atomicAdd(&counts[1], (unsigned long long)comps);
-  }
+  add_tallies(tally, counts);
 }
 
 // Per component: accepted << 32 | kept members, so that one exclusive scan
@@ -723,31 +981,40 @@ __global__ void write_spots(const SpotOut *res, const uint32_t *n_comps,
   }
 }
 
-// Appends the kept spots' pixels, one thread per member: entry i of the
-// emission order belongs to component start_rank[i] + start_flag[i] - 1 and
-// is member i - starts[c] of it.
-__global__ void write_pixels(const SpotOut *res, const uint32_t *n_closed,
-                             const uint32_t *start_flag,
-                             const uint32_t *start_rank,
-                             const uint32_t *starts,
-                             const unsigned long long *rank,
-                             const int32_t *mem_frame,
-                             const uint32_t *mem_index,
-                             const uint32_t *mem_value,
+// Appends the kept spots' pixels, one thread per node: node g of run r, at
+// sorted place i, belongs to component c = (i's scan rank, high half) + i's
+// start bit - 1, and is member (i's first member) + (g - r's head).
+__global__ void write_pixels(LabelArgs a, const SpotOut *res,
+                             const uint32_t *d_closed_runs,
+                             const uint32_t *run_of, const uint32_t *pos_of,
+                             const uint32_t *run_head,
+                             const unsigned long long *pack,
+                             const unsigned long long *run_rank,
+                             const uint32_t *comp_mem,
+                             const unsigned long long *spot_rank,
+                             const int32_t *node_frame,
+                             const uint32_t *node_index,
+                             const uint32_t *node_value,
                              const unsigned long long *totals,
                              sf_pixel *pixels) {
-  const uint32_t end = *n_closed;
+  const uint32_t closed = *d_closed_runs;
   const unsigned long long pixel_base = totals[1];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end;
-       i += gridDim.x * blockDim.x) {
-    const uint32_t c = start_rank[i] + start_flag[i] - 1;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
+       g += gridDim.x * blockDim.x) {
+    const uint32_t r = run_of[g];
+    const uint32_t i = pos_of[r];
+    if (i >= closed)
+      continue;
+    const unsigned long long p = pack[i], q = run_rank[i];
+    const uint32_t c = (uint32_t)(q >> 32) + (uint32_t)(p >> 32) - 1;
     if (!res[c].accepted)
       continue;
-    sf_pixel p;
-    p.frame = mem_frame[i];
-    p.index = mem_index[i];
-    p.value = mem_value[i];
-    pixels[pixel_base + (uint32_t)rank[c] + (i - starts[c])] = p;
+    const uint32_t at = (uint32_t)q + (g - run_head[r]);
+    sf_pixel px;
+    px.frame = node_frame[g];
+    px.index = node_index[g];
+    px.value = node_value[g];
+    pixels[pixel_base + (uint32_t)spot_rank[c] + (at - comp_mem[c])] = px;
   }
 }
 
@@ -763,32 +1030,32 @@ __global__ void advance_totals(const uint32_t *n_comps,
   }
 }
 
-// The open nodes, in node order, become the next frontier; each points at
-// its component's first open node (the root, which is open with it).
-__global__ void carry_out(LabelArgs a, const uint32_t *open_flag,
-                          const uint32_t *open_rank, const uint32_t *root,
-                          const int32_t *node_frame,
+// The open runs' pixels, in node order, become the next frontier; each
+// points at its component's first pixel -- the first pixel of its root run,
+// which is open with it. One thread per node.
+__global__ void carry_out(LabelArgs a, const uint32_t *run_of,
+                          const uint32_t *open_len, const uint32_t *open_at,
+                          const uint32_t *root, const uint32_t *run_head,
+                          const uint32_t *n_runs, const int32_t *node_frame,
                           const uint32_t *node_index,
                           const uint32_t *node_value, int32_t *carry_frame,
                           uint32_t *carry_index, uint32_t *carry_value,
                           uint32_t *carry_root, uint32_t *n_carry_out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const uint32_t last = *n_runs - 1;
+    *n_carry_out = open_at[last] + open_len[last];
+  }
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
        g += gridDim.x * blockDim.x) {
-    if (g + 1 == a.n_nodes)
-      *n_carry_out = open_rank[g] + open_flag[g];
-    if (!open_flag[g])
+    const uint32_t r = run_of[g];
+    if (!open_len[r])
       continue;
-    const uint32_t at = open_rank[g];
+    const uint32_t at = open_at[r] + (g - run_head[r]);
     carry_frame[at] = node_frame[g];
     carry_index[at] = node_index[g];
     carry_value[at] = node_value[g];
-    carry_root[at] = open_rank[root[g]];
+    carry_root[at] = open_at[root[r]];
   }
-}
-
-__global__ void count_components(const uint32_t *rank, const uint32_t *flag,
-                                 uint32_t n, uint32_t *out) {
-  *out = rank[n - 1] + flag[n - 1];
 }
 
 __global__ void add_signal_count(const ChunkDesc *frames, uint32_t n_frames,
@@ -875,20 +1142,26 @@ struct sf_labeller {
   uint32_t carry_tail = 0;   // first carried node on the last frame
   uint32_t last_count = 0;   // pixels of the last labelled frame
   uint32_t *d_n_carry = nullptr;
-  uint32_t *h_scalars = nullptr; // pinned: n_carry
-  // node arrays
+  uint32_t *h_scalars = nullptr; // pinned: n_carry, runs
+  // per node
   sf::DevArray<int32_t> node_frame;
-  sf::DevArray<uint32_t> node_index, node_value, node_slot, parent, root,
-      last_slot;
+  sf::DevArray<uint32_t> node_index, node_value, node_slot, run_of;
+  sf::DevArray<uint64_t> head_bits;  // run heads, a bit per node
+  sf::DevArray<uint32_t> head_count; // run heads per tile of nodes
+  // per run (at most one per node)
+  sf::DevArray<uint32_t> run_head, run_idx, run_last, run_slot, parent, root,
+      last_slot, pos_of;
+  sf::DevArray<uint32_t> run_rows; // per (slot, row): first run
   sf::DevArray<uint64_t> keys, keys_sorted;
-  sf::DevArray<uint32_t> row_start; // per (slot, row): first node
-  sf::DevArray<uint32_t> values, order, open_flag, open_rank, start_flag,
-      start_rank, starts;
+  sf::DevArray<uint32_t> values, order, open_len, open_at;
+  sf::DevArray<unsigned long long> run_pack, run_rank;
+  // per component
+  sf::DevArray<uint32_t> starts, comp_mem;
   sf::DevArray<sf::SpotOut> results;
   sf::DevArray<unsigned long long> spot_pack, spot_rank;
   sf::DevArray<uint8_t> temp;
   sf::DevArray<sf::ChunkDesc> chunk;
-  uint32_t *d_scalars = nullptr; // n_starts, n_closed
+  uint32_t *d_scalars = nullptr; // 8 device counters of a chunk
   // results
   sf::DevArray<sf_spot> spots;
   sf::DevArray<sf_pixel> pixels;
@@ -981,6 +1254,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   a.min_size = l->options.min_spot_size;
   a.max_size = l->options.max_spot_size;
   a.max_separation = l->options.max_separation;
+  a.inv_width = 1.0 / (double)l->width;
   // outputs: at worst every node is kept
   l->bound_pixels += n - l->n_carry;
   l->bound_spots += (n - l->n_carry) / std::max<uint64_t>(1, a.min_size) + 1;
@@ -1022,6 +1296,12 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(l->node_index.ensure(n), "alloc");
   LAB_CHECK(l->node_value.ensure(n), "alloc");
   LAB_CHECK(l->node_slot.ensure(n), "alloc");
+  LAB_CHECK(l->run_of.ensure(n), "alloc");
+  LAB_CHECK(l->run_head.ensure(n), "alloc");
+  LAB_CHECK(l->run_idx.ensure(n), "alloc");
+  LAB_CHECK(l->run_last.ensure(n), "alloc");
+  LAB_CHECK(l->pos_of.ensure(n), "alloc");
+  LAB_CHECK(l->run_slot.ensure(n), "alloc");
   LAB_CHECK(l->parent.ensure(n), "alloc");
   LAB_CHECK(l->root.ensure(n), "alloc");
   LAB_CHECK(l->last_slot.ensure(n), "alloc");
@@ -1029,23 +1309,26 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(l->keys_sorted.ensure(n), "alloc");
   LAB_CHECK(l->values.ensure(n), "alloc");
   LAB_CHECK(l->order.ensure(n), "alloc");
-  LAB_CHECK(l->open_flag.ensure(n), "alloc");
-  LAB_CHECK(l->open_rank.ensure(n), "alloc");
-  LAB_CHECK(l->start_flag.ensure(n), "alloc");
-  LAB_CHECK(l->start_rank.ensure(n), "alloc");
+  LAB_CHECK(l->open_len.ensure(n), "alloc");
+  LAB_CHECK(l->open_at.ensure(n), "alloc");
+  LAB_CHECK(l->run_pack.ensure(n), "alloc");
+  LAB_CHECK(l->run_rank.ensure(n), "alloc");
   LAB_CHECK(l->starts.ensure(n), "alloc");
+  LAB_CHECK(l->comp_mem.ensure(n), "alloc");
   LAB_CHECK(l->results.ensure(n), "alloc");
   LAB_CHECK(l->spot_pack.ensure(n), "alloc");
   LAB_CHECK(l->spot_rank.ensure(n), "alloc");
+  const uint32_t tiles = (n + sf::kTile - 1) / sf::kTile;
+  LAB_CHECK(l->head_bits.ensure(n / 64 + 1), "alloc");
+  LAB_CHECK(l->head_count.ensure(tiles), "alloc");
   const int in = l->carry_side, outside = 1 - l->carry_side;
   LAB_CHECK(l->carry_frame[outside].ensure(n), "alloc");
   LAB_CHECK(l->carry_index[outside].ensure(n), "alloc");
   LAB_CHECK(l->carry_value[outside].ensure(n), "alloc");
   LAB_CHECK(l->carry_root[outside].ensure(n), "alloc");
   LAB_CHECK(l->chunk.ensure(std::max<uint32_t>(n_frames, 1)), "alloc");
-  // temp storage: the largest of the rocPRIM calls
-  size_t need = 0, bytes = 0;
-  // Sort keys: 32 bits when (last slot, root) fit, else 64.
+  // Sort keys: 32 bits when (last slot, root run) fit, else 64. The runs
+  // are at most the nodes.
   int root_bits = 1, slot_bits = 1;
   while (root_bits < 32 && (1ull << root_bits) < (uint64_t)n)
     root_bits++;
@@ -1054,28 +1337,13 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   const bool narrow = root_bits + slot_bits <= 32;
   uint32_t *keys32 = reinterpret_cast<uint32_t *>(l->keys.ptr);
   uint32_t *keys32_sorted = reinterpret_cast<uint32_t *>(l->keys_sorted.ptr);
-  if (narrow)
-    LAB_CHECK(sort32(nullptr, bytes, keys32, keys32_sorted, l->values.ptr,
-                     l->order.ptr, n, root_bits + slot_bits, s),
-              "sort size");
-  else
-    LAB_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, l->keys.ptr,
-                                        l->keys_sorted.ptr, l->values.ptr,
-                                        l->order.ptr, n, 0, 64, s),
-              "sort size");
-  need = std::max(need, bytes);
-  LAB_CHECK(rocprim::exclusive_scan(nullptr, bytes, l->open_flag.ptr,
-                                    l->open_rank.ptr, 0u, (size_t)n,
-                                    rocprim::plus<uint32_t>(), s),
-            "scan size");
-  need = std::max(need, bytes);
-  LAB_CHECK(rocprim::exclusive_scan(nullptr, bytes, l->spot_pack.ptr,
-                                    l->spot_rank.ptr, 0ull, (size_t)n,
-                                    rocprim::plus<unsigned long long>(), s),
-            "scan size");
-  need = std::max(need, bytes);
-  LAB_CHECK(l->temp.ensure(need + 256), "alloc temp");
-
+  // d_scalars: components, closed members, closed runs, components left to
+  // the wave-per-component pass, runs
+  uint32_t *d_comps = l->d_scalars, *d_closed_mem = l->d_scalars + 1,
+           *d_closed_runs = l->d_scalars + 2, *n_wide = l->d_scalars + 3,
+           *n_runs = l->d_scalars + 4;
+  LAB_CHECK(hipMemsetAsync(l->d_scalars, 0, 8 * sizeof(uint32_t), s),
+            "memset");
   if (n_frames)
     LAB_CHECK(hipMemcpyAsync(l->chunk.ptr, frames.data(),
                              n_frames * sizeof(sf::ChunkDesc),
@@ -1090,11 +1358,11 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
     LAB_CHECK(hipStreamWaitEvent(gs, l->done, 0), "event wait");
   }
   const uint32_t grid = grid_for(n);
-  sf::gather_nodes<<<grid, 256, 0, gs>>>(
+  sf::gather_nodes<<<tiles, 256, 0, gs>>>(
       a, l->chunk.ptr, l->carry_frame[in].ptr, l->carry_index[in].ptr,
-      l->carry_value[in].ptr, l->carry_root[in].ptr, l->node_frame.ptr,
-      l->node_index.ptr, l->node_value.ptr, l->node_slot.ptr, l->parent.ptr,
-      l->last_slot.ptr);
+      l->carry_value[in].ptr, l->node_frame.ptr, l->node_index.ptr,
+      l->node_value.ptr, l->node_slot.ptr, l->head_bits.ptr,
+      l->head_count.ptr);
   if (n_frames)
     sf::add_signal_count<<<1, 1, 0, gs>>>(l->chunk.ptr, n_frames,
                                           l->d_counts);
@@ -1103,108 +1371,141 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
     LAB_CHECK(hipEventRecord(l->done, gs), "event");
     LAB_CHECK(hipStreamWaitEvent(s, l->done, 0), "event wait");
   }
+  sf::number_runs<<<tiles, 256, 0, s>>>(
+      a, l->head_bits.ptr, l->head_count.ptr, l->node_index.ptr,
+      l->node_slot.ptr, l->carry_root[in].ptr, l->run_of.ptr,
+      l->run_head.ptr, l->run_idx.ptr, l->run_last.ptr, l->run_slot.ptr,
+      l->parent.ptr, l->last_slot.ptr, n_runs);
   {
     const size_t rows = (size_t)(n_frames + 1) * (l->height + 1);
-    LAB_CHECK(l->row_start.ensure(rows), "alloc");
-    sf::fill_rows<<<grid_for(rows), 256, 0, s>>>(a, l->chunk.ptr,
-                                                 l->node_index.ptr,
-                                                 l->row_start.ptr);
+    LAB_CHECK(l->run_rows.ensure(rows), "alloc");
+    sf::fill_run_rows<<<grid_for(rows), 256, 0, s>>>(
+        a, l->chunk.ptr, l->run_of.ptr, l->run_idx.ptr, n_runs,
+        l->run_rows.ptr);
   }
-  // `values` holds each node's run start until make_keys reuses it
-  sf::link_runs<<<grid, 256, 0, s>>>(a, l->chunk.ptr, l->node_index.ptr,
-                                     l->node_slot.ptr, l->parent.ptr,
-                                     l->values.ptr);
-  sf::unite_edges<<<grid, 256, 0, s>>>(a, l->chunk.ptr, l->node_index.ptr,
-                                       l->node_slot.ptr, l->row_start.ptr,
-                                       l->parent.ptr);
-  sf::resolve_roots<<<grid, 256, 0, s>>>(a, l->parent.ptr, l->values.ptr,
-                                         l->node_slot.ptr, l->root.ptr,
-                                         l->last_slot.ptr);
-  sf::resolve_runs<<<grid, 256, 0, s>>>(a, l->values.ptr, l->root.ptr);
+  // the run count is on the device: bounded grids that stride (a multiple of
+  // 8 blocks for the XCD mapping of unite_runs)
+  const uint32_t run_grid = std::min<uint32_t>((grid + 7) & ~7u, 4096);
+  sf::unite_runs<<<run_grid, 256, 0, s>>>(
+      a, l->chunk.ptr, l->run_idx.ptr, l->run_last.ptr, l->run_slot.ptr,
+      l->run_rows.ptr, n_runs, l->parent.ptr);
+  sf::resolve_runs<<<run_grid, 256, 0, s>>>(l->parent.ptr, l->run_slot.ptr,
+                                            n_runs, l->root.ptr,
+                                            l->last_slot.ptr);
   if (narrow)
-    sf::make_keys<uint32_t><<<grid, 256, 0, s>>>(
-        a, l->root.ptr, l->last_slot.ptr, keys32, l->values.ptr,
-        l->open_flag.ptr, root_bits);
+    sf::make_keys<uint32_t><<<run_grid, 256, 0, s>>>(
+        a, l->root.ptr, l->last_slot.ptr, l->run_head.ptr, n_runs, keys32,
+        l->values.ptr, l->open_len.ptr, root_bits);
   else
-    sf::make_keys<uint64_t><<<grid, 256, 0, s>>>(
-        a, l->root.ptr, l->last_slot.ptr, l->keys.ptr, l->values.ptr,
-        l->open_flag.ptr, 32);
+    sf::make_keys<uint64_t><<<run_grid, 256, 0, s>>>(
+        a, l->root.ptr, l->last_slot.ptr, l->run_head.ptr, n_runs,
+        l->keys.ptr, l->values.ptr, l->open_len.ptr, 32);
   LAB_CHECK(hipGetLastError(), "label kernels");
+  // the sort and the scans take their sizes from the host
+  LAB_CHECK(hipMemcpyAsync(l->h_scalars + 1, n_runs, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s),
+            "reading the run count");
+  LAB_CHECK(hipStreamSynchronize(s), "reading the run count");
+  const uint32_t runs = l->h_scalars[1];
+  // temp storage: the largest of the rocPRIM calls
+  size_t need = 0, bytes = 0;
+  if (narrow)
+    LAB_CHECK(sort32(nullptr, bytes, keys32, keys32_sorted, l->values.ptr,
+                     l->order.ptr, runs, root_bits + slot_bits, s),
+              "sort size");
+  else
+    LAB_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, l->keys.ptr,
+                                        l->keys_sorted.ptr, l->values.ptr,
+                                        l->order.ptr, runs, 0, 64, s),
+              "sort size");
+  need = std::max(need, bytes);
+  LAB_CHECK(rocprim::exclusive_scan(nullptr, bytes, l->open_len.ptr,
+                                    l->open_at.ptr, 0u, (size_t)runs,
+                                    rocprim::plus<uint32_t>(), s),
+            "scan size");
+  need = std::max(need, bytes);
+  LAB_CHECK(rocprim::exclusive_scan(nullptr, bytes, l->run_pack.ptr,
+                                    l->run_rank.ptr, 0ull, (size_t)runs,
+                                    rocprim::plus<unsigned long long>(), s),
+            "scan size");
+  need = std::max(need, bytes);
+  LAB_CHECK(l->temp.ensure(need + 256), "alloc temp");
+  const uint32_t rgrid = grid_for(runs);
   bytes = l->temp.cap;
   if (narrow) {
     LAB_CHECK(sort32(l->temp.ptr, bytes, keys32, keys32_sorted,
-                     l->values.ptr, l->order.ptr, n, root_bits + slot_bits,
-                     s),
+                     l->values.ptr, l->order.ptr, runs,
+                     root_bits + slot_bits, s),
               "sorting components");
     // closed keys stay below 2^(root_bits + slot_bits) - 1, so the open key
     // (all ones) sorts last on those bits too
-    sf::mark_starts<uint32_t><<<grid, 256, 0, s>>>(a, keys32_sorted,
-                                                   ~0u, l->start_flag.ptr);
+    sf::mark_runs<uint32_t><<<rgrid, 256, 0, s>>>(
+        a, keys32_sorted, ~0u, l->order.ptr, l->run_head.ptr, runs,
+        l->run_pack.ptr, l->pos_of.ptr, d_closed_runs);
   } else {
     LAB_CHECK(rocprim::radix_sort_pairs(l->temp.ptr, bytes, l->keys.ptr,
                                         l->keys_sorted.ptr, l->values.ptr,
-                                        l->order.ptr, n, 0, 64, s),
+                                        l->order.ptr, runs, 0, 64, s),
               "sorting components");
-    sf::mark_starts<uint64_t><<<grid, 256, 0, s>>>(a, l->keys_sorted.ptr,
-                                                   ~0ull, l->start_flag.ptr);
+    sf::mark_runs<uint64_t><<<rgrid, 256, 0, s>>>(
+        a, l->keys_sorted.ptr, ~0ull, l->order.ptr, l->run_head.ptr, runs,
+        l->run_pack.ptr, l->pos_of.ptr, d_closed_runs);
   }
   bytes = l->temp.cap;
-  LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->start_flag.ptr,
-                                    l->start_rank.ptr, 0u, (size_t)n,
-                                    rocprim::plus<uint32_t>(), s),
-            "ranking components");
-  LAB_CHECK(hipMemsetAsync(l->d_scalars, 0, 2 * sizeof(uint32_t), s),
-            "memset");
-  if (narrow)
-    sf::list_starts<uint32_t><<<grid, 256, 0, s>>>(
-        a, l->start_flag.ptr, l->start_rank.ptr, l->starts.ptr, keys32_sorted,
-        ~0u, l->d_scalars + 1);
-  else
-    sf::list_starts<uint64_t><<<grid, 256, 0, s>>>(
-        a, l->start_flag.ptr, l->start_rank.ptr, l->starts.ptr,
-        l->keys_sorted.ptr, ~0ull, l->d_scalars + 1);
-  sf::count_components<<<1, 1, 0, s>>>(l->start_rank.ptr, l->start_flag.ptr,
-                                       n, l->d_scalars);
+  LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->run_pack.ptr,
+                                    l->run_rank.ptr, 0ull, (size_t)runs,
+                                    rocprim::plus<unsigned long long>(), s),
+            "ranking components and their members");
+  sf::list_starts<<<rgrid, 256, 0, s>>>(l->run_pack.ptr, l->run_rank.ptr,
+                                        runs, d_closed_runs, l->starts.ptr,
+                                        l->comp_mem.ptr, l->d_scalars);
   // the members in emission order, in the sort's input arrays (dead now)
   int32_t *mem_frame = reinterpret_cast<int32_t *>(l->values.ptr);
   uint32_t *mem_index = reinterpret_cast<uint32_t *>(l->keys.ptr);
   uint32_t *mem_value = mem_index + n;
-  sf::gather_members<<<grid, 256, 0, s>>>(l->d_scalars + 1, l->order.ptr,
-                                          l->node_frame.ptr,
-                                          l->node_index.ptr,
-                                          l->node_value.ptr, mem_frame,
-                                          mem_index, mem_value);
-  // a bounded grid: each wave loops over components, each block adds its
-  // tallies once
-  sf::finalise<<<std::min<uint32_t>(grid, 1024), 256, 0, s>>>(
-      a, l->starts.ptr, l->d_scalars, l->d_scalars + 1, mem_frame, mem_index,
-      mem_value, l->results.ptr, l->d_counts);
-  sf::spot_flags<<<grid, 256, 0, s>>>(l->results.ptr, l->d_scalars,
-                                      l->spot_pack.ptr);
+  sf::gather_members<<<grid, 256, 0, s>>>(
+      a, d_closed_runs, l->run_of.ptr, l->pos_of.ptr, l->run_head.ptr,
+      l->run_rank.ptr, l->node_frame.ptr, l->node_index.ptr,
+      l->node_value.ptr, mem_frame, mem_index, mem_value);
+  // components past kLaneMembers members go to the wave-per-component pass,
+  // listed in `order` (no longer needed once the members are gathered); each
+  // pass loops over components on a bounded grid
+  uint32_t *wide = l->order.ptr;
+  sf::finalise_narrow<<<std::min<uint32_t>(rgrid, 1024), 256, 0, s>>>(
+      a, l->comp_mem.ptr, d_comps, d_closed_mem, mem_frame, mem_index,
+      mem_value, l->results.ptr, wide, n_wide, l->d_counts);
+  sf::finalise_wide<<<std::min<uint32_t>(rgrid, 1024), 256, 0, s>>>(
+      a, l->comp_mem.ptr, d_comps, d_closed_mem, mem_frame, mem_index,
+      mem_value, wide, n_wide, l->results.ptr, l->d_counts);
+  sf::spot_flags<<<rgrid, 256, 0, s>>>(l->results.ptr, d_comps,
+                                       l->spot_pack.ptr);
   bytes = l->temp.cap;
+  // components are at most the runs
   LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->spot_pack.ptr,
-                                    l->spot_rank.ptr, 0ull, (size_t)n,
+                                    l->spot_rank.ptr, 0ull, (size_t)runs,
                                     rocprim::plus<unsigned long long>(), s),
             "ranking spots and their pixels");
-  sf::write_spots<<<grid, 256, 0, s>>>(l->results.ptr, l->d_scalars,
-                                       l->spot_rank.ptr, l->d_totals,
-                                       l->spots.ptr);
+  sf::write_spots<<<rgrid, 256, 0, s>>>(l->results.ptr, d_comps,
+                                        l->spot_rank.ptr, l->d_totals,
+                                        l->spots.ptr);
   sf::write_pixels<<<grid, 256, 0, s>>>(
-      l->results.ptr, l->d_scalars + 1, l->start_flag.ptr, l->start_rank.ptr,
-      l->starts.ptr, l->spot_rank.ptr, mem_frame, mem_index, mem_value,
-      l->d_totals, l->pixels.ptr);
-  sf::advance_totals<<<1, 1, 0, s>>>(l->d_scalars, l->spot_rank.ptr,
+      a, l->results.ptr, d_closed_runs, l->run_of.ptr, l->pos_of.ptr,
+      l->run_head.ptr, l->run_pack.ptr, l->run_rank.ptr, l->comp_mem.ptr,
+      l->spot_rank.ptr, l->node_frame.ptr, l->node_index.ptr,
+      l->node_value.ptr, l->d_totals, l->pixels.ptr);
+  sf::advance_totals<<<1, 1, 0, s>>>(d_comps, l->spot_rank.ptr,
                                      l->results.ptr, l->d_totals);
   bytes = l->temp.cap;
-  LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->open_flag.ptr,
-                                    l->open_rank.ptr, 0u, (size_t)n,
+  LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->open_len.ptr,
+                                    l->open_at.ptr, 0u, (size_t)runs,
                                     rocprim::plus<uint32_t>(), s),
             "ranking the frontier");
   sf::carry_out<<<grid, 256, 0, s>>>(
-      a, l->open_flag.ptr, l->open_rank.ptr, l->root.ptr, l->node_frame.ptr,
-      l->node_index.ptr, l->node_value.ptr, l->carry_frame[outside].ptr,
-      l->carry_index[outside].ptr, l->carry_value[outside].ptr,
-      l->carry_root[outside].ptr, l->d_n_carry);
+      a, l->run_of.ptr, l->open_len.ptr, l->open_at.ptr, l->root.ptr,
+      l->run_head.ptr, n_runs, l->node_frame.ptr, l->node_index.ptr,
+      l->node_value.ptr,
+      l->carry_frame[outside].ptr, l->carry_index[outside].ptr,
+      l->carry_value[outside].ptr, l->carry_root[outside].ptr, l->d_n_carry);
   LAB_CHECK(hipGetLastError(), "finalise kernels");
   LAB_CHECK(hipMemcpyAsync(l->h_scalars, l->d_n_carry, sizeof(uint32_t),
                            hipMemcpyDeviceToHost, s),
@@ -1349,7 +1650,7 @@ sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
   ok = ok && hipEventCreateWithFlags(&l->done, hipEventDisableTiming) ==
                  hipSuccess;
   ok = ok && hipMalloc(&l->d_n_carry, sizeof(uint32_t)) == hipSuccess;
-  ok = ok && hipMalloc(&l->d_scalars, 4 * sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipMalloc(&l->d_scalars, 8 * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipMalloc(&l->d_counts, 6 * sizeof(unsigned long long)) ==
                  hipSuccess;
   ok = ok && hipMalloc(&l->d_totals, 2 * sizeof(unsigned long long)) ==
@@ -1386,19 +1687,28 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   l->node_index.release();
   l->node_value.release();
   l->node_slot.release();
+  l->run_of.release();
+  l->head_bits.release();
+  l->head_count.release();
+  l->run_rows.release();
+  l->run_head.release();
+  l->run_idx.release();
+  l->run_last.release();
+  l->pos_of.release();
+  l->run_slot.release();
   l->parent.release();
   l->root.release();
   l->last_slot.release();
   l->keys.release();
-  l->row_start.release();
   l->keys_sorted.release();
   l->values.release();
   l->order.release();
-  l->open_flag.release();
-  l->open_rank.release();
-  l->start_flag.release();
-  l->start_rank.release();
+  l->open_len.release();
+  l->open_at.release();
+  l->run_pack.release();
+  l->run_rank.release();
   l->starts.release();
+  l->comp_mem.release();
   l->results.release();
   l->spot_pack.release();
   l->spot_rank.release();
