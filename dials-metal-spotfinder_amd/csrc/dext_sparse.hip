@@ -799,7 +799,7 @@ __device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i) {
         for (int q = 0; q < 5; q++)
           if ((uint32_t)q < k && src + q < fd.capacity)
             fd.dstof[src + q] = dst + q;
-        start[t] += (uint32_t)__shfl((int)incl, 63, 64);
+        start[t] += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
       }
     }
   }

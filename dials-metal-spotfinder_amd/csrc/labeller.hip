@@ -250,13 +250,14 @@ __global__ __launch_bounds__(256) void gather_nodes(
 // parent -- itself, or for a carried node its carried root: carried nodes
 // are all heads and come first, so their run numbers are their node
 // numbers -- and a zero last_slot. The block of the last tile writes the run
-// count.
+// count, here and to the host's mapped copy.
 __global__ __launch_bounds__(256) void number_runs(
     LabelArgs a, const uint64_t *head_bits, const uint32_t *head_count,
     const uint32_t *node_index, const uint32_t *node_slot,
     const uint32_t *carry_root, uint32_t *run_of, uint32_t *run_head,
     uint32_t *run_idx, uint32_t *run_last, uint32_t *run_slot,
-    uint32_t *parent, uint32_t *last_slot, uint32_t *n_runs) {
+    uint32_t *parent, uint32_t *last_slot, uint32_t *n_runs,
+    uint32_t *host_runs) {
   // part[w]: wave w's share of the earlier tiles' heads; part[4 + w]: the
   // heads among wave w's own 512 nodes
   __shared__ uint32_t part[8];
@@ -281,8 +282,11 @@ __global__ __launch_bounds__(256) void number_runs(
   uint32_t run = tiles_before; // heads before this wave's nodes
   for (uint32_t w = 0; w < wave; w++)
     run += part[4 + w];
-  if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0)
+  if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0) {
     *n_runs = tiles_before + part[4] + part[5] + part[6] + part[7];
+    *host_runs = *n_runs; // host-mapped
+    __threadfence_system();
+  }
   const uint64_t upto = ~0ull >> (63 - lane);
   for (uint32_t j = 0; j < 8; j++) {
     const uint32_t g = first + 64 * j + lane;
@@ -1018,15 +1022,20 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
   }
 }
 
+// The spots and pixels written so far, also into the host's mapped copy.
 __global__ void advance_totals(const uint32_t *n_comps,
                                const unsigned long long *rank,
                                const SpotOut *res,
-                               unsigned long long *totals) {
+                               unsigned long long *totals,
+                               unsigned long long *host_totals) {
   const uint32_t comps = *n_comps;
   if (comps) {
     const uint32_t last = comps - 1;
     totals[0] += (rank[last] >> 32) + res[last].accepted;
     totals[1] += (uint32_t)rank[last] + res[last].members;
+    host_totals[0] = totals[0];
+    host_totals[1] = totals[1];
+    __threadfence_system();
   }
 }
 
@@ -1043,7 +1052,8 @@ __global__ void carry_out(LabelArgs a, const uint32_t *run_of,
                           uint32_t *carry_root, uint32_t *n_carry_out) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const uint32_t last = *n_runs - 1;
-    *n_carry_out = open_at[last] + open_len[last];
+    *n_carry_out = open_at[last] + open_len[last]; // host-mapped
+    __threadfence_system();
   }
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
        g += gridDim.x * blockDim.x) {
@@ -1058,14 +1068,17 @@ __global__ void carry_out(LabelArgs a, const uint32_t *run_of,
   }
 }
 
-__global__ void add_signal_count(const ChunkDesc *frames, uint32_t n_frames,
-                                 unsigned long long *counts) {
-  unsigned long long total = 0;
-  for (uint32_t k = 0; k < n_frames; k++) {
+// The chunk's signal pixels (frames whose list fit), by one wave.
+__global__ __launch_bounds__(64) void add_signal_count(
+    const ChunkDesc *frames, uint32_t n_frames, unsigned long long *counts) {
+  uint32_t total = 0; // a chunk's lists hold fewer than 2^32 pixels
+  for (uint32_t k = threadIdx.x; k < n_frames; k += 64) {
     const uint32_t c = *frames[k].count;
     total += c <= frames[k].capacity ? c : 0;
   }
-  counts[0] += total;
+  total = wave_reduce(total, 0u, [](uint32_t p, uint32_t q) { return p + q; });
+  if (threadIdx.x == 0)
+    counts[0] += total;
 }
 
 } // namespace sf
@@ -1141,8 +1154,13 @@ struct sf_labeller {
   uint32_t n_carry = 0;      // host copy, valid after a sync
   uint32_t carry_tail = 0;   // first carried node on the last frame
   uint32_t last_count = 0;   // pixels of the last labelled frame
-  uint32_t *d_n_carry = nullptr;
-  uint32_t *h_scalars = nullptr; // pinned: n_carry, runs
+  // pinned and mapped, written by the kernels: the frontier size of the
+  // last chunk, the current chunk's run count; the spots and pixels written
+  uint32_t *h_scalars = nullptr, *hd_scalars = nullptr;
+  unsigned long long *h_totals = nullptr, *hd_totals = nullptr;
+  hipEvent_t runs_ready = nullptr;
+  sf::ChunkDesc *h_chunk = nullptr; // pinned staging of the chunk table
+  uint32_t h_chunk_cap = 0;
   // per node
   sf::DevArray<int32_t> node_frame;
   sf::DevArray<uint32_t> node_index, node_value, node_slot, run_of;
@@ -1196,13 +1214,21 @@ uint32_t grid_for(size_t n) {
   return (uint32_t)std::min<size_t>(std::max<size_t>(blocks, 1), 65535);
 }
 
-// The component sort on 32-bit keys: keys of 25 to 30 bits go in three
-// 10-bit passes instead of four 8-bit ones.
+// The component sort. rocPRIM sorts up to a million items by merge sort
+// unless told otherwise (about twenty passes over a chunk's runs); above 64K
+// items these configurations take the radix (onesweep) path, and keys of 25
+// to 30 bits go in three 10-bit passes instead of four 8-bit ones.
+constexpr size_t kMergeSortLimit = 64 * 1024;
 using Sort10 = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
     rocprim::radix_sort_onesweep_config<
         rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 10,
-        rocprim::block_radix_rank_algorithm::match>>;
+        rocprim::block_radix_rank_algorithm::match>,
+    kMergeSortLimit>;
+using Sort8 = rocprim::radix_sort_config<rocprim::default_config,
+                                         rocprim::default_config,
+                                         rocprim::default_config,
+                                         kMergeSortLimit>;
 
 hipError_t sort32(void *temp, size_t &bytes, const uint32_t *keys_in,
                   uint32_t *keys_out, const uint32_t *values_in,
@@ -1212,8 +1238,16 @@ hipError_t sort32(void *temp, size_t &bytes, const uint32_t *keys_in,
     return rocprim::radix_sort_pairs<Sort10>(temp, bytes, keys_in, keys_out,
                                              values_in, values_out, n, 0, bits,
                                              s);
-  return rocprim::radix_sort_pairs(temp, bytes, keys_in, keys_out, values_in,
-                                   values_out, n, 0, bits, s);
+  return rocprim::radix_sort_pairs<Sort8>(temp, bytes, keys_in, keys_out,
+                                          values_in, values_out, n, 0, bits,
+                                          s);
+}
+
+hipError_t sort64(void *temp, size_t &bytes, const uint64_t *keys_in,
+                  uint64_t *keys_out, const uint32_t *values_in,
+                  uint32_t *values_out, uint32_t n, hipStream_t s) {
+  return rocprim::radix_sort_pairs<Sort8>(temp, bytes, keys_in, keys_out,
+                                          values_in, values_out, n, 0, 64, s);
 }
 
 // Label one chunk whose frame lists are device-resident. `frames` holds
@@ -1259,13 +1293,9 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   l->bound_pixels += n - l->n_carry;
   l->bound_spots += (n - l->n_carry) / std::max<uint64_t>(1, a.min_size) + 1;
   {
-    unsigned long long totals[2] = {0, 0};
-    // on the labeller's own stream (idle here), so that no null-stream copy
-    // waits for other streams' work
-    LAB_CHECK(hipMemcpyAsync(totals, l->d_totals, sizeof(totals),
-                             hipMemcpyDeviceToHost, s),
-              "reading totals");
-    LAB_CHECK(hipStreamSynchronize(s), "reading totals");
+    // the totals as the last chunk's kernels left them in mapped memory
+    // (the stream is idle here)
+    const unsigned long long totals[2] = {l->h_totals[0], l->h_totals[1]};
     LAB_CHECK(l->spots.grow_keep(totals[0] + n / std::max<uint64_t>(
                                                   1, a.min_size) + 1,
                                  totals[0]),
@@ -1273,6 +1303,19 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
     LAB_CHECK(l->pixels.grow_keep(totals[1] + n + 1, totals[1]),
               "growing the pixel table");
   }
+  // the chunk table, through pinned staging (the stream is idle, so the
+  // last chunk's copy out of it is done)
+  if (n_frames > l->h_chunk_cap) {
+    if (l->h_chunk)
+      (void)hipHostFree(l->h_chunk);
+    l->h_chunk = nullptr;
+    l->h_chunk_cap = 0;
+    LAB_CHECK(hipHostMalloc(&l->h_chunk, n_frames * sizeof(sf::ChunkDesc)),
+              "chunk staging");
+    l->h_chunk_cap = n_frames;
+  }
+  if (n_frames)
+    std::memcpy(l->h_chunk, frames.data(), n_frames * sizeof(sf::ChunkDesc));
   l->last_count = 0;
   if (n_frames && !closing && !l->options.two_d)
     l->last_count = counts[n_frames - 1] <= frames[n_frames - 1].capacity
@@ -1282,14 +1325,13 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
     *l->h_scalars = 0;
     if (n_frames) {
       LAB_CHECK(l->chunk.ensure(n_frames), "chunk table");
-      LAB_CHECK(hipMemcpyAsync(l->chunk.ptr, frames.data(),
+      LAB_CHECK(hipMemcpyAsync(l->chunk.ptr, l->h_chunk,
                                n_frames * sizeof(sf::ChunkDesc),
                                hipMemcpyHostToDevice, s),
                 "chunk table");
-      sf::add_signal_count<<<1, 1, 0, s>>>(l->chunk.ptr, n_frames,
-                                           l->d_counts);
+      sf::add_signal_count<<<1, 64, 0, s>>>(l->chunk.ptr, n_frames,
+                                            l->d_counts);
     }
-    LAB_CHECK(hipMemsetAsync(l->d_n_carry, 0, sizeof(uint32_t), s), "memset");
     return SF_OK;
   }
   LAB_CHECK(l->node_frame.ensure(n), "alloc");
@@ -1345,7 +1387,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(hipMemsetAsync(l->d_scalars, 0, 8 * sizeof(uint32_t), s),
             "memset");
   if (n_frames)
-    LAB_CHECK(hipMemcpyAsync(l->chunk.ptr, frames.data(),
+    LAB_CHECK(hipMemcpyAsync(l->chunk.ptr, l->h_chunk,
                              n_frames * sizeof(sf::ChunkDesc),
                              hipMemcpyHostToDevice, s),
               "chunk table");
@@ -1364,8 +1406,8 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
       l->node_value.ptr, l->node_slot.ptr, l->head_bits.ptr,
       l->head_count.ptr);
   if (n_frames)
-    sf::add_signal_count<<<1, 1, 0, gs>>>(l->chunk.ptr, n_frames,
-                                          l->d_counts);
+    sf::add_signal_count<<<1, 64, 0, gs>>>(l->chunk.ptr, n_frames,
+                                           l->d_counts);
   LAB_CHECK(hipGetLastError(), "gather");
   if (producer) {
     LAB_CHECK(hipEventRecord(l->done, gs), "event");
@@ -1375,7 +1417,10 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
       a, l->head_bits.ptr, l->head_count.ptr, l->node_index.ptr,
       l->node_slot.ptr, l->carry_root[in].ptr, l->run_of.ptr,
       l->run_head.ptr, l->run_idx.ptr, l->run_last.ptr, l->run_slot.ptr,
-      l->parent.ptr, l->last_slot.ptr, n_runs);
+      l->parent.ptr, l->last_slot.ptr, n_runs, l->hd_scalars + 1);
+  // the host needs the run count for the sort; the kernels up to the keys
+  // are queued before it waits
+  LAB_CHECK(hipEventRecord(l->runs_ready, s), "event");
   {
     const size_t rows = (size_t)(n_frames + 1) * (l->height + 1);
     LAB_CHECK(l->run_rows.ensure(rows), "alloc");
@@ -1402,10 +1447,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
         l->keys.ptr, l->values.ptr, l->open_len.ptr, 32);
   LAB_CHECK(hipGetLastError(), "label kernels");
   // the sort and the scans take their sizes from the host
-  LAB_CHECK(hipMemcpyAsync(l->h_scalars + 1, n_runs, sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, s),
-            "reading the run count");
-  LAB_CHECK(hipStreamSynchronize(s), "reading the run count");
+  LAB_CHECK(hipEventSynchronize(l->runs_ready), "waiting for the run count");
   const uint32_t runs = l->h_scalars[1];
   // temp storage: the largest of the rocPRIM calls
   size_t need = 0, bytes = 0;
@@ -1414,9 +1456,8 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
                      l->order.ptr, runs, root_bits + slot_bits, s),
               "sort size");
   else
-    LAB_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, l->keys.ptr,
-                                        l->keys_sorted.ptr, l->values.ptr,
-                                        l->order.ptr, runs, 0, 64, s),
+    LAB_CHECK(sort64(nullptr, bytes, l->keys.ptr, l->keys_sorted.ptr,
+                     l->values.ptr, l->order.ptr, runs, s),
               "sort size");
   need = std::max(need, bytes);
   LAB_CHECK(rocprim::exclusive_scan(nullptr, bytes, l->open_len.ptr,
@@ -1443,9 +1484,8 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
         a, keys32_sorted, ~0u, l->order.ptr, l->run_head.ptr, runs,
         l->run_pack.ptr, l->pos_of.ptr, d_closed_runs);
   } else {
-    LAB_CHECK(rocprim::radix_sort_pairs(l->temp.ptr, bytes, l->keys.ptr,
-                                        l->keys_sorted.ptr, l->values.ptr,
-                                        l->order.ptr, runs, 0, 64, s),
+    LAB_CHECK(sort64(l->temp.ptr, bytes, l->keys.ptr, l->keys_sorted.ptr,
+                     l->values.ptr, l->order.ptr, runs, s),
               "sorting components");
     sf::mark_runs<uint64_t><<<rgrid, 256, 0, s>>>(
         a, l->keys_sorted.ptr, ~0ull, l->order.ptr, l->run_head.ptr, runs,
@@ -1494,7 +1534,8 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
       l->spot_rank.ptr, l->node_frame.ptr, l->node_index.ptr,
       l->node_value.ptr, l->d_totals, l->pixels.ptr);
   sf::advance_totals<<<1, 1, 0, s>>>(d_comps, l->spot_rank.ptr,
-                                     l->results.ptr, l->d_totals);
+                                     l->results.ptr, l->d_totals,
+                                     l->hd_totals);
   bytes = l->temp.cap;
   LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->open_len.ptr,
                                     l->open_at.ptr, 0u, (size_t)runs,
@@ -1505,11 +1546,9 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
       l->run_head.ptr, n_runs, l->node_frame.ptr, l->node_index.ptr,
       l->node_value.ptr,
       l->carry_frame[outside].ptr, l->carry_index[outside].ptr,
-      l->carry_value[outside].ptr, l->carry_root[outside].ptr, l->d_n_carry);
+      l->carry_value[outside].ptr, l->carry_root[outside].ptr,
+      l->hd_scalars);
   LAB_CHECK(hipGetLastError(), "finalise kernels");
-  LAB_CHECK(hipMemcpyAsync(l->h_scalars, l->d_n_carry, sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, s),
-            "reading the frontier size");
   l->carry_side = outside;
   return SF_OK;
 }
@@ -1649,20 +1688,28 @@ sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
       hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) == hipSuccess;
   ok = ok && hipEventCreateWithFlags(&l->done, hipEventDisableTiming) ==
                  hipSuccess;
-  ok = ok && hipMalloc(&l->d_n_carry, sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&l->runs_ready, hipEventDisableTiming) ==
+                 hipSuccess;
   ok = ok && hipMalloc(&l->d_scalars, 8 * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipMalloc(&l->d_counts, 6 * sizeof(unsigned long long)) ==
                  hipSuccess;
   ok = ok && hipMalloc(&l->d_totals, 2 * sizeof(unsigned long long)) ==
                  hipSuccess;
-  ok = ok && hipHostMalloc(&l->h_scalars, 4 * sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipHostMalloc(&l->h_scalars, 4 * sizeof(uint32_t),
+                           hipHostMallocMapped) == hipSuccess;
+  ok = ok && hipHostGetDevicePointer(reinterpret_cast<void **>(&l->hd_scalars),
+                                     l->h_scalars, 0) == hipSuccess;
+  ok = ok && hipHostMalloc(&l->h_totals, 2 * sizeof(unsigned long long),
+                           hipHostMallocMapped) == hipSuccess;
+  ok = ok && hipHostGetDevicePointer(reinterpret_cast<void **>(&l->hd_totals),
+                                     l->h_totals, 0) == hipSuccess;
   if (ok) {
     ok = hipMemset(l->d_counts, 0, 6 * sizeof(unsigned long long)) ==
              hipSuccess &&
          hipMemset(l->d_totals, 0, 2 * sizeof(unsigned long long)) ==
-             hipSuccess &&
-         hipMemset(l->d_n_carry, 0, sizeof(uint32_t)) == hipSuccess;
-    l->h_scalars[0] = 0;
+             hipSuccess;
+    std::memset(l->h_scalars, 0, 4 * sizeof(uint32_t));
+    std::memset(l->h_totals, 0, 2 * sizeof(unsigned long long));
   }
   if (!ok) {
     g_label_error = "labeller: device allocation failed";
@@ -1718,8 +1765,12 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   l->pixels.release();
   l->pending_dev.release();
   l->pending_count_dev.release();
-  if (l->d_n_carry)
-    (void)hipFree(l->d_n_carry);
+  if (l->runs_ready)
+    (void)hipEventDestroy(l->runs_ready);
+  if (l->h_totals)
+    (void)hipHostFree(l->h_totals);
+  if (l->h_chunk)
+    (void)hipHostFree(l->h_chunk);
   if (l->d_scalars)
     (void)hipFree(l->d_scalars);
   if (l->d_counts)
