@@ -754,12 +754,16 @@ __global__ __launch_bounds__(1024) void rowscan_kernel(const FrameDesc *frames) 
     fd.rowsurv[n] = carry;
 }
 
-constexpr int kTagWords = 16; // act words a cell row's pass holds at once
+constexpr int kTagWords = 16; // act words a cell row's pass reads at once
 
 // Places in index order for one cell row's survivors: pixel row 5i+t before
-// 5i+t+1, within a pixel row by cell, then by column. Each of the row's
-// staging entries gets its place; no entry is read.
-__device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i) {
+// 5i+t+1, within a pixel row by cell, then by column. The row's active
+// cells, a batch of kTagWords act words at a time, are staged in LDS in
+// column order, one lane per cell; then one prefix over the staged cells
+// per pixel row gives each cell its places. Each of the row's staging
+// entries gets its place; no entry is read.
+__device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i,
+                                        uint32_t *cell_s, uint32_t *cell_from) {
   const uint32_t lane = lane_id();
   const uint32_t *first = fd.rowsurv + 5 * i; // places of the five rows
   if (first[5] == first[0])
@@ -780,21 +784,34 @@ __device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i) {
       const uint32_t ci = i * fd.n_cell_cols + 64 * (w0 + u) + lane;
       const bool on = (bits[u] >> lane) & 1u;
       s[u] = on ? fd.scell[ci] : 0u;
-      from[u] = on ? fd.sbase[ci] : 0u; // meaningful where s != 0
+      from[u] = on ? fd.sbase[ci] : 0u;
     }
+    // stage the active cells with survivors, in column order
+    uint32_t staged = 0;
 #pragma unroll
     for (int u = 0; u < kTagWords; u++) {
-      if (!bits[u])
-        continue;
+      const uint64_t has = ballot(s[u] != 0);
+      if (s[u]) {
+        const uint32_t at = staged + count_below(has);
+        cell_s[at] = s[u];
+        cell_from[at] = from[u];
+      }
+      staged += (uint32_t)__builtin_popcountll(has);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t c0 = 0; c0 < staged; c0 += 64) {
+      const uint32_t c = c0 + lane;
+      const uint32_t cs = c < staged ? cell_s[c] : 0u;
+      const uint32_t cf = c < staged ? cell_from[c] : 0u;
 #pragma unroll
       for (int t = 0; t < 5; t++) {
-        const uint32_t k = (uint32_t)__builtin_popcount((s[u] >> (5 * t)) & 31u);
-        if (!ballot(k != 0))
-          continue;
+        const uint32_t k = (uint32_t)__builtin_popcount((cs >> (5 * t)) & 31u);
         const uint32_t incl = wave_prefix(k);
         const uint32_t dst = start[t] + incl - k;
         const uint32_t src =
-            from[u] + (uint32_t)__builtin_popcount(s[u] & ((1u << (5 * t)) - 1u));
+            cf + (uint32_t)__builtin_popcount(cs & ((1u << (5 * t)) - 1u));
 #pragma unroll
         for (int q = 0; q < 5; q++)
           if ((uint32_t)q < k && src + q < fd.capacity)
@@ -802,18 +819,24 @@ __device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i) {
         start[t] += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
 // Grid (cell-row quads, frames): a wave a cell row. Places are given only
 // when the list fits (the capacity contract of src/dext_gpu.cc).
 __global__ __launch_bounds__(256) void tag_kernel(const FrameDesc *frames) {
+  // per wave: the staged cells of one batch of act words
+  __shared__ uint32_t cells[4][2][64 * kTagWords];
   const FrameDesc fd = frames[blockIdx.y];
+  const uint32_t wave = threadIdx.x >> 6;
   const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane(
-      (int)(4 * blockIdx.x + (threadIdx.x >> 6)));
+      (int)(4 * blockIdx.x + wave));
   if (i >= fd.n_cell_rows || *fd.counter > fd.capacity)
     return;
-  tag_row(fd, i);
+  tag_row(fd, i, cells[wave][0], cells[wave][1]);
 }
 
 // Every staging entry to its place; the count goes out whether or not the
