@@ -432,6 +432,20 @@ def main() -> int:
         order_ms.append(o)
         counts.append(plan.counts())
     plan.set_timing(False)
+    # The labeller alone, by the host clock: a chunk thresholded and waited
+    # for, then labelled and waited for, so that nothing shares the device.
+    # Frame numbers carry on from the timed loop's.
+    label_ms = []
+    if labeller is not None:
+        first = (args.warmup + args.steps) * chunk
+        for s in range(4):
+            c = s % n_chunks
+            plan.run(ptr[c * chunk:(c + 1) * chunk])
+            plan.wait()
+            t_a = time.perf_counter()
+            labeller.add_plan(plan, first_frame=first + s * chunk)
+            labeller.wait()
+            label_ms.append((time.perf_counter() - t_a) * 1e3)
     kernel_s = float(np.median(scout_ms)) / 1e3
     threshold_s = float(np.median(np.add(scout_ms, rest_ms))) / 1e3
     mean_signal = float(np.mean(np.concatenate(counts)))
@@ -537,6 +551,12 @@ def main() -> int:
             "ms_per_launch": round(threshold_s * 1e3, 4),
             "order_ms_per_launch": round(float(np.median(order_ms)), 4),
             "algorithmic_bytes_per_launch": int(threshold_bytes),
+        },
+        "labeller_alone": None if not label_ms else {
+            "ms_per_chunk": round(float(np.median(label_ms)), 4),
+            "chunk_frames": chunk,
+            "frames_per_s": round(chunk / float(np.median(label_ms)) * 1e3,
+                                  1),
         },
         "plain_form_fps": round(plain_fps, 2),
         "speedup_vs_plain": round(fps / plain_fps, 3),
