@@ -187,27 +187,36 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return readlane_f64(x, 63);
 }
 
-// Nodes per tile of gather_nodes and number_runs.
+// Nodes per tile of gather_nodes and number_runs, threads per tile (two
+// nodes a thread), and each wave's share of a tile.
 constexpr uint32_t kTile = 2048;
+constexpr uint32_t kTileThreads = 1024;
+constexpr uint32_t kTileWaves = kTileThreads / 64;
+constexpr uint32_t kWaveNodes = kTile / kTileWaves;
 
 // Nodes from the frontier and the chunk's lists, in (frame, index) order,
 // and which of them begin a run: a horizontal run is consecutive indices on
 // one row of one frame, and every carried node is a run of its own. Bit
 // (g & 63) of head_bits[g >> 6] marks node g a run head; tile t (nodes
 // [t, t + 1) * kTile) counts its heads in head_count[t].
-__global__ __launch_bounds__(256) void gather_nodes(
+__global__ __launch_bounds__(kTileThreads) void gather_nodes(
     LabelArgs a, const ChunkDesc *frames, const int32_t *carry_frame,
     const uint32_t *carry_index, const uint32_t *carry_value,
     int32_t *node_frame, uint32_t *node_index, uint32_t *node_value,
     uint32_t *node_slot, uint64_t *head_bits, uint32_t *head_count) {
   __shared__ uint32_t listed;
+  __shared__ int tile_frame; // the chunk frame of the tile's first chunk node
   const uint32_t lane = lane_id();
   const uint32_t tile = blockIdx.x * kTile;
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     listed = 0;
+    tile_frame = a.n_frames ? chunk_frame_of(frames, a.n_frames,
+                                             max(tile, a.n_carry))
+                            : 0;
+  }
   __syncthreads();
   uint32_t mine = 0;
-  int k = -1; // this thread's chunk frame, found once, then stepped
+  int k = tile_frame; // stepped forward to each node's frame
   for (uint32_t g = tile + threadIdx.x;
        g < min(tile + kTile, a.n_nodes); g += blockDim.x) {
     bool head = true;
@@ -217,8 +226,6 @@ __global__ __launch_bounds__(256) void gather_nodes(
       node_value[g] = carry_value[g];
       node_slot[g] = 0;
     } else {
-      if (k < 0)
-        k = chunk_frame_of(frames, a.n_frames, g);
       while (k + 1 < (int)a.n_frames && frames[k + 1].offset <= g)
         k++;
       const sf_signal_pixel *list = frames[k].list;
@@ -245,22 +252,22 @@ __global__ __launch_bounds__(256) void gather_nodes(
 
 // Runs numbered in node order: run r is the r-th head. Tile t's first run
 // number is the heads of tiles 0 .. t - 1, which each block adds up itself;
-// each wave takes 512 consecutive nodes of its tile. Writes run_of per
+// each wave takes kWaveNodes consecutive nodes of its tile. Writes run_of per
 // node; per run its head node, first and last pixel index, slot and initial
 // parent -- itself, or for a carried node its carried root: carried nodes
 // are all heads and come first, so their run numbers are their node
 // numbers -- and a zero last_slot. The block of the last tile writes the run
 // count, here and to the host's mapped copy.
-__global__ __launch_bounds__(256) void number_runs(
+__global__ __launch_bounds__(kTileThreads) void number_runs(
     LabelArgs a, const uint64_t *head_bits, const uint32_t *head_count,
     const uint32_t *node_index, const uint32_t *node_slot,
     const uint32_t *carry_root, uint32_t *run_of, uint32_t *run_head,
     uint32_t *run_idx, uint32_t *run_last, uint32_t *run_slot,
     uint32_t *parent, uint32_t *last_slot, uint32_t *n_runs,
     uint32_t *host_runs) {
-  // part[w]: wave w's share of the earlier tiles' heads; part[4 + w]: the
-  // heads among wave w's own 512 nodes
-  __shared__ uint32_t part[8];
+  // part[w]: wave w's share of the earlier tiles' heads; part[kTileWaves +
+  // w]: the heads among wave w's own nodes
+  __shared__ uint32_t part[2 * kTileWaves];
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t tile = blockIdx.x * kTile;
   uint32_t before = 0;
@@ -268,27 +275,31 @@ __global__ __launch_bounds__(256) void number_runs(
     before += head_count[i];
   before = wave_reduce(before, 0u,
                        [](uint32_t p, uint32_t q) { return p + q; });
-  const uint32_t first = tile + 512 * wave;
+  const uint32_t first = tile + kWaveNodes * wave;
   uint32_t own = 0;
-  if (lane < 8 && first + 64 * lane < a.n_nodes)
+  if (lane < kWaveNodes / 64 && first + 64 * lane < a.n_nodes)
     own = (uint32_t)__builtin_popcountll(head_bits[(first >> 6) + lane]);
   own = wave_reduce(own, 0u, [](uint32_t p, uint32_t q) { return p + q; });
   if (lane == 0) {
     part[wave] = before;
-    part[4 + wave] = own;
+    part[kTileWaves + wave] = own;
   }
   __syncthreads();
-  const uint32_t tiles_before = part[0] + part[1] + part[2] + part[3];
+  uint32_t tiles_before = 0, tile_runs = 0;
+  for (uint32_t w = 0; w < kTileWaves; w++) {
+    tiles_before += part[w];
+    tile_runs += part[kTileWaves + w];
+  }
   uint32_t run = tiles_before; // heads before this wave's nodes
   for (uint32_t w = 0; w < wave; w++)
-    run += part[4 + w];
+    run += part[kTileWaves + w];
   if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0) {
-    *n_runs = tiles_before + part[4] + part[5] + part[6] + part[7];
+    *n_runs = tiles_before + tile_runs;
     *host_runs = *n_runs; // host-mapped
     __threadfence_system();
   }
   const uint64_t upto = ~0ull >> (63 - lane);
-  for (uint32_t j = 0; j < 8; j++) {
+  for (uint32_t j = 0; j < kWaveNodes / 64; j++) {
     const uint32_t g = first + 64 * j + lane;
     if (first + 64 * j >= a.n_nodes)
       break;
@@ -1400,7 +1411,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
     LAB_CHECK(hipStreamWaitEvent(gs, l->done, 0), "event wait");
   }
   const uint32_t grid = grid_for(n);
-  sf::gather_nodes<<<tiles, 256, 0, gs>>>(
+  sf::gather_nodes<<<tiles, sf::kTileThreads, 0, gs>>>(
       a, l->chunk.ptr, l->carry_frame[in].ptr, l->carry_index[in].ptr,
       l->carry_value[in].ptr, l->node_frame.ptr, l->node_index.ptr,
       l->node_value.ptr, l->node_slot.ptr, l->head_bits.ptr,
@@ -1413,7 +1424,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
     LAB_CHECK(hipEventRecord(l->done, gs), "event");
     LAB_CHECK(hipStreamWaitEvent(s, l->done, 0), "event wait");
   }
-  sf::number_runs<<<tiles, 256, 0, s>>>(
+  sf::number_runs<<<tiles, sf::kTileThreads, 0, s>>>(
       a, l->head_bits.ptr, l->head_count.ptr, l->node_index.ptr,
       l->node_slot.ptr, l->carry_root[in].ptr, l->run_of.ptr,
       l->run_head.ptr, l->run_idx.ptr, l->run_last.ptr, l->run_slot.ptr,
