@@ -602,6 +602,30 @@ __device__ int finish_spot(const LabelArgs &a, sf_spot &spot, double sum,
   return kAccepted;
 }
 
+// Calls f(k, index, value, frame) for members k = 0 .. n - 1 in order; the
+// members are read eight at a time, ahead of the arithmetic that uses them.
+template <class F>
+__device__ __forceinline__ void for_members(uint32_t s0, uint32_t n,
+                                            const int32_t *mem_frame,
+                                            const uint32_t *mem_index,
+                                            const uint32_t *mem_value, F f) {
+  for (uint32_t k0 = 0; k0 < n; k0 += 8) {
+    uint32_t idx[8], val[8];
+    int32_t z[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const bool in = k0 + q < n;
+      idx[q] = in ? mem_index[s0 + k0 + q] : 0u;
+      val[q] = in ? mem_value[s0 + k0 + q] : 0u;
+      z[q] = in ? mem_frame[s0 + k0 + q] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (k0 + q < n)
+        f(k0 + q, idx[q], val[q], z[q]);
+  }
+}
+
 // One component by one thread: the reference's emit() (src/dials_spots.cc:
 // 221-360) operation for operation, every sum in member order.
 __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
@@ -614,64 +638,62 @@ __device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
   int32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
   uint32_t peak_value = 0;
   double peak[3] = {0.0, 0.0, 0.0};
-  for (uint32_t k = 0; k < n; k++) {
-    const uint32_t g = s0 + k;
-    const uint32_t idx = mem_index[g];
-    const uint32_t row = row_of(a, idx);
-    const int32_t x = (int32_t)(idx - row * a.width);
-    const int32_t y = (int32_t)row;
-    const int32_t z = mem_frame[g];
-    if (k == 0) {
-      x0 = x;
-      x1 = x + 1;
-      y0 = y;
-      y1 = y + 1;
-      z0 = z;
-      z1 = z + 1;
-    } else {
-      x0 = x < x0 ? x : x0;
-      x1 = x + 1 > x1 ? x + 1 : x1;
-      y0 = y < y0 ? y : y0;
-      y1 = y + 1 > y1 ? y + 1 : y1;
-      z0 = z < z0 ? z : z0;
-      z1 = z + 1 > z1 ? z + 1 : z1;
-    }
-    const double value = (double)mem_value[g];
-    const double position[3] = {(double)x + 0.5, (double)y + 0.5,
-                                (double)z + 0.5};
-    sum += value;
-    sum_sq += value * value;
-    for (int axis = 0; axis < 3; axis++)
-      weighted[axis] += value * position[axis];
-    if (mem_value[g] > peak_value) {
-      peak_value = mem_value[g];
-      peak[0] = position[0];
-      peak[1] = position[1];
-      peak[2] = position[2];
-    }
-  }
+  for_members(s0, n, mem_frame, mem_index, mem_value,
+              [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
+                const uint32_t row = row_of(a, idx);
+                const int32_t x = (int32_t)(idx - row * a.width);
+                const int32_t y = (int32_t)row;
+                if (k == 0) {
+                  x0 = x;
+                  x1 = x + 1;
+                  y0 = y;
+                  y1 = y + 1;
+                  z0 = z;
+                  z1 = z + 1;
+                } else {
+                  x0 = x < x0 ? x : x0;
+                  x1 = x + 1 > x1 ? x + 1 : x1;
+                  y0 = y < y0 ? y : y0;
+                  y1 = y + 1 > y1 ? y + 1 : y1;
+                  z0 = z < z0 ? z : z0;
+                  z1 = z + 1 > z1 ? z + 1 : z1;
+                }
+                const double value = (double)v;
+                const double position[3] = {(double)x + 0.5, (double)y + 0.5,
+                                            (double)z + 0.5};
+                sum += value;
+                sum_sq += value * value;
+                for (int axis = 0; axis < 3; axis++)
+                  weighted[axis] += value * position[axis];
+                if (v > peak_value) {
+                  peak_value = v;
+                  peak[0] = position[0];
+                  peak[1] = position[1];
+                  peak[2] = position[2];
+                }
+              });
   spot.bbox[0] = x0;
   spot.bbox[1] = x1;
   spot.bbox[2] = y0;
   spot.bbox[3] = y1;
   spot.bbox[4] = z0;
   spot.bbox[5] = z1;
-  return finish_spot(a, spot, sum, sum_sq, weighted, peak,
-                     [&](const double (&centre)[3], double (&delta_sq)[3]) {
-                       for (uint32_t k = 0; k < n; k++) {
-                         const uint32_t g = s0 + k;
-                         const uint32_t idx = mem_index[g];
-                         const uint32_t row = row_of(a, idx);
-                         const double value = (double)mem_value[g];
-                         const double position[3] = {
-                             (double)(idx - row * a.width) + 0.5,
-                             (double)row + 0.5, (double)mem_frame[g] + 0.5};
-                         for (int axis = 0; axis < 3; axis++) {
-                           const double delta = position[axis] - centre[axis];
-                           delta_sq[axis] += value * delta * delta;
-                         }
-                       }
-                     });
+  return finish_spot(
+      a, spot, sum, sum_sq, weighted, peak,
+      [&](const double (&centre)[3], double (&delta_sq)[3]) {
+        for_members(s0, n, mem_frame, mem_index, mem_value,
+                    [&](uint32_t, uint32_t idx, uint32_t v, int32_t z) {
+                      const uint32_t row = row_of(a, idx);
+                      const double value = (double)v;
+                      const double position[3] = {
+                          (double)(idx - row * a.width) + 0.5,
+                          (double)row + 0.5, (double)z + 0.5};
+                      for (int axis = 0; axis < 3; axis++) {
+                        const double delta = position[axis] - centre[axis];
+                        delta_sq[axis] += value * delta * delta;
+                      }
+                    });
+      });
 }
 
 // The closed members' frame, index and value in emission order, gathered
@@ -699,11 +721,12 @@ __global__ void gather_members(LabelArgs a, const uint32_t *d_closed_runs,
   }
 }
 
-// Adds a block's tallies of emitted kinds into `counts` (groups,
-// too_small, too_large, separated, accepted, at 1..5), one atomic per
-// counter and block.
-__device__ void add_tallies(const uint32_t (&tally)[4],
-                            unsigned long long *counts) {
+// A block's tallies of emitted kinds (kAccepted, kTooSmall, kTooLarge,
+// kSeparated), summed in LDS and stored in the block's own row of `rows`
+// (four counters a row); sum_tallies adds the rows up. (One atomic per
+// counter and block on the shared counters serialised over thousands of
+// blocks.)
+__device__ void store_tallies(const uint32_t (&tally)[4], uint32_t *rows) {
   __shared__ uint32_t block_tally[4];
   if (threadIdx.x < 4)
     block_tally[threadIdx.x] = 0;
@@ -712,11 +735,35 @@ __device__ void add_tallies(const uint32_t (&tally)[4],
     if (tally[t])
       atomicAdd(&block_tally[t], tally[t]);
   __syncthreads();
+  if (threadIdx.x < 4)
+    rows[4 * blockIdx.x + threadIdx.x] = block_tally[threadIdx.x];
+}
+
+// counts (groups, too_small, too_large, separated, accepted, at 1..5) +=
+// the centroid passes' tally rows and the chunk's components. One block.
+__global__ __launch_bounds__(256) void sum_tallies(const uint32_t *rows,
+                                                   uint32_t n_rows,
+                                                   const uint32_t *n_comps,
+                                                   unsigned long long *counts) {
+  __shared__ unsigned long long part[4][4];
+  unsigned long long t4[4] = {0, 0, 0, 0};
+  for (uint32_t r = threadIdx.x; r < n_rows; r += blockDim.x)
+    for (int t = 0; t < 4; t++)
+      t4[t] += rows[4 * r + t];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  for (int t = 0; t < 4; t++) {
+    // the row counts stay below 2^32 in a chunk
+    const uint32_t w = wave_reduce((uint32_t)t4[t], 0u,
+                                   [](uint32_t p, uint32_t q) { return p + q; });
+    if (lane == 0)
+      part[wave][t] = w;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     const int slot[4] = {5, 2, 3, 4}; // kAccepted, kTooSmall, ...
     for (int t = 0; t < 4; t++)
-      if (block_tally[t])
-        atomicAdd(&counts[slot[t]], (unsigned long long)block_tally[t]);
+      counts[slot[t]] += part[0][t] + part[1][t] + part[2][t] + part[3][t];
+    counts[1] += *n_comps;
   }
 }
 
@@ -730,7 +777,7 @@ __global__ __launch_bounds__(256) void finalise_narrow(
     LabelArgs a, const uint32_t *starts, const uint32_t *n_starts,
     const uint32_t *n_closed, const int32_t *mem_frame,
     const uint32_t *mem_index, const uint32_t *mem_value, SpotOut *out,
-    uint32_t *wide, uint32_t *n_wide, unsigned long long *counts) {
+    uint32_t *wide, uint32_t *n_wide, uint32_t *tallies) {
   const uint32_t comps = *n_starts;
   const uint32_t end_all = *n_closed;
   const uint32_t lane = lane_id();
@@ -770,9 +817,7 @@ __global__ __launch_bounds__(256) void finalise_narrow(
     out[c] = r;
     tally[kind]++;
   }
-  add_tallies(tally, counts);
-  if (blockIdx.x == 0 && threadIdx.x == 0)
-    atomicAdd(&counts[1], (unsigned long long)comps);
+  store_tallies(tally, tallies);
 }
 
 // Variance terms staged per round, and the stride of each axis's row (two
@@ -797,7 +842,7 @@ __global__ __launch_bounds__(256) void finalise_wide(
     const uint32_t *n_closed, const int32_t *mem_frame,
     const uint32_t *mem_index, const uint32_t *mem_value,
     const uint32_t *wide, const uint32_t *n_wide, SpotOut *out,
-    unsigned long long *counts) {
+    uint32_t *tallies) {
   __shared__ double terms[4][3 * kTermStride];
   const uint32_t count = *n_wide;
   const uint32_t comps = *n_starts;
@@ -965,7 +1010,7 @@ __global__ __launch_bounds__(256) void finalise_wide(
       tally[kind]++;
     }
   }
-  add_tallies(tally, counts);
+  store_tallies(tally, tallies);
 }
 
 // Per component: accepted << 32 | kept members, so that one exclusive scan
@@ -1186,6 +1231,7 @@ struct sf_labeller {
   sf::DevArray<unsigned long long> run_pack, run_rank;
   // per component
   sf::DevArray<uint32_t> starts, comp_mem;
+  sf::DevArray<uint32_t> tallies; // per centroid-pass block: kinds emitted
   sf::DevArray<sf::SpotOut> results;
   sf::DevArray<unsigned long long> spot_pack, spot_rank;
   sf::DevArray<uint8_t> temp;
@@ -1522,12 +1568,18 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   // listed in `order` (no longer needed once the members are gathered); each
   // pass loops over components on a bounded grid
   uint32_t *wide = l->order.ptr;
-  sf::finalise_narrow<<<std::min<uint32_t>(rgrid, 1024), 256, 0, s>>>(
+  const uint32_t narrow_grid = std::min<uint32_t>(rgrid, 1024);
+  const uint32_t wide_grid = std::min<uint32_t>(rgrid, 4096);
+  LAB_CHECK(l->tallies.ensure(4 * (narrow_grid + wide_grid)), "alloc");
+  sf::finalise_narrow<<<narrow_grid, 256, 0, s>>>(
       a, l->comp_mem.ptr, d_comps, d_closed_mem, mem_frame, mem_index,
-      mem_value, l->results.ptr, wide, n_wide, l->d_counts);
-  sf::finalise_wide<<<std::min<uint32_t>(rgrid, 1024), 256, 0, s>>>(
+      mem_value, l->results.ptr, wide, n_wide, l->tallies.ptr);
+  sf::finalise_wide<<<wide_grid, 256, 0, s>>>(
       a, l->comp_mem.ptr, d_comps, d_closed_mem, mem_frame, mem_index,
-      mem_value, wide, n_wide, l->results.ptr, l->d_counts);
+      mem_value, wide, n_wide, l->results.ptr,
+      l->tallies.ptr + 4 * narrow_grid);
+  sf::sum_tallies<<<1, 256, 0, s>>>(l->tallies.ptr, narrow_grid + wide_grid,
+                                    d_comps, l->d_counts);
   sf::spot_flags<<<rgrid, 256, 0, s>>>(l->results.ptr, d_comps,
                                        l->spot_pack.ptr);
   bytes = l->temp.cap;
@@ -1767,6 +1819,7 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   l->run_rank.release();
   l->starts.release();
   l->comp_mem.release();
+  l->tallies.release();
   l->results.release();
   l->spot_pack.release();
   l->spot_rank.release();

@@ -192,3 +192,38 @@ def test_device_lists_across_chunks(sf):
         assert same_tables(got, expected), two_d
         assert expected[2]["accepted"] > 20
         plan.close()
+
+
+def test_chunks_past_the_merge_sort_limit(sf):
+    """Chunks of more than 65,536 runs, where the component sort takes the
+    radix (onesweep) path instead of rocPRIM's merge sort: device lists of
+    24-frame plans against the oracle fed the same lists frame by frame."""
+    h, w, n, per = 2048, 2048, 48, 24
+    buf = sf.DeviceBuffer(2 * h * w * n)
+    spots = fr.diffraction_spots(9, h, w, n, 1500.0 * h * w / (4362 * 4148))
+    sf.synth_frames(buf, 2, h, w, n, 9, 0.3, True, 0.0005, spots)
+    plan = sf.Plan(2, [(h, w)] * per)
+    oracle = cpu.Labeller(h, w)
+    lab = sf.Labeller(h, w)
+    for c in range(n // per):
+        plan.run([buf.ptr + 2 * h * w * (per * c + k) for k in range(per)])
+        plan.wait()
+        counts = plan.counts()
+        runs = 0
+        for k in range(per):
+            lst = plan.output(k, int(counts[k]))
+            idx = lst["index"].astype(np.int64)
+            # a run starts where the index does not follow on the same row
+            runs += int(np.count_nonzero(
+                (np.diff(idx, prepend=-2) != 1) | (idx % w == 0)))
+            assert oracle.add(300 + per * c + k, lst) == 0
+        assert runs > 65536, runs
+        lab.add_plan(plan, 300 + per * c)
+    oracle.finish()
+    lab.finish()
+    expected, got = oracle.results(), lab.results()
+    assert got[2] == expected[2], (got[2], expected[2])
+    assert same_tables(got, expected)
+    assert expected[2]["accepted"] > 1000
+    plan.close()
+    buf.close()
