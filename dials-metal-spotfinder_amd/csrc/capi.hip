@@ -263,12 +263,17 @@ int upload_desc(sf_plan *p, const void *const *frames) {
   if (!changed)
     for (int f = 0; f < p->n; f++)
       changed |= p->bound[f] != frames[f];
+  // Row loads of several pixels: the sparse scout reads 16 bytes from an
+  // even column, so 4-byte aligned frames of even width suffice; the other
+  // forms read K pixels a lane and need 8-byte alignment and K | width.
+  const bool sparse = kernel_for(p, p->form) == kSparse;
   p->any_vec = p->any_scalar = false;
   for (int f = 0; f < p->n; f++) {
     sf::FrameDesc &d = p->desc[f];
     d.pixels = frames[f];
-    const bool aligned = (reinterpret_cast<uintptr_t>(frames[f]) % 8) == 0;
-    d.vec = (aligned && d.width % (uint32_t)k == 0) ? 1u : 0u;
+    const uintptr_t at = reinterpret_cast<uintptr_t>(frames[f]);
+    d.vec = sparse ? (at % 4 == 0 && d.width % 2 == 0 ? 1u : 0u)
+                   : (at % 8 == 0 && d.width % (uint32_t)k == 0 ? 1u : 0u);
     p->any_vec |= d.vec != 0;
     p->any_scalar |= d.vec == 0;
   }

@@ -289,9 +289,13 @@ hipError_t launch(Batch *b) {
     items += d.n_strips * d.n_bands;
     max_bands = std::max(max_bands, d.n_bands);
     max_cell_rows = std::max(max_cell_rows, d.n_cell_rows);
-    const bool aligned =
-        (reinterpret_cast<uintptr_t>(d.pixels) % 8) == 0;
-    d.vec = (aligned && d.width % k == 0) ? 1u : 0u;
+    // u16 frames go to the sparse scout (16-byte row loads from an even
+    // column: 4-byte aligned frames of even width), u32 frames to the sweep
+    // (K pixels a lane: 8-byte aligned, K | width)
+    const uintptr_t at = reinterpret_cast<uintptr_t>(d.pixels);
+    d.vec = b->pixel_bytes == 2
+                ? (at % 4 == 0 && d.width % 2 == 0 ? 1u : 0u)
+                : (at % 8 == 0 && d.width % k == 0 ? 1u : 0u);
     any_vec |= d.vec != 0;
     any_scalar |= d.vec == 0;
     b->h_desc[f] = d;
