@@ -754,14 +754,14 @@ __global__ __launch_bounds__(1024) void rowscan_kernel(const FrameDesc *frames) 
     fd.rowsurv[n] = carry;
 }
 
-constexpr int kTagWords = 16; // act words a cell row's pass reads at once
+constexpr int kTagWords = 8; // act words a cell row's pass reads at once
 
 // Places in index order for one cell row's survivors: pixel row 5i+t before
-// 5i+t+1, within a pixel row by cell, then by column. The row's active
-// cells, a batch of kTagWords act words at a time, are staged in LDS in
-// column order, one lane per cell; then one prefix over the staged cells
-// per pixel row gives each cell its places. Each of the row's staging
-// entries gets its place; no entry is read.
+// 5i+t+1, within a pixel row by cell, then by column. The row's cells that
+// hold survivors are staged in LDS in column order, up to 64 at a time (one
+// per lane), from kTagWords act words at a time; for each 64 staged cells,
+// one prefix over them per pixel row gives each cell its places. Each of
+// the row's staging entries gets its place; no entry is read.
 __device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i,
                                         uint32_t *cell_s, uint32_t *cell_from) {
   const uint32_t lane = lane_id();
@@ -772,6 +772,32 @@ __device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i,
 #pragma unroll
   for (int t = 0; t < 5; t++)
     start[t] = first[t];
+  // the staged cells' places, and then the stage is empty again
+  uint32_t staged = 0;
+  auto place = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t cs = lane < staged ? cell_s[lane] : 0u;
+    const uint32_t cf = lane < staged ? cell_from[lane] : 0u;
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      const uint32_t k = (uint32_t)__builtin_popcount((cs >> (5 * t)) & 31u);
+      const uint32_t incl = wave_prefix(k);
+      const uint32_t dst = start[t] + incl - k;
+      const uint32_t src =
+          cf + (uint32_t)__builtin_popcount(cs & ((1u << (5 * t)) - 1u));
+#pragma unroll
+      for (int q = 0; q < 5; q++)
+        if ((uint32_t)q < k && src + q < fd.capacity)
+          fd.dstof[src + q] = dst + q;
+      start[t] += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    staged = 0;
+  };
   const uint64_t *act = fd.act + (size_t)i * fd.n_strips;
   for (uint32_t w0 = 0; w0 < fd.n_strips; w0 += kTagWords) {
     uint64_t bits[kTagWords];
@@ -786,50 +812,29 @@ __device__ __forceinline__ void tag_row(const FrameDesc &fd, uint32_t i,
       s[u] = on ? fd.scell[ci] : 0u;
       from[u] = on ? fd.sbase[ci] : 0u;
     }
-    // stage the active cells with survivors, in column order
-    uint32_t staged = 0;
 #pragma unroll
     for (int u = 0; u < kTagWords; u++) {
       const uint64_t has = ballot(s[u] != 0);
+      const uint32_t nh = (uint32_t)__builtin_popcountll(has);
+      if (staged + nh > 64)
+        place();
       if (s[u]) {
         const uint32_t at = staged + count_below(has);
         cell_s[at] = s[u];
         cell_from[at] = from[u];
       }
-      staged += (uint32_t)__builtin_popcountll(has);
+      staged += nh;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t c0 = 0; c0 < staged; c0 += 64) {
-      const uint32_t c = c0 + lane;
-      const uint32_t cs = c < staged ? cell_s[c] : 0u;
-      const uint32_t cf = c < staged ? cell_from[c] : 0u;
-#pragma unroll
-      for (int t = 0; t < 5; t++) {
-        const uint32_t k = (uint32_t)__builtin_popcount((cs >> (5 * t)) & 31u);
-        const uint32_t incl = wave_prefix(k);
-        const uint32_t dst = start[t] + incl - k;
-        const uint32_t src =
-            cf + (uint32_t)__builtin_popcount(cs & ((1u << (5 * t)) - 1u));
-#pragma unroll
-        for (int q = 0; q < 5; q++)
-          if ((uint32_t)q < k && src + q < fd.capacity)
-            fd.dstof[src + q] = dst + q;
-        start[t] += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  if (staged)
+    place();
 }
 
 // Grid (cell-row quads, frames): a wave a cell row. Places are given only
 // when the list fits (the capacity contract of src/dext_gpu.cc).
 __global__ __launch_bounds__(256) void tag_kernel(const FrameDesc *frames) {
-  // per wave: the staged cells of one batch of act words
-  __shared__ uint32_t cells[4][2][64 * kTagWords];
+  // per wave: the staged cells, one per lane
+  __shared__ uint32_t cells[4][2][64];
   const FrameDesc fd = frames[blockIdx.y];
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane(
