@@ -194,8 +194,9 @@ constexpr uint32_t kTileThreads = 1024;
 constexpr uint32_t kTileWaves = kTileThreads / 64;
 constexpr uint32_t kWaveNodes = kTile / kTileWaves;
 
-// Nodes from the frontier and the chunk's lists, in (frame, index) order,
-// and which of them begin a run: a horizontal run is consecutive indices on
+// Nodes from the frontier and the chunk's lists, in (frame, index) order
+// (and the chunk's signal-pixel count into counts[0]), and which of them
+// begin a run: a horizontal run is consecutive indices on
 // one row of one frame, and every carried node is a run of its own. Bit
 // (g & 63) of head_bits[g >> 6] marks node g a run head; tile t (nodes
 // [t, t + 1) * kTile) counts its heads in head_count[t].
@@ -203,8 +204,21 @@ __global__ __launch_bounds__(kTileThreads) void gather_nodes(
     LabelArgs a, const ChunkDesc *frames, const int32_t *carry_frame,
     const uint32_t *carry_index, const uint32_t *carry_value,
     int32_t *node_frame, uint32_t *node_index, uint32_t *node_value,
-    uint32_t *node_slot, uint64_t *head_bits, uint32_t *head_count) {
+    uint32_t *node_slot, uint64_t *head_bits, uint32_t *head_count,
+    unsigned long long *counts) {
   __shared__ uint32_t listed;
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    // the chunk's signal pixels (frames whose list fit), by one wave
+    uint32_t total = 0; // a chunk's lists hold fewer than 2^32 pixels
+    for (uint32_t k = threadIdx.x; k < a.n_frames; k += 64) {
+      const uint32_t c = *frames[k].count;
+      total += c <= frames[k].capacity ? c : 0;
+    }
+    total =
+        wave_reduce(total, 0u, [](uint32_t p, uint32_t q) { return p + q; });
+    if (threadIdx.x == 0)
+      counts[0] += total;
+  }
   __shared__ int tile_frame; // the chunk frame of the tile's first chunk node
   const uint32_t lane = lane_id();
   const uint32_t tile = blockIdx.x * kTile;
@@ -723,7 +737,7 @@ __global__ void gather_members(LabelArgs a, const uint32_t *d_closed_runs,
 
 // A block's tallies of emitted kinds (kAccepted, kTooSmall, kTooLarge,
 // kSeparated), summed in LDS and stored in the block's own row of `rows`
-// (four counters a row); sum_tallies adds the rows up. (One atomic per
+// (four counters a row); close_chunk adds the rows up. (One atomic per
 // counter and block on the shared counters serialised over thousands of
 // blocks.)
 __device__ void store_tallies(const uint32_t (&tally)[4], uint32_t *rows) {
@@ -739,34 +753,6 @@ __device__ void store_tallies(const uint32_t (&tally)[4], uint32_t *rows) {
     rows[4 * blockIdx.x + threadIdx.x] = block_tally[threadIdx.x];
 }
 
-// counts (groups, too_small, too_large, separated, accepted, at 1..5) +=
-// the centroid passes' tally rows and the chunk's components. One block.
-__global__ __launch_bounds__(256) void sum_tallies(const uint32_t *rows,
-                                                   uint32_t n_rows,
-                                                   const uint32_t *n_comps,
-                                                   unsigned long long *counts) {
-  __shared__ unsigned long long part[4][4];
-  unsigned long long t4[4] = {0, 0, 0, 0};
-  for (uint32_t r = threadIdx.x; r < n_rows; r += blockDim.x)
-    for (int t = 0; t < 4; t++)
-      t4[t] += rows[4 * r + t];
-  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-  for (int t = 0; t < 4; t++) {
-    // the row counts stay below 2^32 in a chunk
-    const uint32_t w = wave_reduce((uint32_t)t4[t], 0u,
-                                   [](uint32_t p, uint32_t q) { return p + q; });
-    if (lane == 0)
-      part[wave][t] = w;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int slot[4] = {5, 2, 3, 4}; // kAccepted, kTooSmall, ...
-    for (int t = 0; t < 4; t++)
-      counts[slot[t]] += part[0][t] + part[1][t] + part[2][t] + part[3][t];
-    counts[1] += *n_comps;
-  }
-}
-
 // Components of up to this many members are emitted one to a lane.
 constexpr uint32_t kLaneMembers = 32;
 
@@ -777,7 +763,8 @@ __global__ __launch_bounds__(256) void finalise_narrow(
     LabelArgs a, const uint32_t *starts, const uint32_t *n_starts,
     const uint32_t *n_closed, const int32_t *mem_frame,
     const uint32_t *mem_index, const uint32_t *mem_value, SpotOut *out,
-    uint32_t *wide, uint32_t *n_wide, uint32_t *tallies) {
+    unsigned long long *spot_pack, uint32_t *wide, uint32_t *n_wide,
+    uint32_t *tallies) {
   const uint32_t comps = *n_starts;
   const uint32_t end_all = *n_closed;
   const uint32_t lane = lane_id();
@@ -815,6 +802,7 @@ __global__ __launch_bounds__(256) void finalise_narrow(
       r.members = n;
     }
     out[c] = r;
+    spot_pack[c] = ((unsigned long long)r.accepted << 32) | r.members;
     tally[kind]++;
   }
   store_tallies(tally, tallies);
@@ -842,7 +830,7 @@ __global__ __launch_bounds__(256) void finalise_wide(
     const uint32_t *n_closed, const int32_t *mem_frame,
     const uint32_t *mem_index, const uint32_t *mem_value,
     const uint32_t *wide, const uint32_t *n_wide, SpotOut *out,
-    uint32_t *tallies) {
+    unsigned long long *spot_pack, uint32_t *tallies) {
   __shared__ double terms[4][3 * kTermStride];
   const uint32_t count = *n_wide;
   const uint32_t comps = *n_starts;
@@ -1007,44 +995,18 @@ __global__ __launch_bounds__(256) void finalise_wide(
     }
     if (lane == 0) {
       out[c] = r;
+      spot_pack[c] = ((unsigned long long)r.accepted << 32) | r.members;
       tally[kind]++;
     }
   }
   store_tallies(tally, tallies);
 }
 
-// Per component: accepted << 32 | kept members, so that one exclusive scan
-// gives both the spot rank (high half) and the pixel rank (low half; the
-// members of a chunk stay below 2^32).
-__global__ void spot_flags(const SpotOut *res, const uint32_t *n_comps,
-                           unsigned long long *packed) {
-  const uint32_t comps = *n_comps;
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
-       c += gridDim.x * blockDim.x)
-    packed[c] = ((unsigned long long)res[c].accepted << 32) | res[c].members;
-}
-
-// Appends the kept spots after what earlier chunks wrote.
-__global__ void write_spots(const SpotOut *res, const uint32_t *n_comps,
-                            const unsigned long long *rank,
-                            const unsigned long long *totals,
-                            sf_spot *spots) {
-  const uint32_t comps = *n_comps;
-  const unsigned long long spot_base = totals[0], pixel_base = totals[1];
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
-       c += gridDim.x * blockDim.x) {
-    if (!res[c].accepted)
-      continue;
-    sf_spot s = res[c].spot;
-    s.first = pixel_base + (uint32_t)rank[c];
-    spots[spot_base + (rank[c] >> 32)] = s;
-  }
-}
-
-// Appends the kept spots' pixels, one thread per node: node g of run r, at
+// Appends the kept spots and their pixels (a pixel a thread): node g of run r, at
 // sorted place i, belongs to component c = (i's scan rank, high half) + i's
 // start bit - 1, and is member (i's first member) + (g - r's head).
 __global__ void write_pixels(LabelArgs a, const SpotOut *res,
+                             const uint32_t *n_comps,
                              const uint32_t *d_closed_runs,
                              const uint32_t *run_of, const uint32_t *pos_of,
                              const uint32_t *run_head,
@@ -1056,9 +1018,19 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
                              const uint32_t *node_index,
                              const uint32_t *node_value,
                              const unsigned long long *totals,
-                             sf_pixel *pixels) {
+                             sf_spot *spots, sf_pixel *pixels) {
   const uint32_t closed = *d_closed_runs;
-  const unsigned long long pixel_base = totals[1];
+  const unsigned long long spot_base = totals[0], pixel_base = totals[1];
+  // the kept spots' rows, after what earlier chunks wrote
+  const uint32_t comps = *n_comps;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
+       c += gridDim.x * blockDim.x) {
+    if (!res[c].accepted)
+      continue;
+    sf_spot sp = res[c].spot;
+    sp.first = pixel_base + (uint32_t)spot_rank[c];
+    spots[spot_base + (spot_rank[c] >> 32)] = sp;
+  }
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
        g += gridDim.x * blockDim.x) {
     const uint32_t r = run_of[g];
@@ -1078,20 +1050,43 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
   }
 }
 
-// The spots and pixels written so far, also into the host's mapped copy.
-__global__ void advance_totals(const uint32_t *n_comps,
-                               const unsigned long long *rank,
-                               const SpotOut *res,
-                               unsigned long long *totals,
-                               unsigned long long *host_totals) {
-  const uint32_t comps = *n_comps;
-  if (comps) {
-    const uint32_t last = comps - 1;
-    totals[0] += (rank[last] >> 32) + res[last].accepted;
-    totals[1] += (uint32_t)rank[last] + res[last].members;
-    host_totals[0] = totals[0];
-    host_totals[1] = totals[1];
-    __threadfence_system();
+// The chunk's close, one block: counts (groups, too_small, too_large,
+// separated, accepted, at 1..5) += the centroid passes' tally rows and the
+// chunk's components; the spots and pixels written so far advance, also in
+// the host's mapped copy.
+__global__ __launch_bounds__(256) void close_chunk(
+    const uint32_t *rows, uint32_t n_rows, const uint32_t *n_comps,
+    const unsigned long long *rank, const SpotOut *res,
+    unsigned long long *counts, unsigned long long *totals,
+    unsigned long long *host_totals) {
+  __shared__ unsigned long long part[4][4];
+  unsigned long long t4[4] = {0, 0, 0, 0};
+  for (uint32_t r = threadIdx.x; r < n_rows; r += blockDim.x)
+    for (int t = 0; t < 4; t++)
+      t4[t] += rows[4 * r + t];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  for (int t = 0; t < 4; t++) {
+    // the row counts stay below 2^32 in a chunk
+    const uint32_t w = wave_reduce((uint32_t)t4[t], 0u,
+                                   [](uint32_t p, uint32_t q) { return p + q; });
+    if (lane == 0)
+      part[wave][t] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int slot[4] = {5, 2, 3, 4}; // kAccepted, kTooSmall, ...
+    for (int t = 0; t < 4; t++)
+      counts[slot[t]] += part[0][t] + part[1][t] + part[2][t] + part[3][t];
+    const uint32_t comps = *n_comps;
+    counts[1] += comps;
+    if (comps) {
+      const uint32_t last = comps - 1;
+      totals[0] += (rank[last] >> 32) + res[last].accepted;
+      totals[1] += (uint32_t)rank[last] + res[last].members;
+      host_totals[0] = totals[0];
+      host_totals[1] = totals[1];
+      __threadfence_system();
+    }
   }
 }
 
@@ -1461,10 +1456,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
       a, l->chunk.ptr, l->carry_frame[in].ptr, l->carry_index[in].ptr,
       l->carry_value[in].ptr, l->node_frame.ptr, l->node_index.ptr,
       l->node_value.ptr, l->node_slot.ptr, l->head_bits.ptr,
-      l->head_count.ptr);
-  if (n_frames)
-    sf::add_signal_count<<<1, 64, 0, gs>>>(l->chunk.ptr, n_frames,
-                                           l->d_counts);
+      l->head_count.ptr, l->d_counts);
   LAB_CHECK(hipGetLastError(), "gather");
   if (producer) {
     LAB_CHECK(hipEventRecord(l->done, gs), "event");
@@ -1573,32 +1565,27 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(l->tallies.ensure(4 * (narrow_grid + wide_grid)), "alloc");
   sf::finalise_narrow<<<narrow_grid, 256, 0, s>>>(
       a, l->comp_mem.ptr, d_comps, d_closed_mem, mem_frame, mem_index,
-      mem_value, l->results.ptr, wide, n_wide, l->tallies.ptr);
+      mem_value, l->results.ptr, l->spot_pack.ptr, wide, n_wide,
+      l->tallies.ptr);
   sf::finalise_wide<<<wide_grid, 256, 0, s>>>(
       a, l->comp_mem.ptr, d_comps, d_closed_mem, mem_frame, mem_index,
-      mem_value, wide, n_wide, l->results.ptr,
+      mem_value, wide, n_wide, l->results.ptr, l->spot_pack.ptr,
       l->tallies.ptr + 4 * narrow_grid);
-  sf::sum_tallies<<<1, 256, 0, s>>>(l->tallies.ptr, narrow_grid + wide_grid,
-                                    d_comps, l->d_counts);
-  sf::spot_flags<<<rgrid, 256, 0, s>>>(l->results.ptr, d_comps,
-                                       l->spot_pack.ptr);
   bytes = l->temp.cap;
   // components are at most the runs
   LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->spot_pack.ptr,
                                     l->spot_rank.ptr, 0ull, (size_t)runs,
                                     rocprim::plus<unsigned long long>(), s),
             "ranking spots and their pixels");
-  sf::write_spots<<<rgrid, 256, 0, s>>>(l->results.ptr, d_comps,
-                                        l->spot_rank.ptr, l->d_totals,
-                                        l->spots.ptr);
   sf::write_pixels<<<grid, 256, 0, s>>>(
-      a, l->results.ptr, d_closed_runs, l->run_of.ptr, l->pos_of.ptr,
-      l->run_head.ptr, l->run_pack.ptr, l->run_rank.ptr, l->comp_mem.ptr,
-      l->spot_rank.ptr, l->node_frame.ptr, l->node_index.ptr,
-      l->node_value.ptr, l->d_totals, l->pixels.ptr);
-  sf::advance_totals<<<1, 1, 0, s>>>(d_comps, l->spot_rank.ptr,
-                                     l->results.ptr, l->d_totals,
-                                     l->hd_totals);
+      a, l->results.ptr, d_comps, d_closed_runs, l->run_of.ptr,
+      l->pos_of.ptr, l->run_head.ptr, l->run_pack.ptr, l->run_rank.ptr,
+      l->comp_mem.ptr, l->spot_rank.ptr, l->node_frame.ptr,
+      l->node_index.ptr, l->node_value.ptr, l->d_totals, l->spots.ptr,
+      l->pixels.ptr);
+  sf::close_chunk<<<1, 256, 0, s>>>(l->tallies.ptr, narrow_grid + wide_grid,
+                                    d_comps, l->spot_rank.ptr, l->results.ptr,
+                                    l->d_counts, l->d_totals, l->hd_totals);
   bytes = l->temp.cap;
   LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->open_len.ptr,
                                     l->open_at.ptr, 0u, (size_t)runs,
