@@ -26,6 +26,7 @@ SF_BAD_ARGUMENT = -1
 SF_OVERFLOW = -2
 SF_DEVICE_ERROR = -3
 SF_ORDER_ERROR = -4
+SF_IO_ERROR = -5
 
 DIRECT, TILE = 0, 1
 
@@ -60,6 +61,12 @@ class Options(ctypes.Structure):
                 ("max_spot_size", ctypes.c_uint64),
                 ("max_separation", ctypes.c_double),
                 ("two_d", ctypes.c_int32)]
+
+
+class ReflOptions(ctypes.Structure):
+    """include/refl_hip.h sf_refl_options (src/refl.hh:52-63)."""
+    _fields_ = [("panel", ctypes.c_uint64), ("id", ctypes.c_int32),
+                ("shoeboxes", ctypes.c_int32), ("identifier", ctypes.c_char_p)]
 
 
 class ServerStats(ctypes.Structure):
@@ -156,6 +163,9 @@ def _declare(lib: ctypes.CDLL) -> None:
        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
        ctypes.POINTER(_p), ctypes.POINTER(_i), ctypes.c_int64)
     fn("sf_hip_server_last_error", ctypes.c_char_p)
+    fn("sf_hip_refl_write", _i, ctypes.c_char_p, _p, _sz, _p, _sz, _sz,
+       ctypes.POINTER(ReflOptions))
+    fn("sf_hip_refl_last_error", ctypes.c_char_p)
 
 
 _lib: ctypes.CDLL | None = None
@@ -652,3 +662,26 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# --- src/refl.hh:21-78 --------------------------------------------------------
+
+def write_refl(path: str, spots: np.ndarray, pixels: np.ndarray, width: int,
+               panel: int = 0, id: int = 0, identifier: str = "",
+               shoeboxes: bool = True) -> None:
+    """refl::write: the Labeller's spot and pixel tables (as results() returns
+    them) to a DIALS reflection table. Raises ValueError on a bad table and
+    OSError when the file cannot be written."""
+    spots = np.ascontiguousarray(spots, dtype=SPOT)
+    pixels = np.ascontiguousarray(pixels, dtype=PIXEL)
+    options = ReflOptions(panel, id, int(bool(shoeboxes)),
+                          identifier.encode() if identifier else None)
+    status = lib().sf_hip_refl_write(
+        os.fsencode(path), spots.ctypes.data, len(spots), pixels.ctypes.data,
+        len(pixels), int(width), ctypes.byref(options))
+    if status == SF_OK:
+        return
+    message = lib().sf_hip_refl_last_error().decode()
+    if status == SF_IO_ERROR:
+        raise OSError(message)
+    raise ValueError(message)

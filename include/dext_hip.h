@@ -6,12 +6,13 @@
  * reference interface it stands in for (paths relative to the reference
  * repository graeme-winter/dials-metal-spotfinder).
  *
- * Status codes mirror the reference's return codes and add two:
+ * Status codes mirror the reference's return codes and add three:
  *   SF_OK (0), SF_BAD_ARGUMENT (-1, dext_gpu.hh:141 / dext_cuda.cu:588-593),
  *   SF_OVERFLOW (-2, more signal pixels than the list holds,
  *   dext_gpu.hh:148-151), SF_DEVICE_ERROR (-3, where gpu::find throws
  *   std::runtime_error, dext_gpu.hh:153-155), SF_ORDER_ERROR (-4, where
- *   dials_spots::Labeller::add throws, dials_spots.cc:58-91).
+ *   dials_spots::Labeller::add throws, dials_spots.cc:58-91), SF_IO_ERROR
+ *   (-5, where refl::write throws, refl.cc:36-80 and 160-164).
  * sf_hip_last_error() returns the calling thread's last message.
  */
 #ifndef DEXT_HIP_H
@@ -29,6 +30,7 @@ extern "C" {
 #define SF_OVERFLOW (-2)
 #define SF_DEVICE_ERROR (-3)
 #define SF_ORDER_ERROR (-4)
+#define SF_IO_ERROR (-5)
 
 /* src/signal_pixel.hh:20-28: 16 bytes, identical layout to SignalPixel. */
 typedef struct sf_signal_pixel {

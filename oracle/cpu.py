@@ -118,8 +118,23 @@ def ref_lib() -> ctypes.CDLL:
             fn.restype = _c_size
             fn.argtypes = [_ptr]
         lib.ref_labeller_results.argtypes = [_ptr, _ptr, _ptr, _ptr]
+        lib.ref_refl_write.restype = ctypes.c_int
+        lib.ref_refl_write.argtypes = [ctypes.c_char_p, _ptr, _c_size, _ptr,
+                                       _c_size, _c_size, _c_size, ctypes.c_int,
+                                       ctypes.c_char_p, ctypes.c_int]
         _ref = lib
     return _ref
+
+
+def ref_refl_write(path: str, spots: np.ndarray, pixels: np.ndarray,
+                   width: int, panel: int = 0, id: int = 0,
+                   identifier: str = "", shoeboxes: bool = True) -> int:
+    """The reference's own refl::write (src/refl.cc) on these tables."""
+    spots = np.ascontiguousarray(spots, dtype=SPOT)
+    pixels = np.ascontiguousarray(pixels, dtype=PIXEL)
+    return ref_lib().ref_refl_write(
+        path.encode(), spots.ctypes.data, len(spots), pixels.ctypes.data,
+        len(pixels), width, panel, id, identifier.encode(), int(shoeboxes))
 
 
 def _threshold(fn, image: np.ndarray, capacity: int | None):

@@ -1,7 +1,7 @@
 // C entry points over the reference's own CPU path -- TEST INFRASTRUCTURE.
 //
-// Compiled together with the reference's src/dext.cc, src/dials_spots.cc and
-// src/signal_order.cc where they lie (oracle/Makefile); the result goes to
+// Compiled together with the reference's src/dext.cc, src/dials_spots.cc,
+// src/signal_order.cc and src/refl.cc where they lie (oracle/Makefile); the result goes to
 // oracle/_ref/ and is never committed. Tests use it to pin the oracle and the
 // device path against the real reference, and bench.py times it as the CPU
 // baseline. Everything here is glue: the arithmetic is the reference's.
@@ -14,6 +14,7 @@
 
 #include "dext.hh"
 #include "dials_spots.hh"
+#include "refl.hh"
 #include "signal_order.hh"
 
 static_assert(sizeof(dials_spots::Spot) == 104, "Spot layout");
@@ -165,6 +166,26 @@ void ref_labeller_results(void *handle, dials_spots::Spot *spots,
                 l.pixels().size() * sizeof(dials_spots::Pixel));
   if (counts)
     *counts = l.counts();
+}
+
+// refl::write over C tables: 0, or -5 where it throws.
+int ref_refl_write(const char *path, const dials_spots::Spot *spots,
+                   std::size_t n_spots, const dials_spots::Pixel *pixels,
+                   std::size_t n_pixels, std::size_t width, std::size_t panel,
+                   int id, const char *identifier, int shoeboxes) {
+  try {
+    refl::Options options;
+    options.panel = panel;
+    options.id = id;
+    options.identifier = identifier ? identifier : "";
+    options.shoeboxes = shoeboxes != 0;
+    refl::write(path, std::vector<dials_spots::Spot>(spots, spots + n_spots),
+                std::vector<dials_spots::Pixel>(pixels, pixels + n_pixels),
+                width, options);
+    return 0;
+  } catch (const std::exception &) {
+    return -5;
+  }
 }
 
 } // extern "C"

@@ -12,6 +12,7 @@
 
 #include "dext_gpu.hh"
 #include "dials_spots_gpu.hh"
+#include "refl_gpu.hh"
 #include "dext_oracle.h"
 
 namespace {
@@ -127,6 +128,46 @@ int main() {
     if (!ok || spots.empty()) {
       std::printf("FAIL labeller: %zu spots vs %zu\n", ds.size(), spots.size());
       failures++;
+    }
+    // the table writer: the device tables through the refl:: interface,
+    // the oracle's through the C entry point, to the same bytes
+    {
+      const std::string a = "/tmp/sf_dropin_device.refl";
+      const std::string b = "/tmp/sf_dropin_oracle.refl";
+      refl::Options ro;
+      ro.identifier = "dropin";
+      refl::write(a, ds, dp, w, ro);
+      sf_refl_options co{0, 0, 1, "dropin"};
+      const int status = sf_hip_refl_write(
+          b.c_str(), reinterpret_cast<const sf_spot *>(spots.data()),
+          spots.size(), reinterpret_cast<const sf_pixel *>(pixels.data()),
+          pixels.size(), w, &co);
+      auto slurp = [](const std::string &path) {
+        std::vector<char> bytes;
+        if (std::FILE *f = std::fopen(path.c_str(), "rb")) {
+          char chunk[4096];
+          std::size_t n;
+          while ((n = std::fread(chunk, 1, sizeof chunk, f)) > 0)
+            bytes.insert(bytes.end(), chunk, chunk + n);
+          std::fclose(f);
+        }
+        return bytes;
+      };
+      const std::vector<char> fa = slurp(a), fb = slurp(b);
+      bool zero_width_threw = false;
+      try {
+        refl::write(a, ds, dp, 0, ro);
+      } catch (const std::exception &) {
+        zero_width_threw = true;
+      }
+      std::remove(a.c_str());
+      std::remove(b.c_str());
+      checks++;
+      if (status != SF_OK || fa.empty() || fa != fb || !zero_width_threw) {
+        std::printf("FAIL refl: status %d, %zu vs %zu bytes\n", status,
+                    fa.size(), fb.size());
+        failures++;
+      }
     }
     bool threw = false;
     try {
