@@ -165,6 +165,9 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_server_last_error", ctypes.c_char_p)
     fn("sf_hip_refl_write", _i, ctypes.c_char_p, _p, _sz, _p, _sz, _sz,
        ctypes.POINTER(ReflOptions))
+    fn("sf_hip_stamp_mask", _i, _i, _p, _i, _sz, _sz, _p, _p)
+    fn("sf_hip_pack_mask", _i, _p, _sz, _sz, _p, _p)
+    fn("sf_hip_mask_last_error", ctypes.c_char_p)
     fn("sf_hip_refl_last_error", ctypes.c_char_p)
 
 
@@ -685,3 +688,48 @@ def write_refl(path: str, spots: np.ndarray, pixels: np.ndarray, width: int,
     if status == SF_IO_ERROR:
         raise OSError(message)
     raise ValueError(message)
+
+
+# --- the detector's pixel mask (the reference's README.md:227-234) ------------
+
+def _mask_status(status: int, what: str) -> None:
+    if status != SF_OK:
+        message = lib().sf_hip_mask_last_error().decode()
+        if status == SF_DEVICE_ERROR:
+            raise DeviceError(f"{what}: {message}")
+        raise ValueError(f"{what}: {message}")
+
+
+class PixelMask:
+    """A detector pixel mask (nonzero = masked, as NXmx's pixel_mask) as the
+    device bit array sf_hip_stamp_mask reads."""
+
+    def __init__(self, mask: np.ndarray):
+        mask = np.ascontiguousarray(mask)
+        if mask.ndim != 2 or mask.size == 0:
+            raise ValueError("PixelMask: a non-empty 2-D mask")
+        self.height, self.width = mask.shape
+        packed = np.packbits(mask.reshape(-1) != 0, bitorder="little")
+        words = (mask.size + 31) // 32
+        padded = np.zeros(4 * words, dtype=np.uint8)
+        padded[:len(packed)] = packed
+        self.bits = DeviceBuffer(4 * words)
+        self.bits.upload(padded)
+
+    def stamp(self, pixel_bytes: int, frames, stream: int = 0) -> None:
+        """Writes the masked sentinel max(T) into each device frame
+        (pointers), asynchronously on `stream`; then synchronise."""
+        ptrs = [int(f) for f in frames]
+        _mask_status(lib().sf_hip_stamp_mask(
+            pixel_bytes, (_p * len(ptrs))(*ptrs), len(ptrs), self.height,
+            self.width, self.bits.ptr, stream or None), "stamp")
+
+    def close(self) -> None:
+        self.bits.close()
+
+
+def pack_mask(mask_device: int, height: int, width: int, bits_device: int,
+              stream: int = 0) -> None:
+    """sf_hip_pack_mask: a device byte mask to the device bit array."""
+    _mask_status(lib().sf_hip_pack_mask(mask_device, height, width,
+                                        bits_device, stream or None), "pack")

@@ -311,6 +311,34 @@ int sf_hip_synth_frames(void *device, int pixel_bytes, uint32_t height,
                         const double *spots, uint32_t n_spots,
                         double sigma_z);
 
+/* --- The detector's pixel mask ------------------------------------------
+ * No counterpart in the reference, which names it as its one omission
+ * (README.md:227-234): DIALS masks the pixels of
+ * /entry/instrument/detector/pixel_mask, the reference only the sentinels at
+ * or above max - 1 (src/dext.cc:40). Stamping max(T) into a frame before the
+ * threshold closes the gap: the threshold then treats a masked pixel as it
+ * treats a module gap. The mask is a device bit array: bit (p & 31) of
+ * bits[p >> 5] for pixel p of a row-major height x width frame.
+ */
+
+/* Writes max(T) (0xFFFF or 0xFFFFFFFF) at every masked pixel of each of the
+ * n_frames device frames (pixel_bytes 2 or 4), and nothing else; a clear
+ * mask byte (8 pixels) costs no frame traffic. Asynchronous on `stream` (a hipStream_t;
+ * NULL: the null stream). Returns SF_OK, SF_BAD_ARGUMENT or SF_DEVICE_ERROR
+ * (message in sf_hip_mask_last_error()). */
+int sf_hip_stamp_mask(int pixel_bytes, void *const *frames, int n_frames,
+                      size_t height, size_t width, const uint32_t *bits,
+                      void *stream);
+
+/* Packs a device byte mask (height*width bytes, nonzero = masked, as NXmx's
+ * pixel_mask) into the bit array above (ceil(height*width / 32) words).
+ * Asynchronous on `stream`. Returns SF_OK, SF_BAD_ARGUMENT or
+ * SF_DEVICE_ERROR. */
+int sf_hip_pack_mask(const uint8_t *mask, size_t height, size_t width,
+                     uint32_t *bits, void *stream);
+
+const char *sf_hip_mask_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif
