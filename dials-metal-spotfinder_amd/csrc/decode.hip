@@ -1,0 +1,863 @@
+// HDF5 chunks decoded into device frames: decompress::image
+// (src/decompress.cc:76-136) for a batch of chunks in one launch.
+//
+// The host walks each chunk's block lengths (a dependent chain of 4-byte
+// reads that is cheap on the host and serial on a device) and writes a block
+// table: where each block's LZ4 bytes sit in its chunk and where its decoded
+// bytes go in its frame. The device then gives each bitshuffle block one
+// wave: the wave copies the block's LZ4 bytes into LDS with 16-byte loads,
+// decodes the LZ4 sequences in LDS (every lane reads the same token, so the
+// parse is wave-uniform; literal and match bytes are copied 64 lanes at a
+// time), and writes the frame's elements from the bit planes, each lane 8
+// elements at a time by 8x8 bit transposes in registers. A block's bytes
+// touch HBM twice: its compressed bytes read once, its elements written
+// once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/decode_hip.h"
+
+namespace {
+
+thread_local std::string g_decode_error;
+
+int fail(int status, std::string what) {
+  g_decode_error = std::move(what);
+  return status;
+}
+
+constexpr int kWave = 64;
+constexpr size_t kHeader = 12;       // src/decompress.cc:14-16
+constexpr uint32_t kMaxShuffled = 16384; // LDS holds a block and its LZ4 bytes
+constexpr int kMinMatch = 4, kLastLiterals = 5, kMfLimit = 12;
+
+enum Kind : uint32_t { kShuffled = 0, kLz4 = 1, kRaw = 2 };
+
+// One block of the batch. `src` is the byte offset of its LZ4 bytes (after
+// the 4-byte length) or raw bytes in chunk `chunk`; `dst` the byte offset of
+// its decoded bytes in frame `chunk`.
+struct Block {
+  uint64_t src;
+  uint64_t dst;
+  uint32_t comp;  // LZ4 bytes (0 for a raw copy)
+  uint32_t bytes; // decoded bytes
+  uint32_t chunk;
+  uint32_t kind;
+};
+static_assert(sizeof(Block) == 32, "block table entry");
+
+uint32_t be32(const uint8_t *p) {
+  return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) |
+         ((uint32_t)p[2] << 8) | p[3];
+}
+
+size_t lz4_bound(size_t n) { return n + n / 255 + 16; }
+
+// bitshuffle's default block: 8192 bytes of elements, a multiple of 8, at
+// least 128 elements.
+size_t default_block(size_t elem_size) {
+  const size_t block = 8192 / elem_size / 8 * 8;
+  return std::max<size_t>(block, 128);
+}
+
+// ---------------------------------------------------------------- device
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// The 8x8 bit matrix in x (byte r, bit c) transposed: bit 8r + c moves to
+// bit 8c + r. Byte k of the input holds bit plane k for 8 elements; byte i
+// of the output is element i's byte.
+__device__ __forceinline__ uint64_t transpose8(uint64_t x) {
+  uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+  x ^= t ^ (t << 7);
+  t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+  x ^= t ^ (t << 14);
+  t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+  x ^= t ^ (t << 28);
+  return x;
+}
+
+// len bytes from src to dst, 64 lanes at a time, every load of a round
+// issued before its stores (src never overlaps the bytes written).
+template <typename I>
+__device__ __forceinline__ void wave_copy(uint8_t *dst, const uint8_t *src,
+                                          I len, int lane) {
+  for (I base = 0; base < len; base += 4 * kWave) {
+    uint8_t v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const I k = base + u * kWave + lane;
+      v[u] = k < len ? src[k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const I k = base + u * kWave + lane;
+      if (k < len)
+        dst[k] = v[u];
+    }
+  }
+}
+
+// A match of len bytes at distance off < len: the pattern of the off bytes
+// before dst repeats, dst[k] = dst[k % off - off]. Every byte read lies
+// before dst, so no byte read is one this copy writes.
+template <typename I>
+__device__ __forceinline__ void wave_repeat(uint8_t *dst, I off, I len,
+                                            int lane) {
+  const uint8_t *pattern = dst - off;
+  I r[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    r[u] = (I)(u * kWave + lane) % off;
+  const I step = (I)(4 * kWave) % off;
+  for (I base = 0; base < len; base += 4 * kWave) {
+    uint8_t v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const I k = base + u * kWave + lane;
+      v[u] = k < len ? pattern[r[u]] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const I k = base + u * kWave + lane;
+      if (k < len)
+        dst[k] = v[u];
+      r[u] += step;
+      if (r[u] >= off)
+        r[u] -= off;
+    }
+  }
+}
+
+// One LZ4 block (the published block format, with LZ4_decompress_safe's
+// end-of-block rules) from src[0, n) into dst[0, cap) by the whole wave:
+// the bytes produced, or -1 when the block is malformed. Every length and
+// offset is read by all lanes from one address and made uniform, so the
+// parse runs on the scalar unit.
+template <typename I>
+__device__ I wave_lz4(const uint8_t *src, I n, uint8_t *dst, I cap,
+                      int lane) {
+  I i = 0, o = 0;
+  for (;;) {
+    if (i >= n)
+      return -1;
+    const uint32_t token = uniform(src[i]);
+    i++;
+    I lit = token >> 4;
+    if (lit == 15) {
+      uint32_t b;
+      do {
+        if (i >= n)
+          return -1;
+        b = uniform(src[i]);
+        i++;
+        lit += b;
+      } while (b == 255);
+    }
+    const bool last = o + lit + kMfLimit > cap || i + lit + 8 > n;
+    if (last && (i + lit != n || o + lit > cap))
+      return -1;
+    wave_copy<I>(dst + o, src + i, lit, lane);
+    i += lit;
+    o += lit;
+    if (last)
+      return o;
+    const I off = uniform(src[i]) | (uniform(src[i + 1]) << 8);
+    i += 2;
+    if (off == 0 || off > o)
+      return -1;
+    I len = token & 15;
+    if (len == 15) {
+      uint32_t b;
+      do {
+        if (i >= n)
+          return -1;
+        b = uniform(src[i]);
+        i++;
+        len += b;
+      } while (b == 255);
+    }
+    len += kMinMatch;
+    if (o + len + kLastLiterals > cap)
+      return -1;
+    // The match reads bytes the lanes have just written.
+    __syncthreads();
+    if (off >= len)
+      wave_copy<I>(dst + o, dst + o - off, len, lane);
+    else
+      wave_repeat<I>(dst + o, off, len, lane);
+    o += len;
+    __syncthreads();
+  }
+}
+
+// Elements of ES bytes from the bit planes of one block (groups = elements
+// / 8, plane p at planes + p * groups), each lane 8 elements at a time.
+template <int ES>
+__device__ __forceinline__ void unshuffle(const uint8_t *planes, int groups,
+                                          uint8_t *out, bool aligned,
+                                          int lane) {
+  for (int g = lane; g < groups; g += kWave) {
+    uint64_t x[ES];
+#pragma unroll
+    for (int j = 0; j < ES; j++) {
+      uint64_t v = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        v |= (uint64_t)planes[(8 * j + k) * groups + g] << (8 * k);
+      x[j] = transpose8(v);
+    }
+    // element i's byte j is byte i of x[j]
+    uint8_t *at = out + (size_t)g * 8 * ES;
+    if (ES == 1) {
+      if (aligned)
+        *reinterpret_cast<uint64_t *>(at) = x[0];
+      else
+        for (int i = 0; i < 8; i++)
+          at[i] = (uint8_t)(x[0] >> (8 * i));
+    } else {
+      uint32_t w[2 * ES]; // 8 elements of ES bytes as 32-bit words
+#pragma unroll
+      for (int q = 0; q < 2 * ES; q++) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int byte = 4 * q + b; // output byte: element byte / ES
+          const int i = byte / ES, j = byte % ES;
+          word |= (uint32_t)((x[j] >> (8 * i)) & 0xFF) << (8 * b);
+        }
+        w[q] = word;
+      }
+      if (aligned) {
+        uint4 *v = reinterpret_cast<uint4 *>(at);
+#pragma unroll
+        for (int q = 0; q < 2 * ES; q += 4)
+          v[q / 4] = make_uint4(w[q], w[q + 1], w[q + 2], w[q + 3]);
+      } else {
+        for (int q = 0; q < 2 * ES; q++)
+          for (int b = 0; b < 4; b++)
+            at[4 * q + b] = (uint8_t)(w[q] >> (8 * b));
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void zero_status(int32_t *status, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    status[i] = 0;
+}
+
+// One bitshuffle block a workgroup of one wave. LDS: `stage` bytes for the
+// LZ4 bytes (from the 16-byte boundary below them), then the bit planes.
+template <int ES>
+__global__ __launch_bounds__(64) void shuffled_kernel(
+    const Block *blocks, const uint8_t *const *chunks, uint8_t *const *frames,
+    uint32_t stage, int32_t *status) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int lane = threadIdx.x;
+  const Block b = blocks[blockIdx.x];
+  const uint8_t *src = chunks[b.chunk] + b.src;
+  // 16-byte loads from the boundary below src: each holds a byte of the
+  // block, so none reaches past the page of a byte the chunk owns.
+  const uintptr_t base = reinterpret_cast<uintptr_t>(src) & ~uintptr_t(15);
+  const uint32_t head = (uint32_t)(reinterpret_cast<uintptr_t>(src) - base);
+  const uint32_t span = head + b.comp;
+  for (uint32_t k = 16 * lane; k < span; k += 16 * kWave)
+    *reinterpret_cast<uint4 *>(lds + k) =
+        *reinterpret_cast<const uint4 *>(base + k);
+  __syncthreads();
+  uint8_t *planes = lds + stage;
+  const int got = wave_lz4<int>(lds + head, (int)b.comp, planes,
+                                (int)b.bytes, lane);
+  if (got != (int)b.bytes) {
+    if (lane == 0)
+      status[b.chunk] = -91;
+    return;
+  }
+  __syncthreads();
+  uint8_t *out = frames[b.chunk] + b.dst;
+  const bool aligned = (reinterpret_cast<uintptr_t>(frames[b.chunk]) & 15) == 0;
+  unshuffle<ES>(planes, (int)(b.bytes / ES / 8), out, aligned, lane);
+}
+
+// One plain-LZ4 block a wave, decoded from the chunk straight into the
+// frame (a block may be the whole frame).
+__global__ __launch_bounds__(64) void lz4_kernel(const Block *blocks,
+                                                 const uint8_t *const *chunks,
+                                                 uint8_t *const *frames,
+                                                 int32_t *status) {
+  const Block b = blocks[blockIdx.x];
+  const long long got =
+      wave_lz4<long long>(chunks[b.chunk] + b.src, (long long)b.comp,
+                          frames[b.chunk] + b.dst, (long long)b.bytes,
+                          threadIdx.x);
+  if (got != (long long)b.bytes && threadIdx.x == 0)
+    status[b.chunk] = -1;
+}
+
+// The bytes a bitshuffle chunk stores raw after its blocks.
+__global__ __launch_bounds__(64) void raw_kernel(const Block *blocks,
+                                                 const uint8_t *const *chunks,
+                                                 uint8_t *const *frames) {
+  const Block b = blocks[blockIdx.x];
+  for (uint32_t k = threadIdx.x; k < b.bytes; k += kWave)
+    frames[b.chunk][b.dst + k] = chunks[b.chunk][b.src + k];
+}
+
+// ------------------------------------------------------------ host encoder
+
+size_t lz4_encode(const uint8_t *in, size_t n, uint8_t *out, size_t cap) {
+  uint32_t table[1u << 12] = {}; // position + 1, by a hash of 4 bytes
+  size_t o = 0;
+  auto put_length = [&](size_t v) {
+    while (v >= 255) {
+      if (o >= cap)
+        return false;
+      out[o++] = 255;
+      v -= 255;
+    }
+    if (o >= cap)
+      return false;
+    out[o++] = (uint8_t)v;
+    return true;
+  };
+  // A sequence: literals in[from, from + lit), then a match (mlen 0: none).
+  auto sequence = [&](size_t from, size_t lit, size_t off, size_t mlen) {
+    if (o >= cap)
+      return false;
+    const size_t token_at = o++;
+    uint8_t token = (uint8_t)(std::min<size_t>(lit, 15) << 4);
+    if (lit >= 15 && !put_length(lit - 15))
+      return false;
+    if (lit > cap - o)
+      return false;
+    memcpy(out + o, in + from, lit);
+    o += lit;
+    if (mlen) {
+      if (cap - o < 2)
+        return false;
+      out[o++] = (uint8_t)off;
+      out[o++] = (uint8_t)(off >> 8);
+      token |= (uint8_t)std::min<size_t>(mlen - kMinMatch, 15);
+      if (mlen - kMinMatch >= 15 && !put_length(mlen - kMinMatch - 15))
+        return false;
+    }
+    out[token_at] = token;
+    return true;
+  };
+  size_t anchor = 0, misses = 0;
+  // Matches start at least 12 bytes before the end and end at least 5
+  // before it, as the block format requires. A run of misses takes longer
+  // steps (incompressible bit planes).
+  for (size_t i = 0; i + kMfLimit <= n;) {
+    uint32_t word, seen;
+    memcpy(&word, in + i, 4);
+    const uint32_t h = (word * 2654435761u) >> 20;
+    const size_t cand = table[h];
+    table[h] = (uint32_t)(i + 1);
+    if (cand && i - (cand - 1) <= 65535 &&
+        (memcpy(&seen, in + cand - 1, 4), seen == word)) {
+      const size_t from = cand - 1;
+      size_t len = kMinMatch;
+      while (i + len < n - kLastLiterals && in[from + len] == in[i + len])
+        len++;
+      if (!sequence(anchor, i - anchor, i - from, len))
+        return 0;
+      i += len;
+      anchor = i;
+      misses = 0;
+    } else {
+      i += 1 + (misses++ >> 6);
+    }
+  }
+  if (!sequence(anchor, n - anchor, 0, 0))
+    return 0;
+  return o;
+}
+
+// bitshuffle's forward transpose of one block of n elements (n % 8 == 0):
+// bit k of byte j of element i goes to bit i % 8 of byte i / 8 of plane
+// 8 j + k.
+void bitshuffle_block(const uint8_t *in, size_t n, size_t es, uint8_t *planes) {
+  const size_t groups = n / 8;
+  for (size_t g = 0; g < groups; g++)
+    for (size_t j = 0; j < es; j++) {
+      uint64_t x = 0;
+      for (int i = 0; i < 8; i++)
+        x |= (uint64_t)in[(8 * g + i) * es + j] << (8 * i);
+      // transpose: bit 8i + k -> bit 8k + i
+      uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+      x ^= t ^ (t << 7);
+      t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+      x ^= t ^ (t << 14);
+      t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+      x ^= t ^ (t << 28);
+      for (int k = 0; k < 8; k++)
+        planes[(8 * j + k) * groups + g] = (uint8_t)(x >> (8 * k));
+    }
+}
+
+void put_be(uint8_t *p, uint64_t v, int bytes) {
+  for (int i = bytes - 1; i >= 0; i--, v >>= 8)
+    p[i] = (uint8_t)v;
+}
+
+} // namespace
+
+// ------------------------------------------------------------- decoder
+
+struct sf_decoder {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t element = 0, expected = 0;
+  int max_frames = 0;
+  // the indexed batch
+  int n = 0;
+  bool stored = false;
+  std::string algorithm;
+  std::vector<Block> kinds[3];
+  uint32_t first[3] = {0, 0, 0}, count[3] = {0, 0, 0};
+  uint32_t max_comp = 0, max_bytes = 0;
+  Block *h_blocks = nullptr, *d_blocks = nullptr;
+  size_t block_cap = 0;
+  // chunk pointers then frame pointers, pinned and on the device
+  const void **h_ptrs = nullptr;
+  void **d_ptrs = nullptr;
+  bool ptrs_valid = false;
+  int32_t *h_status = nullptr, *d_status = nullptr;
+  uint8_t *d_stage = nullptr;
+  size_t stage_cap = 0;
+};
+
+namespace {
+
+std::string prefix(int n, int f) {
+  return n > 1 ? "chunk " + std::to_string(f) + ": " : std::string();
+}
+
+// The checks of decompress::image (src/decompress.cc:83-135) on one chunk,
+// and its blocks appended to the tables.
+int walk_chunk(sf_decoder *d, const uint8_t *data, size_t size, int f,
+               int n) {
+  const std::string at = prefix(n, f);
+  const size_t expected = d->expected, es = d->element;
+  if (d->stored) {
+    if (size != expected)
+      return fail(SF_BAD_DATA, at + "uncompressed frame is " +
+                                   std::to_string(size) + " bytes, expected " +
+                                   std::to_string(expected));
+    return SF_OK;
+  }
+  if (size < kHeader)
+    return fail(SF_BAD_DATA, at + "too short for a filter header");
+  uint64_t total = 0;
+  for (size_t i = 0; i < 8; i++)
+    total = (total << 8) | data[i];
+  const uint32_t block = be32(data + 8);
+  if (total != expected)
+    return fail(SF_BAD_DATA, at + "filter header says " +
+                                 std::to_string(total) +
+                                 " bytes uncompressed, expected " +
+                                 std::to_string(expected));
+  size_t pos = kHeader;
+  if (d->algorithm == "bslz4") {
+    const std::string bad = at + "bitshuffle decompression failed with code ";
+    const size_t elems = expected / es;
+    size_t bs = block / es;
+    if (bs == 0)
+      bs = default_block(es);
+    if (bs % 8)
+      return fail(SF_BAD_DATA, bad + "-81");
+    if (bs * es > kMaxShuffled)
+      return fail(SF_BAD_ARGUMENT,
+                  at + "bitshuffle blocks of " + std::to_string(bs * es) +
+                      " bytes exceed the device decoder's " +
+                      std::to_string(kMaxShuffled));
+    const size_t full = elems / bs;
+    const size_t last = elems % bs - elems % bs % 8;
+    size_t done = 0;
+    for (size_t k = 0; k < full + (last ? 1 : 0); k++) {
+      const size_t count = k < full ? bs : last;
+      if (size - pos < 4)
+        return fail(SF_BAD_DATA, bad + "-91");
+      const uint32_t comp = be32(data + pos);
+      pos += 4;
+      if (comp > size - pos || comp > lz4_bound(count * es))
+        return fail(SF_BAD_DATA, bad + "-91");
+      d->kinds[kShuffled].push_back(Block{pos, done * es, comp,
+                                          (uint32_t)(count * es), (uint32_t)f,
+                                          kShuffled});
+      d->max_comp = std::max(d->max_comp, comp);
+      d->max_bytes = std::max(d->max_bytes, (uint32_t)(count * es));
+      pos += comp;
+      done += count;
+    }
+    const size_t leftover = (elems - done) * es;
+    if (leftover > size - pos)
+      return fail(SF_BAD_DATA, bad + "-91");
+    if (leftover)
+      d->kinds[kRaw].push_back(Block{pos, done * es, 0, (uint32_t)leftover,
+                                     (uint32_t)f, kRaw});
+    return SF_OK;
+  }
+  // "lz4": the HDF5 LZ4 filter's framing (src/decompress.cc:42-70)
+  const size_t block_size = block > 0 ? block : expected;
+  size_t written = 0;
+  while (written < expected) {
+    if (pos + 4 > size)
+      return fail(SF_BAD_DATA, at + "lz4 data truncated");
+    const uint32_t comp = be32(data + pos);
+    pos += 4;
+    if (comp == 0 || pos + comp > size)
+      return fail(SF_BAD_DATA, at + "lz4 block size out of range");
+    const size_t want = std::min(expected - written, block_size);
+    if (want > UINT32_MAX)
+      return fail(SF_BAD_ARGUMENT, at + "an lz4 block over 4 GiB");
+    d->kinds[kLz4].push_back(
+        Block{pos, written, comp, (uint32_t)want, (uint32_t)f, kLz4});
+    pos += comp;
+    written += want;
+  }
+  return SF_OK;
+}
+
+int device_fail(const char *what, hipError_t e) {
+  return fail(SF_DEVICE_ERROR,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+} // namespace
+
+extern "C" int sf_hip_decoder_create(unsigned bit_depth, size_t height,
+                                     size_t width, int max_frames,
+                                     sf_decoder **out) {
+  if (!out || (bit_depth != 8 && bit_depth != 16 && bit_depth != 32 &&
+               bit_depth != 64) ||
+      height == 0 || width == 0 || max_frames <= 0)
+    return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_create: bad argument");
+  *out = nullptr;
+  auto *d = new sf_decoder;
+  d->element = bit_depth / 8;
+  d->expected = height * width * bit_depth / 8; // src/decompress.cc:71-74
+  d->max_frames = max_frames;
+  hipError_t e = hipGetDevice(&d->device);
+  if (e == hipSuccess)
+    e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void **>(&d->h_ptrs),
+                      2 * max_frames * sizeof(void *), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void **>(&d->d_ptrs),
+                  2 * max_frames * sizeof(void *));
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void **>(&d->h_status),
+                      max_frames * sizeof(int32_t), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void **>(&d->d_status),
+                  max_frames * sizeof(int32_t));
+  if (e != hipSuccess) {
+    sf_hip_decoder_destroy(d);
+    return device_fail("sf_hip_decoder_create", e);
+  }
+  *out = d;
+  return SF_OK;
+}
+
+extern "C" int sf_hip_decoder_index(sf_decoder *d,
+                                    const uint8_t *const *chunks,
+                                    const size_t *sizes, int n,
+                                    const char *algorithm) {
+  if (!d || !chunks || !sizes || !algorithm || n <= 0 || n > d->max_frames)
+    return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_index: bad argument");
+  for (int f = 0; f < n; f++)
+    if (!chunks[f])
+      return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_index: a null chunk");
+  // The pinned table may still be on its way to the device.
+  hipError_t e = hipStreamSynchronize(d->stream);
+  if (e != hipSuccess)
+    return device_fail("sf_hip_decoder_index", e);
+  d->n = 0;
+  d->algorithm = algorithm;
+  d->stored = d->algorithm.empty();
+  if (!d->stored && d->algorithm != "bslz4" && d->algorithm != "lz4")
+    return fail(SF_BAD_DATA, "unknown compression algorithm " + d->algorithm);
+  for (auto &k : d->kinds)
+    k.clear();
+  d->max_comp = 0;
+  d->max_bytes = 0;
+  for (int f = 0; f < n; f++) {
+    const int status = walk_chunk(d, chunks[f], sizes[f], f, n);
+    if (status != SF_OK)
+      return status;
+  }
+  size_t total = 0;
+  for (int k = 0; k < 3; k++) {
+    if (d->kinds[k].size() > (size_t)INT32_MAX)
+      return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_index: too many blocks");
+    d->first[k] = (uint32_t)total;
+    d->count[k] = (uint32_t)d->kinds[k].size();
+    total += d->kinds[k].size();
+  }
+  if (total > d->block_cap) {
+    if (d->h_blocks)
+      (void)hipHostFree(d->h_blocks);
+    if (d->d_blocks)
+      (void)hipFree(d->d_blocks);
+    d->h_blocks = nullptr;
+    d->d_blocks = nullptr;
+    d->block_cap = 0;
+    const size_t cap = std::max<size_t>(total, 1024);
+    e = hipHostMalloc(reinterpret_cast<void **>(&d->h_blocks),
+                      cap * sizeof(Block), hipHostMallocDefault);
+    if (e == hipSuccess)
+      e = hipMalloc(reinterpret_cast<void **>(&d->d_blocks),
+                    cap * sizeof(Block));
+    if (e != hipSuccess)
+      return device_fail("sf_hip_decoder_index", e);
+    d->block_cap = cap;
+  }
+  for (int k = 0; k < 3; k++)
+    std::copy(d->kinds[k].begin(), d->kinds[k].end(),
+              d->h_blocks + d->first[k]);
+  if (total) {
+    e = hipMemcpyAsync(d->d_blocks, d->h_blocks, total * sizeof(Block),
+                       hipMemcpyHostToDevice, d->stream);
+    if (e != hipSuccess)
+      return device_fail("sf_hip_decoder_index", e);
+  }
+  d->n = n;
+  d->ptrs_valid = false;
+  return SF_OK;
+}
+
+extern "C" int sf_hip_decoder_decode(sf_decoder *d,
+                                     const uint8_t *const *device_chunks,
+                                     void *const *device_frames) {
+  if (!d || !device_chunks || !device_frames)
+    return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_decode: bad argument");
+  if (d->n == 0)
+    return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_decode: nothing indexed");
+  const int n = d->n;
+  for (int f = 0; f < n; f++)
+    if (!device_chunks[f] || !device_frames[f])
+      return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_decode: a null pointer");
+  hipError_t e;
+  if (d->stored) {
+    for (int f = 0; f < n; f++) {
+      e = hipMemcpyAsync(device_frames[f], device_chunks[f], d->expected,
+                         hipMemcpyDeviceToDevice, d->stream);
+      if (e != hipSuccess)
+        return device_fail("sf_hip_decoder_decode", e);
+    }
+    return SF_OK;
+  }
+  bool same = d->ptrs_valid;
+  for (int f = 0; same && f < n; f++)
+    same = d->h_ptrs[f] == device_chunks[f] &&
+           d->h_ptrs[d->max_frames + f] == device_frames[f];
+  if (!same) {
+    // the pinned pointers may still be on their way to the device
+    e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess)
+      return device_fail("sf_hip_decoder_decode", e);
+    for (int f = 0; f < n; f++) {
+      d->h_ptrs[f] = device_chunks[f];
+      d->h_ptrs[d->max_frames + f] = device_frames[f];
+    }
+    e = hipMemcpyAsync(d->d_ptrs, d->h_ptrs,
+                       2 * d->max_frames * sizeof(void *),
+                       hipMemcpyHostToDevice, d->stream);
+    if (e != hipSuccess)
+      return device_fail("sf_hip_decoder_decode", e);
+    d->ptrs_valid = true;
+  }
+  const auto *chunks = reinterpret_cast<const uint8_t *const *>(d->d_ptrs);
+  auto *frames = reinterpret_cast<uint8_t *const *>(d->d_ptrs + d->max_frames);
+  zero_status<<<1, 64, 0, d->stream>>>(d->d_status, n);
+  if (d->count[kShuffled]) {
+    // LZ4 bytes from the 16-byte boundary below them, rounded up to 16
+    const uint32_t stage = (d->max_comp + 15 + 15) / 16 * 16;
+    const uint32_t lds = stage + (d->max_bytes + 15) / 16 * 16;
+    const Block *b = d->d_blocks + d->first[kShuffled];
+    const dim3 grid(d->count[kShuffled]);
+    switch (d->element) {
+    case 1:
+      shuffled_kernel<1><<<grid, 64, lds, d->stream>>>(b, chunks, frames,
+                                                       stage, d->d_status);
+      break;
+    case 2:
+      shuffled_kernel<2><<<grid, 64, lds, d->stream>>>(b, chunks, frames,
+                                                       stage, d->d_status);
+      break;
+    case 4:
+      shuffled_kernel<4><<<grid, 64, lds, d->stream>>>(b, chunks, frames,
+                                                       stage, d->d_status);
+      break;
+    default:
+      shuffled_kernel<8><<<grid, 64, lds, d->stream>>>(b, chunks, frames,
+                                                       stage, d->d_status);
+      break;
+    }
+  }
+  if (d->count[kLz4])
+    lz4_kernel<<<d->count[kLz4], 64, 0, d->stream>>>(
+        d->d_blocks + d->first[kLz4], chunks, frames, d->d_status);
+  if (d->count[kRaw])
+    raw_kernel<<<d->count[kRaw], 64, 0, d->stream>>>(
+        d->d_blocks + d->first[kRaw], chunks, frames);
+  e = hipGetLastError();
+  if (e != hipSuccess)
+    return device_fail("sf_hip_decoder_decode", e);
+  return SF_OK;
+}
+
+extern "C" int sf_hip_decoder_wait(sf_decoder *d) {
+  if (!d)
+    return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_wait: bad argument");
+  if (d->n == 0 || d->stored) {
+    const hipError_t e = hipStreamSynchronize(d->stream);
+    return e == hipSuccess ? SF_OK : device_fail("sf_hip_decoder_wait", e);
+  }
+  hipError_t e = hipMemcpyAsync(d->h_status, d->d_status,
+                                d->n * sizeof(int32_t),
+                                hipMemcpyDeviceToHost, d->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(d->stream);
+  if (e != hipSuccess)
+    return device_fail("sf_hip_decoder_wait", e);
+  for (int f = 0; f < d->n; f++)
+    if (d->h_status[f] != 0) {
+      const bool shuffled = d->algorithm == "bslz4";
+      return fail(SF_BAD_DATA,
+                  prefix(d->n, f) +
+                      (shuffled ? "bitshuffle decompression failed with code "
+                                : "lz4 decompression failed with code ") +
+                      std::to_string(d->h_status[f]));
+    }
+  return SF_OK;
+}
+
+extern "C" int sf_hip_decoder_image(sf_decoder *d, const uint8_t *data,
+                                    size_t size, const char *algorithm,
+                                    void *device_frame) {
+  if (!d || !data || !algorithm || !device_frame)
+    return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_image: bad argument");
+  int status = sf_hip_decoder_index(d, &data, &size, 1, algorithm);
+  if (status != SF_OK)
+    return status;
+  if (size > d->stage_cap) {
+    if (d->d_stage)
+      (void)hipFree(d->d_stage);
+    d->d_stage = nullptr;
+    d->stage_cap = 0;
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(&d->d_stage),
+                                   std::max<size_t>(size, 1 << 20));
+    if (e != hipSuccess)
+      return device_fail("sf_hip_decoder_image", e);
+    d->stage_cap = std::max<size_t>(size, 1 << 20);
+  }
+  hipError_t e = hipMemcpyAsync(d->d_stage, data, size, hipMemcpyHostToDevice,
+                                d->stream);
+  if (e != hipSuccess)
+    return device_fail("sf_hip_decoder_image", e);
+  const uint8_t *staged = d->d_stage;
+  status = sf_hip_decoder_decode(d, &staged, &device_frame);
+  if (status != SF_OK)
+    return status;
+  return sf_hip_decoder_wait(d);
+}
+
+extern "C" void *sf_hip_decoder_stream(sf_decoder *d) {
+  return d ? static_cast<void *>(d->stream) : nullptr;
+}
+
+extern "C" void sf_hip_decoder_destroy(sf_decoder *d) {
+  if (!d)
+    return;
+  if (d->stream)
+    (void)hipStreamSynchronize(d->stream);
+  if (d->h_blocks)
+    (void)hipHostFree(d->h_blocks);
+  if (d->d_blocks)
+    (void)hipFree(d->d_blocks);
+  if (d->h_ptrs)
+    (void)hipHostFree(d->h_ptrs);
+  if (d->d_ptrs)
+    (void)hipFree(d->d_ptrs);
+  if (d->h_status)
+    (void)hipHostFree(d->h_status);
+  if (d->d_status)
+    (void)hipFree(d->d_status);
+  if (d->d_stage)
+    (void)hipFree(d->d_stage);
+  if (d->stream)
+    (void)hipStreamDestroy(d->stream);
+  delete d;
+}
+
+extern "C" const char *sf_hip_decode_last_error(void) {
+  return g_decode_error.c_str();
+}
+
+// ---------------------------------------------------------- host encoders
+
+extern "C" size_t sf_lz4_compress_block(const uint8_t *src, size_t n,
+                                        uint8_t *dst, size_t cap) {
+  if ((!src && n) || !dst)
+    return 0;
+  return lz4_encode(src, n, dst, cap);
+}
+
+extern "C" size_t sf_bslz4_bound(size_t n_elems, size_t elem_size,
+                                 size_t block_size) {
+  if (elem_size == 0)
+    return 0;
+  if (block_size == 0)
+    block_size = default_block(elem_size);
+  const size_t blocks = n_elems / block_size + 1;
+  return kHeader + blocks * (4 + lz4_bound(block_size * elem_size)) +
+         8 * elem_size;
+}
+
+extern "C" int sf_bslz4_compress(const void *elements, size_t n_elems,
+                                 size_t elem_size, size_t block_size,
+                                 uint8_t *out, size_t cap, size_t *written) {
+  if ((!elements && n_elems) || !out || !written || elem_size == 0)
+    return fail(SF_BAD_ARGUMENT, "sf_bslz4_compress: bad argument");
+  if (block_size == 0)
+    block_size = default_block(elem_size);
+  if (block_size % 8)
+    return fail(SF_BAD_ARGUMENT,
+                "sf_bslz4_compress: block size not a multiple of 8");
+  if (cap < sf_bslz4_bound(n_elems, elem_size, block_size))
+    return fail(SF_OVERFLOW, "sf_bslz4_compress: output below the bound");
+  const auto *in = static_cast<const uint8_t *>(elements);
+  put_be(out, (uint64_t)n_elems * elem_size, 8);
+  put_be(out + 8, (uint64_t)(block_size * elem_size), 4);
+  size_t o = kHeader, done = 0;
+  std::vector<uint8_t> planes(block_size * elem_size);
+  const size_t full = n_elems / block_size;
+  const size_t last = n_elems % block_size - n_elems % block_size % 8;
+  for (size_t k = 0; k < full + (last ? 1 : 0); k++) {
+    const size_t count = k < full ? block_size : last;
+    bitshuffle_block(in + done * elem_size, count, elem_size, planes.data());
+    const size_t comp = lz4_encode(planes.data(), count * elem_size,
+                                   out + o + 4, cap - o - 4);
+    if (comp == 0)
+      return fail(SF_OVERFLOW, "sf_bslz4_compress: output below the bound");
+    put_be(out + o, comp, 4);
+    o += 4 + comp;
+    done += count;
+  }
+  const size_t leftover = (n_elems - done) * elem_size;
+  memcpy(out + o, in + done * elem_size, leftover);
+  *written = o + leftover;
+  return SF_OK;
+}

@@ -27,6 +27,7 @@ SF_OVERFLOW = -2
 SF_DEVICE_ERROR = -3
 SF_ORDER_ERROR = -4
 SF_IO_ERROR = -5
+SF_BAD_DATA = -6
 
 DIRECT, TILE = 0, 1
 
@@ -169,6 +170,19 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_pack_mask", _i, _p, _sz, _sz, _p, _p)
     fn("sf_hip_mask_last_error", ctypes.c_char_p)
     fn("sf_hip_refl_last_error", ctypes.c_char_p)
+    fn("sf_hip_decoder_create", _i, ctypes.c_uint, _sz, _sz, _i,
+       ctypes.POINTER(_p))
+    fn("sf_hip_decoder_index", _i, _p, _p, _p, _i, ctypes.c_char_p)
+    fn("sf_hip_decoder_decode", _i, _p, _p, _p)
+    fn("sf_hip_decoder_wait", _i, _p)
+    fn("sf_hip_decoder_image", _i, _p, _p, _sz, ctypes.c_char_p, _p)
+    fn("sf_hip_decoder_stream", _p, _p)
+    fn("sf_hip_decoder_destroy", None, _p)
+    fn("sf_hip_decode_last_error", ctypes.c_char_p)
+    fn("sf_bslz4_bound", _sz, _sz, _sz, _sz)
+    fn("sf_bslz4_compress", _i, _p, _sz, _sz, _sz, _p, _sz,
+       ctypes.POINTER(_sz))
+    fn("sf_lz4_compress_block", _sz, _p, _sz, _p, _sz)
 
 
 _lib: ctypes.CDLL | None = None
@@ -733,3 +747,105 @@ def pack_mask(mask_device: int, height: int, width: int, bits_device: int,
     """sf_hip_pack_mask: a device byte mask to the device bit array."""
     _mask_status(lib().sf_hip_pack_mask(mask_device, height, width,
                                         bits_device, stream or None), "pack")
+
+
+class DecodeError(RuntimeError):
+    """A chunk that does not decode to its frame: decompress::image's
+    std::runtime_error, with its message."""
+
+
+def _decode_status(status: int, what: str) -> None:
+    if status != SF_OK:
+        message = lib().sf_hip_decode_last_error().decode()
+        if status == SF_DEVICE_ERROR:
+            raise DeviceError(f"{what}: {message}")
+        if status == SF_BAD_DATA:
+            raise DecodeError(message)
+        raise ValueError(f"{what}: {message}")
+
+
+def bslz4_compress(elements: np.ndarray, block_size: int = 0) -> bytes:
+    """The HDF5 bitshuffle filter's chunk for these elements (host code):
+    12-byte header, bitshuffled LZ4 blocks, the last n % 8 elements raw."""
+    elements = np.ascontiguousarray(elements)
+    n, es = elements.size, elements.itemsize
+    cap = lib().sf_bslz4_bound(n, es, block_size)
+    out = np.empty(cap, dtype=np.uint8)
+    written = _sz(0)
+    _decode_status(lib().sf_bslz4_compress(
+        elements.ctypes.data, n, es, block_size, out.ctypes.data, cap,
+        ctypes.byref(written)), "bslz4_compress")
+    return out[:written.value].tobytes()
+
+
+def lz4_compress_block(data: bytes) -> bytes:
+    """One LZ4 block (the published block format), host code."""
+    src = np.frombuffer(data, dtype=np.uint8)
+    cap = len(data) + len(data) // 255 + 16
+    out = np.empty(cap, dtype=np.uint8)
+    got = lib().sf_lz4_compress_block(src.ctypes.data if len(data) else None,
+                                      len(data), out.ctypes.data, cap)
+    if got == 0:
+        raise ValueError("lz4_compress_block: output too small")
+    return out[:got].tobytes()
+
+
+class Decoder:
+    """HDF5 chunks ("bslz4", "lz4" or "" stored) decoded into device frames
+    of height x width elements of bit_depth bits: decompress::image
+    (src/decompress.cc:76-136) on the device, a batch a launch."""
+
+    def __init__(self, bit_depth: int, height: int, width: int,
+                 max_frames: int = 64):
+        handle = _p()
+        _decode_status(lib().sf_hip_decoder_create(
+            bit_depth, height, width, max_frames, ctypes.byref(handle)),
+            "Decoder")
+        self.handle = handle.value
+        self.frame_bytes = height * width * bit_depth // 8
+        self._held = None
+
+    def index(self, chunks, algorithm: str = "bslz4") -> None:
+        """Walks the chunks (bytes in host memory) and uploads their block
+        table; raises DecodeError with the reference's message."""
+        chunks = [np.frombuffer(c if len(c) else b"\0", dtype=np.uint8)[
+            :len(c)] for c in chunks]
+        self._held = chunks  # the walk reads them; keep them alive
+        ptrs = (_p * len(chunks))(*[c.ctypes.data for c in chunks])
+        sizes = (_sz * len(chunks))(*[len(c) for c in chunks])
+        _decode_status(lib().sf_hip_decoder_index(
+            self.handle, ptrs, sizes, len(chunks), algorithm.encode()),
+            "index")
+
+    def decode(self, device_chunks, device_frames) -> None:
+        """Decodes the indexed chunks, held at the device pointers
+        device_chunks, into device_frames; asynchronous."""
+        c = [int(x) for x in device_chunks]
+        f = [int(x) for x in device_frames]
+        _decode_status(lib().sf_hip_decoder_decode(
+            self.handle, (_p * len(c))(*c), (_p * len(f))(*f)), "decode")
+
+    def wait(self) -> None:
+        _decode_status(lib().sf_hip_decoder_wait(self.handle), "wait")
+
+    def image(self, data: bytes, algorithm: str, device_frame: int) -> None:
+        """decompress::image into one device frame (synchronous)."""
+        src = np.frombuffer(data if len(data) else b"\0", dtype=np.uint8)
+        _decode_status(lib().sf_hip_decoder_image(
+            self.handle, src.ctypes.data, len(data), algorithm.encode(),
+            int(device_frame)), "image")
+
+    @property
+    def stream(self) -> int:
+        return lib().sf_hip_decoder_stream(self.handle) or 0
+
+    def close(self) -> None:
+        if self.handle:
+            lib().sf_hip_decoder_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

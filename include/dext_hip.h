@@ -31,6 +31,7 @@ extern "C" {
 #define SF_DEVICE_ERROR (-3)
 #define SF_ORDER_ERROR (-4)
 #define SF_IO_ERROR (-5)
+#define SF_BAD_DATA (-6) /* a chunk that does not decode to its frame */
 
 /* src/signal_pixel.hh:20-28: 16 bytes, identical layout to SignalPixel. */
 typedef struct sf_signal_pixel {

@@ -4,7 +4,8 @@ Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
 this module, and only to check or time: nothing in the product imports it.
 
     oracle   liboracle.so, the plain-C restatement of src/dext.cc,
-             src/signal_order.cc and src/dials_spots.cc (dext_oracle.c)
+             src/signal_order.cc and src/dials_spots.cc (dext_oracle.c) and
+             of src/decompress.cc (bslz4_oracle.c)
     ref      _ref/libdext_ref.so, the reference's own sources compiled by
              oracle/Makefile (absent when the reference was not there to
              build; every user checks `have_ref()`)
@@ -77,6 +78,21 @@ def oracle_lib() -> ctypes.CDLL:
         lib.oracle_labeller_pixel_count.restype = _c_size
         lib.oracle_labeller_pixel_count.argtypes = [_ptr]
         lib.oracle_labeller_results.argtypes = [_ptr, _ptr, _ptr, _ptr]
+        lib.oracle_lz4_block.restype = ctypes.c_long
+        lib.oracle_lz4_block.argtypes = [_ptr, _c_size, _ptr, _c_size]
+        for name in ("oracle_bitshuffle", "oracle_bitunshuffle"):
+            fn = getattr(lib, name)
+            fn.restype = None
+            fn.argtypes = [_ptr, _c_size, _c_size, _ptr]
+        lib.oracle_bshuf_default_block.restype = _c_size
+        lib.oracle_bshuf_default_block.argtypes = [_c_size]
+        lib.oracle_bshuf_decompress_lz4.restype = ctypes.c_longlong
+        lib.oracle_bshuf_decompress_lz4.argtypes = [_ptr, _c_size, _ptr,
+                                                    _c_size, _c_size, _c_size]
+        lib.oracle_decompress_image.restype = ctypes.c_int
+        lib.oracle_decompress_image.argtypes = [
+            _ptr, _c_size, ctypes.c_char_p, ctypes.c_uint, _c_size, _c_size,
+            _ptr, _c_size, ctypes.c_char_p, _c_size]
         lib.oracle_synthetic_frame.restype = None
         lib.oracle_synthetic_frame.argtypes = [_ptr, _c_size, _c_size,
                                                ctypes.c_uint64, ctypes.c_double,
@@ -271,3 +287,50 @@ class RefLabeller(_LabellerBase):
 
     def live(self) -> int:
         return int(self._lib.ref_labeller_live(self._handle))
+
+
+def _bytes_ptr(data: bytes) -> tuple[np.ndarray, int]:
+    array = np.frombuffer(data if len(data) else b"\0", dtype=np.uint8)
+    return array, array.ctypes.data
+
+
+def lz4_block(data: bytes, cap: int) -> bytes | None:
+    """The oracle's LZ4 block decoder: the bytes, or None when malformed."""
+    src, at = _bytes_ptr(data)
+    out = np.zeros(max(cap, 1), dtype=np.uint8)
+    got = oracle_lib().oracle_lz4_block(at, len(data), out.ctypes.data, cap)
+    return None if got < 0 else out[:got].tobytes()
+
+
+def bitshuffle(elements: np.ndarray) -> np.ndarray:
+    """bitshuffle's transpose of one block (size a multiple of 8) as bytes."""
+    elements = np.ascontiguousarray(elements)
+    out = np.zeros(elements.nbytes, dtype=np.uint8)
+    oracle_lib().oracle_bitshuffle(elements.ctypes.data, elements.size,
+                                   elements.itemsize, out.ctypes.data)
+    return out
+
+
+def bitunshuffle(planes: np.ndarray, dtype) -> np.ndarray:
+    planes = np.ascontiguousarray(planes, dtype=np.uint8)
+    dtype = np.dtype(dtype)
+    out = np.zeros(planes.size // dtype.itemsize, dtype=dtype)
+    oracle_lib().oracle_bitunshuffle(planes.ctypes.data, out.size,
+                                     dtype.itemsize, out.ctypes.data)
+    return out
+
+
+def decompress_image(data: bytes, algorithm: str, bit_depth: int,
+                     height: int, width: int) -> tuple[bytes | None, str]:
+    """decompress::image (src/decompress.cc:76-136): (frame bytes, "") or
+    (None, the reference's message)."""
+    src, at = _bytes_ptr(data)
+    expected = height * width * bit_depth // 8
+    out = np.zeros(max(expected, 1), dtype=np.uint8)
+    msg = ctypes.create_string_buffer(256)
+    status = oracle_lib().oracle_decompress_image(
+        at, len(data), algorithm.encode(), bit_depth, height, width,
+        out.ctypes.data, expected, msg, len(msg))
+    if status != 0:
+        return None, msg.value.decode()
+    return out[:expected].tobytes(), ""
