@@ -434,6 +434,9 @@ struct sf_decoder {
   int32_t *h_status = nullptr, *d_status = nullptr;
   uint8_t *d_stage = nullptr;
   size_t stage_cap = 0;
+  // HIP events around the decode launches, when timing is on
+  bool timing = false;
+  hipEvent_t events[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -562,6 +565,8 @@ extern "C" int sf_hip_decoder_create(unsigned bit_depth, size_t height,
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void **>(&d->d_status),
                   max_frames * sizeof(int32_t));
+  for (int k = 0; k < 2 && e == hipSuccess; k++)
+    e = hipEventCreate(&d->events[k]);
   if (e != hipSuccess) {
     sf_hip_decoder_destroy(d);
     return device_fail("sf_hip_decoder_create", e);
@@ -680,6 +685,8 @@ extern "C" int sf_hip_decoder_decode(sf_decoder *d,
   }
   const auto *chunks = reinterpret_cast<const uint8_t *const *>(d->d_ptrs);
   auto *frames = reinterpret_cast<uint8_t *const *>(d->d_ptrs + d->max_frames);
+  if (d->timing)
+    (void)hipEventRecord(d->events[0], d->stream);
   zero_status<<<1, 64, 0, d->stream>>>(d->d_status, n);
   if (d->count[kShuffled]) {
     // LZ4 bytes from the 16-byte boundary below them, rounded up to 16
@@ -712,6 +719,8 @@ extern "C" int sf_hip_decoder_decode(sf_decoder *d,
   if (d->count[kRaw])
     raw_kernel<<<d->count[kRaw], 64, 0, d->stream>>>(
         d->d_blocks + d->first[kRaw], chunks, frames);
+  if (d->timing)
+    (void)hipEventRecord(d->events[1], d->stream);
   e = hipGetLastError();
   if (e != hipSuccess)
     return device_fail("sf_hip_decoder_decode", e);
@@ -774,6 +783,22 @@ extern "C" int sf_hip_decoder_image(sf_decoder *d, const uint8_t *data,
   return sf_hip_decoder_wait(d);
 }
 
+extern "C" int sf_hip_decoder_set_timing(sf_decoder *d, int on) {
+  if (!d)
+    return fail(SF_BAD_ARGUMENT, "sf_hip_decoder_set_timing: bad argument");
+  d->timing = on != 0;
+  return SF_OK;
+}
+
+extern "C" double sf_hip_decoder_last_ms(sf_decoder *d) {
+  float ms = -1.0f;
+  if (!d || !d->timing ||
+      hipEventSynchronize(d->events[1]) != hipSuccess ||
+      hipEventElapsedTime(&ms, d->events[0], d->events[1]) != hipSuccess)
+    return -1.0;
+  return ms;
+}
+
 extern "C" void *sf_hip_decoder_stream(sf_decoder *d) {
   return d ? static_cast<void *>(d->stream) : nullptr;
 }
@@ -797,6 +822,9 @@ extern "C" void sf_hip_decoder_destroy(sf_decoder *d) {
     (void)hipFree(d->d_status);
   if (d->d_stage)
     (void)hipFree(d->d_stage);
+  for (hipEvent_t ev : d->events)
+    if (ev)
+      (void)hipEventDestroy(ev);
   if (d->stream)
     (void)hipStreamDestroy(d->stream);
   delete d;

@@ -177,6 +177,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_decoder_wait", _i, _p)
     fn("sf_hip_decoder_image", _i, _p, _p, _sz, ctypes.c_char_p, _p)
     fn("sf_hip_decoder_stream", _p, _p)
+    fn("sf_hip_decoder_set_timing", _i, _p, _i)
+    fn("sf_hip_decoder_last_ms", ctypes.c_double, _p)
     fn("sf_hip_decoder_destroy", None, _p)
     fn("sf_hip_decode_last_error", ctypes.c_char_p)
     fn("sf_bslz4_bound", _sz, _sz, _sz, _sz)
@@ -834,6 +836,14 @@ class Decoder:
         _decode_status(lib().sf_hip_decoder_image(
             self.handle, src.ctypes.data, len(data), algorithm.encode(),
             int(device_frame)), "image")
+
+    def set_timing(self, on: bool) -> None:
+        _decode_status(lib().sf_hip_decoder_set_timing(self.handle, int(on)),
+                       "set_timing")
+
+    def last_ms(self) -> float:
+        """Device time of the last decode (HIP events on its stream)."""
+        return float(lib().sf_hip_decoder_last_ms(self.handle))
 
     @property
     def stream(self) -> int:

@@ -66,6 +66,12 @@ int sf_hip_decoder_wait(sf_decoder *dec);
 int sf_hip_decoder_image(sf_decoder *dec, const uint8_t *data, size_t size,
                          const char *algorithm, void *device_frame);
 
+/* HIP events on the decoder's stream around each decode's launches (off by
+ * default); sf_hip_decoder_last_ms waits for the last timed decode and
+ * returns its device time in ms, or -1 when timing is off. */
+int sf_hip_decoder_set_timing(sf_decoder *dec, int on);
+double sf_hip_decoder_last_ms(sf_decoder *dec);
+
 /* The decoder's stream (a hipStream_t), to order the threshold after it. */
 void *sf_hip_decoder_stream(sf_decoder *dec);
 void sf_hip_decoder_destroy(sf_decoder *dec);
