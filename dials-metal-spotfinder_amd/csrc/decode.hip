@@ -84,24 +84,26 @@ __device__ __forceinline__ uint64_t transpose8(uint64_t x) {
   return x;
 }
 
-// len bytes from src to dst, 64 lanes at a time, every load of a round
-// issued before its stores (src never overlaps the bytes written).
+// len bytes from src to dst, 64 lanes at a time (four rounds' loads issued
+// before their stores while 256 bytes remain). No byte read is one this
+// copy writes.
 template <typename I>
 __device__ __forceinline__ void wave_copy(uint8_t *dst, const uint8_t *src,
                                           I len, int lane) {
-  for (I base = 0; base < len; base += 4 * kWave) {
+  I base = 0;
+  for (; base + 4 * kWave <= len; base += 4 * kWave) {
     uint8_t v[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const I k = base + u * kWave + lane;
-      v[u] = k < len ? src[k] : 0;
-    }
+    for (int u = 0; u < 4; u++)
+      v[u] = src[base + u * kWave + lane];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const I k = base + u * kWave + lane;
-      if (k < len)
-        dst[k] = v[u];
-    }
+    for (int u = 0; u < 4; u++)
+      dst[base + u * kWave + lane] = v[u];
+  }
+  for (; base < len; base += kWave) {
+    const I k = base + lane;
+    if (k < len)
+      dst[k] = src[k];
   }
 }
 
@@ -112,43 +114,52 @@ template <typename I>
 __device__ __forceinline__ void wave_repeat(uint8_t *dst, I off, I len,
                                             int lane) {
   const uint8_t *pattern = dst - off;
-  I r[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++)
-    r[u] = (I)(u * kWave + lane) % off;
-  const I step = (I)(4 * kWave) % off;
-  for (I base = 0; base < len; base += 4 * kWave) {
-    uint8_t v[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const I k = base + u * kWave + lane;
-      v[u] = k < len ? pattern[r[u]] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const I k = base + u * kWave + lane;
-      if (k < len)
-        dst[k] = v[u];
-      r[u] += step;
-      if (r[u] >= off)
-        r[u] -= off;
-    }
+  if (off == 1) {
+    const uint8_t v = pattern[0];
+    for (I k = lane; k < len; k += kWave)
+      dst[k] = v;
+    return;
   }
+  I r = (I)lane % off;
+  const I step = (I)kWave % off;
+  for (I base = 0; base < len; base += kWave) {
+    const I k = base + lane;
+    if (k < len)
+      dst[k] = pattern[r];
+    r += step;
+    if (r >= off)
+      r -= off;
+  }
+}
+
+// Byte q of src: from the window (byte w0 + lane of src in each lane) when
+// it holds it, else read by every lane from src.
+template <typename I>
+__device__ __forceinline__ uint32_t byte_at(const uint8_t *src, uint32_t win,
+                                            I w0, I q) {
+  if (q - w0 < kWave)
+    return __builtin_amdgcn_readlane(win, (int)(q - w0));
+  return uniform(src[q]);
 }
 
 // One LZ4 block (the published block format, with LZ4_decompress_safe's
 // end-of-block rules) from src[0, n) into dst[0, cap) by the whole wave:
-// the bytes produced, or -1 when the block is malformed. Every length and
-// offset is read by all lanes from one address and made uniform, so the
-// parse runs on the scalar unit.
+// the bytes produced, or -1 when the block is malformed. Each sequence
+// starts with one read of the 64 bytes at its token, a byte a lane: the
+// token, lengths and offset come from it by readlane (uniform, so the parse
+// runs on the scalar unit), and literals that lie in it are written from
+// it. A match waits for the wave's stores only when it reads bytes written
+// since the last wait.
 template <typename I>
 __device__ I wave_lz4(const uint8_t *src, I n, uint8_t *dst, I cap,
                       int lane) {
-  I i = 0, o = 0;
+  I i = 0, o = 0, synced = 0;
   for (;;) {
     if (i >= n)
       return -1;
-    const uint32_t token = uniform(src[i]);
+    const I w0 = i;
+    const uint32_t win = w0 + lane < n ? (uint32_t)src[w0 + lane] : 0u;
+    const uint32_t token = __builtin_amdgcn_readlane(win, 0);
     i++;
     I lit = token >> 4;
     if (lit == 15) {
@@ -156,7 +167,7 @@ __device__ I wave_lz4(const uint8_t *src, I n, uint8_t *dst, I cap,
       do {
         if (i >= n)
           return -1;
-        b = uniform(src[i]);
+        b = byte_at<I>(src, win, w0, i);
         i++;
         lit += b;
       } while (b == 255);
@@ -164,12 +175,19 @@ __device__ I wave_lz4(const uint8_t *src, I n, uint8_t *dst, I cap,
     const bool last = o + lit + kMfLimit > cap || i + lit + 8 > n;
     if (last && (i + lit != n || o + lit > cap))
       return -1;
-    wave_copy<I>(dst + o, src + i, lit, lane);
+    if (i + lit <= w0 + kWave) {
+      const I k = (I)lane - (i - w0); // window lanes [i - w0, +lit)
+      if (k >= 0 && k < lit)
+        dst[o + k] = (uint8_t)win;
+    } else {
+      wave_copy<I>(dst + o, src + i, lit, lane);
+    }
     i += lit;
     o += lit;
     if (last)
       return o;
-    const I off = uniform(src[i]) | (uniform(src[i + 1]) << 8);
+    const I off = byte_at<I>(src, win, w0, i) |
+                  (byte_at<I>(src, win, w0, i + 1) << 8);
     i += 2;
     if (off == 0 || off > o)
       return -1;
@@ -179,7 +197,7 @@ __device__ I wave_lz4(const uint8_t *src, I n, uint8_t *dst, I cap,
       do {
         if (i >= n)
           return -1;
-        b = uniform(src[i]);
+        b = byte_at<I>(src, win, w0, i);
         i++;
         len += b;
       } while (b == 255);
@@ -187,14 +205,133 @@ __device__ I wave_lz4(const uint8_t *src, I n, uint8_t *dst, I cap,
     len += kMinMatch;
     if (o + len + kLastLiterals > cap)
       return -1;
-    // The match reads bytes the lanes have just written.
-    __syncthreads();
+    if (o - off + (off < len ? off : len) > synced) {
+      __syncthreads(); // the match reads bytes the lanes have just written
+      synced = o;
+    }
     if (off >= len)
       wave_copy<I>(dst + o, dst + o - off, len, lane);
     else
       wave_repeat<I>(dst + o, off, len, lane);
     o += len;
-    __syncthreads();
+  }
+}
+
+// A wave's LDS operations execute in order: once the compiler keeps them in
+// order too, its lanes see each other's LDS stores.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// k mod m for 0 <= k, m < 2^16, m > 0, on the vector unit: the float
+// quotient is within one of the true one, and one correction fixes it.
+__device__ __forceinline__ int mod_small(int k, int m, float inv) {
+  int r = k - (int)((float)k * inv) * m;
+  r = r < 0 ? r + m : r;
+  return r >= m ? r - m : r;
+}
+
+// len bytes from src to dst within LDS, no byte read being one this copy
+// writes. Idle lanes store to their own byte of `trash` rather than leave
+// the wave's exec mask (no scalar mask work); reads may run up to 63 bytes
+// past src + len.
+__device__ __forceinline__ void lds_copy(uint8_t *dst, const uint8_t *src,
+                                         int len, int lane, uint8_t *trash) {
+  int base = 0;
+  for (; base + 4 * kWave <= len; base += 4 * kWave) {
+    uint8_t v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      v[u] = src[base + u * kWave + lane];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      dst[base + u * kWave + lane] = v[u];
+  }
+  for (; base < len; base += kWave) {
+    const int k = base + lane;
+    const uint8_t v = src[k];
+    *(k < len ? dst + k : trash) = v;
+  }
+}
+
+// One LZ4 block held in LDS (src[0, n), readable to n + 63) into LDS
+// (dst[0, cap), with 64 bytes after it for idle lanes): the same parse and
+// checks as wave_lz4, shaped for the scalar unit, which bounds this kernel:
+// no guard on the window read, no exec-mask branches around copies, k mod
+// off on the vector unit.
+__device__ int lds_lz4(const uint8_t *src, int n, uint8_t *dst, int cap,
+                       int lane) {
+  uint8_t *trash = dst + cap + lane;
+  int i = 0, o = 0;
+  for (;;) {
+    const uint32_t win = src[i + lane];
+    const uint32_t token = __builtin_amdgcn_readlane(win, 0);
+    int p = i + 1;
+    int lit = token >> 4;
+    if (lit == 15) {
+      uint32_t b;
+      do {
+        if (p >= n)
+          return -1;
+        b = byte_at<int>(src, win, i, p);
+        p++;
+        lit += b;
+      } while (b == 255);
+    }
+    const int rem_out = cap - o;
+    if (lit + kMfLimit > rem_out || lit + 8 > n - p) {
+      if (lit != n - p || lit > rem_out)
+        return -1;
+      lds_copy(dst + o, src + p, lit, lane, trash);
+      return o + lit;
+    }
+    if (p + lit <= i + kWave) {
+      const int k = lane - (p - i); // the literals are window lanes
+      *((unsigned)k < (unsigned)lit ? dst + o + k : trash) = (uint8_t)win;
+    } else {
+      lds_copy(dst + o, src + p, lit, lane, trash);
+    }
+    o += lit;
+    p += lit;
+    const int off = (int)(byte_at<int>(src, win, i, p) |
+                          (byte_at<int>(src, win, i, p + 1) << 8));
+    p += 2;
+    int len = token & 15;
+    if (len == 15) {
+      uint32_t b;
+      do {
+        if (p >= n)
+          return -1;
+        b = byte_at<int>(src, win, i, p);
+        p++;
+        len += b;
+      } while (b == 255);
+    }
+    len += kMinMatch;
+    if (off == 0 || off > o || len + kLastLiterals > cap - o)
+      return -1;
+    wave_sync(); // the match may read bytes the lanes have just written
+    uint8_t *at = dst + o;
+    if (off >= len) {
+      lds_copy(at, at - off, len, lane, trash);
+    } else if (off == 1) {
+      const uint8_t v = at[-1];
+      for (int base = 0; base < len; base += kWave) {
+        const int k = base + lane;
+        *(k < len ? at + k : trash) = v;
+      }
+    } else {
+      const float inv = 1.0f / (float)off;
+      for (int base = 0; base < len; base += kWave) {
+        const int k = base + lane;
+        const uint8_t v = at[mod_small(k, off, inv) - off];
+        *(k < len ? at + k : trash) = v;
+      }
+    }
+    o += len;
+    i = p;
   }
 }
 
@@ -255,7 +392,8 @@ __global__ __launch_bounds__(64) void zero_status(int32_t *status, int n) {
 }
 
 // One bitshuffle block a workgroup of one wave. LDS: `stage` bytes for the
-// LZ4 bytes (from the 16-byte boundary below them), then the bit planes.
+// LZ4 bytes (from the 16-byte boundary below them, with 64 bytes to spare
+// for the window reads), then the bit planes and 64 bytes for idle lanes.
 template <int ES>
 __global__ __launch_bounds__(64) void shuffled_kernel(
     const Block *blocks, const uint8_t *const *chunks, uint8_t *const *frames,
@@ -274,8 +412,8 @@ __global__ __launch_bounds__(64) void shuffled_kernel(
         *reinterpret_cast<const uint4 *>(base + k);
   __syncthreads();
   uint8_t *planes = lds + stage;
-  const int got = wave_lz4<int>(lds + head, (int)b.comp, planes,
-                                (int)b.bytes, lane);
+  const int got = lds_lz4(lds + head, (int)b.comp, planes, (int)b.bytes,
+                          lane);
   if (got != (int)b.bytes) {
     if (lane == 0)
       status[b.chunk] = -91;
@@ -689,9 +827,10 @@ extern "C" int sf_hip_decoder_decode(sf_decoder *d,
     (void)hipEventRecord(d->events[0], d->stream);
   zero_status<<<1, 64, 0, d->stream>>>(d->d_status, n);
   if (d->count[kShuffled]) {
-    // LZ4 bytes from the 16-byte boundary below them, rounded up to 16
-    const uint32_t stage = (d->max_comp + 15 + 15) / 16 * 16;
-    const uint32_t lds = stage + (d->max_bytes + 15) / 16 * 16;
+    // LZ4 bytes from the 16-byte boundary below them and 64 bytes that
+    // window reads may touch; the planes and 64 bytes for idle lanes
+    const uint32_t stage = (d->max_comp + 15 + 64 + 15) / 16 * 16;
+    const uint32_t lds = stage + (d->max_bytes + 15) / 16 * 16 + 64;
     const Block *b = d->d_blocks + d->first[kShuffled];
     const dim3 grid(d->count[kShuffled]);
     switch (d->element) {
