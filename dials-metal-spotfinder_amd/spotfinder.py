@@ -70,6 +70,18 @@ class ReflOptions(ctypes.Structure):
                 ("shoeboxes", ctypes.c_int32), ("identifier", ctypes.c_char_p)]
 
 
+class DriverOptions(ctypes.Structure):
+    """include/driver_hip.h sf_driver_options (src/find_spots.cc:70-90)."""
+    _fields_ = [("chunk", ctypes.c_int32), ("z_offset", ctypes.c_int64),
+                ("grouping", Options)]
+
+
+class DriverStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "dispatched", "read", "missing", "failed", "grouped",
+        "out_of_order")]
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "batches", "batched_frames", "admission_waits",
@@ -181,6 +193,18 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_decoder_last_ms", ctypes.c_double, _p)
     fn("sf_hip_decoder_destroy", None, _p)
     fn("sf_hip_decode_last_error", ctypes.c_char_p)
+    fn("sf_hip_driver_create", _p, ctypes.c_uint, _sz, _sz,
+       ctypes.POINTER(DriverOptions))
+    fn("sf_hip_driver_submit", _i, _p, ctypes.c_int64, _p, _sz,
+       ctypes.c_char_p)
+    fn("sf_hip_driver_finish", _i, _p)
+    fn("sf_hip_driver_write", _i, _p, ctypes.c_char_p,
+       ctypes.POINTER(ReflOptions))
+    fn("sf_hip_driver_log", _sz, _p, _p, _sz)
+    fn("sf_hip_driver_stats", _i, _p, ctypes.POINTER(DriverStats))
+    fn("sf_hip_driver_labeller", _p, _p)
+    fn("sf_hip_driver_destroy", None, _p)
+    fn("sf_hip_driver_last_error", ctypes.c_char_p)
     fn("sf_bslz4_bound", _sz, _sz, _sz, _sz)
     fn("sf_bslz4_compress", _i, _p, _sz, _sz, _sz, _p, _sz,
        ctypes.POINTER(_sz))
@@ -852,6 +876,81 @@ class Decoder:
     def close(self) -> None:
         if self.handle:
             lib().sf_hip_decoder_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Driver:
+    """The series driver (src/find_spots.cc:460-713) on the device: frames
+    submitted by number as stored chunks (None: never written), decoded,
+    thresholded and grouped in number order a chunk at a time; finish()
+    logs dials.find_spots' summary lines."""
+
+    def __init__(self, bit_depth: int, height: int, width: int,
+                 chunk: int = 0, z_offset: int = 0, min_spot_size: int = 3,
+                 max_spot_size: int = 1000, max_separation: float = 2.0,
+                 two_d: bool = False):
+        options = DriverOptions(chunk, z_offset, Options(
+            min_spot_size, max_spot_size, max_separation, int(two_d)))
+        self.handle = lib().sf_hip_driver_create(bit_depth, height, width,
+                                                 ctypes.byref(options))
+        if not self.handle:
+            raise ValueError(lib().sf_hip_driver_last_error().decode())
+
+    def _status(self, status: int, what: str) -> None:
+        if status != SF_OK:
+            message = lib().sf_hip_driver_last_error().decode()
+            if status == SF_DEVICE_ERROR:
+                raise DeviceError(f"{what}: {message}")
+            raise ValueError(f"{what}: {message}")
+
+    def submit(self, number: int, data: bytes | None,
+               algorithm: str = "bslz4") -> None:
+        if data is None:
+            status = lib().sf_hip_driver_submit(self.handle, number, None, 0,
+                                                None)
+        else:
+            src = np.frombuffer(data if len(data) else b"\0", np.uint8)
+            status = lib().sf_hip_driver_submit(
+                self.handle, number, src.ctypes.data, len(data),
+                algorithm.encode())
+        self._status(status, "submit")
+
+    def finish(self) -> None:
+        self._status(lib().sf_hip_driver_finish(self.handle), "finish")
+
+    def write(self, path: str, identifier: str = "",
+              shoeboxes: bool = True) -> None:
+        options = ReflOptions(0, 0, int(shoeboxes), identifier.encode())
+        self._status(lib().sf_hip_driver_write(self.handle, path.encode(),
+                                               ctypes.byref(options)),
+                     "write")
+
+    def log(self) -> str:
+        n = lib().sf_hip_driver_log(self.handle, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().sf_hip_driver_log(self.handle, buf, n + 1)
+        return buf.value.decode()
+
+    def stats(self) -> dict:
+        out = DriverStats()
+        self._status(lib().sf_hip_driver_stats(self.handle,
+                                               ctypes.byref(out)), "stats")
+        return {name: int(getattr(out, name)) for name, _ in out._fields_}
+
+    def results(self):
+        """(spots, pixels, counts dict) of the driver's labeller."""
+        handle = lib().sf_hip_driver_labeller(self.handle)
+        return _labeller_results(handle, self._status)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().sf_hip_driver_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
