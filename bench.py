@@ -304,7 +304,7 @@ def serve_latency(sf, fr, seconds, ring, budget_gb, max_batch):
 
 
 def measure_decode(sf, series, chunk: int, frame_bytes: int,
-                   steps: int) -> dict:
+                   steps: int) -> tuple[dict, list]:
     """One chunk of the series as the detector's HDF5 files hold frames
     (bitshuffle+LZ4, 8192-byte blocks), compressed on the host, resident in
     HBM, decoded into frames in one launch (sf_hip_decoder_*). Device time
@@ -348,7 +348,7 @@ def measure_decode(sf, series, chunk: int, frame_bytes: int,
     t = float(np.median(ms)) / 1e3
     compressed = float(sum(sizes))
     decoded = float(chunk * frame_bytes)
-    return {
+    return chunks, {
         "workload": f"{chunk} frames of {HEIGHT} x {WIDTH} u16 as "
                     "bitshuffle+LZ4 chunks (8192-byte blocks) in HBM, "
                     "decoded into frames in one launch",
@@ -361,6 +361,34 @@ def measure_decode(sf, series, chunk: int, frame_bytes: int,
         "hbm_frac": round((compressed + decoded) / t / 8e12, 4),
         "host_compress_s": round(compress_s, 2),
         "frame_diffs": diffs,
+    }
+
+
+def measure_driver(sf, chunks: list, repeats: int = 2) -> dict:
+    """The series driver (sf_hip_driver_*) over the compressed chunks from
+    host memory: submit by number, decode, threshold and group in order,
+    finish with the summary; host clock over the whole run, which includes
+    the chunks' copies to the device (PCIe)."""
+    times, lines, spots = [], [], 0
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        d = sf.Driver(16, HEIGHT, WIDTH, chunk=len(chunks))
+        for z, c in enumerate(chunks):
+            d.submit(z, c)
+        d.finish()
+        times.append(time.perf_counter() - t0)
+        lines = d.log().splitlines()
+        spots = len(d.results()[0])
+        d.close()
+    t = min(times)
+    return {
+        "workload": f"{len(chunks)} bitshuffle+LZ4 chunks of {HEIGHT} x "
+                    f"{WIDTH} u16 in host memory through the driver (copy "
+                    "to the device, decode, threshold, group, summary)",
+        "s_per_run": round(t, 4),
+        "frames_per_s": round(len(chunks) / t, 1),
+        "spots": spots,
+        "summary": lines[1:3],
     }
 
 
@@ -534,11 +562,13 @@ def main() -> int:
         "written_bytes_per_frame": 2 * masked,
     }
     gap_mask.close()
-    decode = None
+    decode = driver = None
     if args.decode_steps > 0:
-        decode = measure_decode(sf, series, chunk, frame_bytes,
-                                args.decode_steps)
+        chunks, decode = measure_decode(sf, series, chunk, frame_bytes,
+                                        args.decode_steps)
         log(f"chunk decode: {decode}")
+        driver = measure_driver(sf, chunks)
+        log(f"driver: {driver}")
     kernel_s = float(np.median(scout_ms)) / 1e3
     threshold_s = float(np.median(np.add(scout_ms, rest_ms))) / 1e3
     mean_signal = float(np.mean(np.concatenate(counts)))
@@ -653,6 +683,7 @@ def main() -> int:
         },
         "pixel_mask": stamp,
         "chunk_decode": decode,
+        "driver": driver,
         "plain_form_fps": round(plain_fps, 2),
         "speedup_vs_plain": round(fps / plain_fps, 3),
         "mean_signal_pixels_per_frame": round(mean_signal, 1),
