@@ -391,15 +391,152 @@ __global__ __launch_bounds__(64) void zero_status(int32_t *status, int n) {
     status[i] = 0;
 }
 
+// Sequences recorded a block by the parse pass; a block with more is
+// parsed again by its wave in LDS.
+constexpr int kMaxSeq = 256;
+
+// One LZ4 sequence: where its literals start in the block's LZ4 bytes, how
+// many, the match offset and length (0: the last sequence, literals only).
+__device__ __forceinline__ uint64_t pack_seq(int at, int lit, int off,
+                                             int len) {
+  return (uint64_t)(uint32_t)at | ((uint64_t)(uint32_t)lit << 16) |
+         ((uint64_t)(uint32_t)off << 32) | ((uint64_t)(uint32_t)len << 48);
+}
+
+// The parse pass: a thread a bitshuffle block walks its LZ4 sequences on
+// the vector unit, with the checks of lds_lz4, and records them. n_seq[b]
+// is the count, -1 for a block that does not decode (its chunk's status
+// -91), -2 for one with more than kMaxSeq sequences. Walking 64 blocks at
+// once spreads the parse's per-sequence work over a wave's lanes, where a
+// wave per block spends it on the scalar unit. One wave a workgroup, so
+// that the few waves spread evenly over the CUs.
+__global__ __launch_bounds__(64) void parse_kernel(
+    const Block *blocks, uint32_t n_blocks, const uint8_t *const *chunks,
+    uint64_t *seqs, int32_t *n_seq, int32_t *status) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_blocks)
+    return;
+  const Block blk = blocks[b];
+  const uint8_t *src = chunks[blk.chunk] + blk.src;
+  const int n = (int)blk.comp, cap = (int)blk.bytes;
+  uint64_t *out = seqs + (size_t)b * kMaxSeq;
+  int i = 0, o = 0, k = 0, result = -1;
+  while (i < n) {
+    const uint32_t token = src[i];
+    int p = i + 1;
+    int lit = token >> 4;
+    bool bad = false;
+    if (lit == 15) {
+      uint32_t v = 255;
+      while (v == 255) {
+        if (p >= n) {
+          bad = true;
+          break;
+        }
+        v = src[p++];
+        lit += v;
+      }
+    }
+    if (bad)
+      break;
+    if (lit + kMfLimit > cap - o || lit + 8 > n - p) {
+      if (lit == n - p && lit <= cap - o) {
+        if (k < kMaxSeq)
+          out[k] = pack_seq(p, lit, 0, 0);
+        k++;
+        result = o + lit;
+      }
+      break;
+    }
+    const int at = p;
+    p += lit;
+    o += lit;
+    const int off = (int)src[p] | ((int)src[p + 1] << 8);
+    p += 2;
+    int len = token & 15;
+    if (len == 15) {
+      uint32_t v = 255;
+      while (v == 255) {
+        if (p >= n) {
+          bad = true;
+          break;
+        }
+        v = src[p++];
+        len += v;
+      }
+    }
+    len += kMinMatch;
+    if (bad || off == 0 || off > o || len + kLastLiterals > cap - o)
+      break;
+    if (k < kMaxSeq)
+      out[k] = pack_seq(at, lit, off, len);
+    k++;
+    o += len;
+    i = p;
+  }
+  if (result != cap) {
+    n_seq[b] = -1;
+    status[blk.chunk] = -91;
+  } else {
+    n_seq[b] = k <= kMaxSeq ? k : -2;
+  }
+}
+
+// The copies of one recorded sequence list into LDS: literals from the
+// staged LZ4 bytes, matches from the bytes before them.
+__device__ __forceinline__ void replay_seq(uint64_t e, const uint8_t *src,
+                                           uint8_t *dst, int &o,
+                                           uint8_t *trash, int lane) {
+  const int at = (int)(e & 0xFFFF), lit = (int)((e >> 16) & 0xFFFF);
+  const int off = (int)((e >> 32) & 0xFFFF), len = (int)(e >> 48);
+  if (lit <= kWave) {
+    const uint8_t v = src[at + lane];
+    *(lane < lit ? dst + o + lane : trash) = v;
+  } else {
+    lds_copy(dst + o, src + at, lit, lane, trash);
+  }
+  o += lit;
+  if (len == 0)
+    return;
+  wave_sync(); // the match may read bytes the lanes have just written
+  uint8_t *to = dst + o;
+  if (off >= len) {
+    if (len <= kWave) {
+      const uint8_t v = to[lane - off];
+      *(lane < len ? to + lane : trash) = v;
+    } else {
+      lds_copy(to, to - off, len, lane, trash);
+    }
+  } else if (off == 1) {
+    const uint8_t v = to[-1];
+    for (int base = 0; base < len; base += kWave) {
+      const int k = base + lane;
+      *(k < len ? to + k : trash) = v;
+    }
+  } else {
+    const float inv = 1.0f / (float)off;
+    for (int base = 0; base < len; base += kWave) {
+      const int k = base + lane;
+      const uint8_t v = to[mod_small(k, off, inv) - off];
+      *(k < len ? to + k : trash) = v;
+    }
+  }
+  o += len;
+}
+
 // One bitshuffle block a workgroup of one wave. LDS: `stage` bytes for the
 // LZ4 bytes (from the 16-byte boundary below them, with 64 bytes to spare
 // for the window reads), then the bit planes and 64 bytes for idle lanes.
 template <int ES>
 __global__ __launch_bounds__(64) void shuffled_kernel(
     const Block *blocks, const uint8_t *const *chunks, uint8_t *const *frames,
-    uint32_t stage, int32_t *status) {
+    uint32_t stage, const uint64_t *seqs, const int32_t *n_seq,
+    int32_t *status) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
+  const int ns = n_seq[blockIdx.x];
+  if (ns == -1)
+    return; // the parse pass found it does not decode
   const Block b = blocks[blockIdx.x];
   const uint8_t *src = chunks[b.chunk] + b.src;
   // 16-byte loads from the boundary below src: each holds a byte of the
@@ -412,12 +549,34 @@ __global__ __launch_bounds__(64) void shuffled_kernel(
         *reinterpret_cast<const uint4 *>(base + k);
   __syncthreads();
   uint8_t *planes = lds + stage;
-  const int got = lds_lz4(lds + head, (int)b.comp, planes, (int)b.bytes,
-                          lane);
-  if (got != (int)b.bytes) {
-    if (lane == 0)
-      status[b.chunk] = -91;
-    return;
+  if (ns >= 0) {
+    // the recorded sequences, four a lane, replayed in order
+    const uint64_t *mine = seqs + (size_t)blockIdx.x * kMaxSeq;
+    uint64_t t[kMaxSeq / kWave];
+#pragma unroll
+    for (int q = 0; q < kMaxSeq / kWave; q++)
+      t[q] = q * kWave + lane < ns ? mine[q * kWave + lane] : 0;
+    uint8_t *trash = planes + b.bytes + lane;
+    int o = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxSeq / kWave; q++) {
+      const int end = ns - q * kWave < kWave ? ns - q * kWave : kWave;
+      for (int l = 0; l < end; l++) {
+        const uint64_t e =
+            (uint64_t)__builtin_amdgcn_readlane((uint32_t)t[q], l) |
+            ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(t[q] >> 32), l)
+             << 32);
+        replay_seq(e, lds + head, planes, o, trash, lane);
+      }
+    }
+  } else {
+    const int got = lds_lz4(lds + head, (int)b.comp, planes, (int)b.bytes,
+                            lane);
+    if (got != (int)b.bytes) {
+      if (lane == 0)
+        status[b.chunk] = -91;
+      return;
+    }
   }
   __syncthreads();
   uint8_t *out = frames[b.chunk] + b.dst;
@@ -570,6 +729,10 @@ struct sf_decoder {
   void **d_ptrs = nullptr;
   bool ptrs_valid = false;
   int32_t *h_status = nullptr, *d_status = nullptr;
+  // the parse pass's sequence lists (kMaxSeq a block) and counts
+  uint64_t *d_seqs = nullptr;
+  int32_t *d_n_seq = nullptr;
+  size_t seq_blocks = 0;
   uint8_t *d_stage = nullptr;
   size_t stage_cap = 0;
   // HIP events around the decode launches, when timing is on
@@ -766,6 +929,24 @@ extern "C" int sf_hip_decoder_index(sf_decoder *d,
       return device_fail("sf_hip_decoder_index", e);
     d->block_cap = cap;
   }
+  if (d->count[kShuffled] > d->seq_blocks) {
+    if (d->d_seqs)
+      (void)hipFree(d->d_seqs);
+    if (d->d_n_seq)
+      (void)hipFree(d->d_n_seq);
+    d->d_seqs = nullptr;
+    d->d_n_seq = nullptr;
+    d->seq_blocks = 0;
+    const size_t blocks = d->count[kShuffled];
+    e = hipMalloc(reinterpret_cast<void **>(&d->d_seqs),
+                  blocks * kMaxSeq * sizeof(uint64_t));
+    if (e == hipSuccess)
+      e = hipMalloc(reinterpret_cast<void **>(&d->d_n_seq),
+                    blocks * sizeof(int32_t));
+    if (e != hipSuccess)
+      return device_fail("sf_hip_decoder_index", e);
+    d->seq_blocks = blocks;
+  }
   for (int k = 0; k < 3; k++)
     std::copy(d->kinds[k].begin(), d->kinds[k].end(),
               d->h_blocks + d->first[k]);
@@ -833,22 +1014,26 @@ extern "C" int sf_hip_decoder_decode(sf_decoder *d,
     const uint32_t lds = stage + (d->max_bytes + 15) / 16 * 16 + 64;
     const Block *b = d->d_blocks + d->first[kShuffled];
     const dim3 grid(d->count[kShuffled]);
+    parse_kernel<<<(d->count[kShuffled] + 63) / 64, 64, 0, d->stream>>>(
+        b, d->count[kShuffled], chunks, d->d_seqs, d->d_n_seq, d->d_status);
+    const uint64_t *seqs = d->d_seqs;
+    const int32_t *n_seq = d->d_n_seq;
     switch (d->element) {
     case 1:
-      shuffled_kernel<1><<<grid, 64, lds, d->stream>>>(b, chunks, frames,
-                                                       stage, d->d_status);
+      shuffled_kernel<1><<<grid, 64, lds, d->stream>>>(
+          b, chunks, frames, stage, seqs, n_seq, d->d_status);
       break;
     case 2:
-      shuffled_kernel<2><<<grid, 64, lds, d->stream>>>(b, chunks, frames,
-                                                       stage, d->d_status);
+      shuffled_kernel<2><<<grid, 64, lds, d->stream>>>(
+          b, chunks, frames, stage, seqs, n_seq, d->d_status);
       break;
     case 4:
-      shuffled_kernel<4><<<grid, 64, lds, d->stream>>>(b, chunks, frames,
-                                                       stage, d->d_status);
+      shuffled_kernel<4><<<grid, 64, lds, d->stream>>>(
+          b, chunks, frames, stage, seqs, n_seq, d->d_status);
       break;
     default:
-      shuffled_kernel<8><<<grid, 64, lds, d->stream>>>(b, chunks, frames,
-                                                       stage, d->d_status);
+      shuffled_kernel<8><<<grid, 64, lds, d->stream>>>(
+          b, chunks, frames, stage, seqs, n_seq, d->d_status);
       break;
     }
   }
@@ -961,6 +1146,10 @@ extern "C" void sf_hip_decoder_destroy(sf_decoder *d) {
     (void)hipFree(d->d_status);
   if (d->d_stage)
     (void)hipFree(d->d_stage);
+  if (d->d_seqs)
+    (void)hipFree(d->d_seqs);
+  if (d->d_n_seq)
+    (void)hipFree(d->d_n_seq);
   for (hipEvent_t ev : d->events)
     if (ev)
       (void)hipEventDestroy(ev);

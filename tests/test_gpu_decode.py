@@ -140,6 +140,41 @@ def test_liblz4_blocks(sf, level):
     check(sf, chunks, "bslz4", 16, h, w)
 
 
+def many_sequence_block(n_seq: int, cap: int, rng) -> bytes:
+    """An LZ4 block of exactly n_seq sequences decoding to cap bytes:
+    n_seq - 1 of one literal and a 4-byte match at offset 1, then the last
+    literals (at least 12 bytes, as the format asks)."""
+    out = bytearray()
+    for _ in range(n_seq - 1):
+        out += bytes([0x10, int(rng.integers(256)), 0x01, 0x00])
+    rest = cap - 5 * (n_seq - 1)
+    assert rest >= 12
+    out.append(0xF0)
+    v = rest - 15
+    while v >= 255:
+        out.append(255)
+        v -= 255
+    out.append(v)
+    out += rng.integers(0, 256, rest, np.uint8).tobytes()
+    return bytes(out)
+
+
+@pytest.mark.parametrize("n_seq", [1, 255, 256, 257, 1500])
+def test_blocks_with_many_sequences(sf, n_seq):
+    """Blocks around the parse pass's 256 recorded sequences (more are
+    parsed again in LDS), in a chunk of two blocks and a raw tail."""
+    rng = np.random.default_rng(n_seq)
+    h, w = 1, 8195  # two 4096-element blocks and 3 raw elements
+    blocks = [many_sequence_block(n_seq, 8192, rng),
+              many_sequence_block(max(1, n_seq // 2), 8192, rng)]
+    assert cpu.lz4_block(blocks[0], 8192) is not None
+    chunk = bytearray((h * w * 2).to_bytes(8, "big") + (8192).to_bytes(4, "big"))
+    for b in blocks:
+        chunk += len(b).to_bytes(4, "big") + b
+    chunk += rng.integers(0, 256, 6, np.uint8).tobytes()
+    check(sf, [bytes(chunk)] * 2, "bslz4", 16, h, w)
+
+
 def test_lz4_and_stored_chunks(sf):
     rng = np.random.default_rng(6)
     h, w = 30, 70
