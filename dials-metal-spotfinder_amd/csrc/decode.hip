@@ -4,14 +4,15 @@
 // The host walks each chunk's block lengths (a dependent chain of 4-byte
 // reads that is cheap on the host and serial on a device) and writes a block
 // table: where each block's LZ4 bytes sit in its chunk and where its decoded
-// bytes go in its frame. The device then gives each bitshuffle block one
-// wave: the wave copies the block's LZ4 bytes into LDS with 16-byte loads,
-// decodes the LZ4 sequences in LDS (every lane reads the same token, so the
-// parse is wave-uniform; literal and match bytes are copied 64 lanes at a
+// bytes go in its frame. The device decodes in two passes. The parse pass
+// gives each bitshuffle block a thread, which walks its LZ4 sequences and
+// records them (a wave per block would spend that walk on the scalar unit,
+// which then bounds the decode). The copy pass gives each block a wave: it
+// copies the block's LZ4 bytes into LDS with 16-byte loads, replays the
+// recorded sequences there (literal and match bytes copied 64 lanes at a
 // time), and writes the frame's elements from the bit planes, each lane 8
-// elements at a time by 8x8 bit transposes in registers. A block's bytes
-// touch HBM twice: its compressed bytes read once, its elements written
-// once.
+// elements at a time by 8x8 bit transposes in registers. The bit planes
+// never reach HBM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

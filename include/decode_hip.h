@@ -7,9 +7,10 @@
  * and the blocks, each a 4-byte big-endian length and an LZ4 block; for ""
  * the frame's bytes as they are. The host walks the block lengths
  * (sf_hip_decoder_index) and validates each chunk as decompress::image does,
- * with its messages; the device then decodes every block of the batch in one
- * launch (sf_hip_decoder_decode): a wave per bitshuffle block decodes the
- * LZ4 stream in LDS and writes the unshuffled elements to the frame.
+ * with its messages; the device then decodes every block of the batch
+ * (sf_hip_decoder_decode): a pass that records each block's LZ4 sequences,
+ * then a wave per bitshuffle block that replays them in LDS and writes the
+ * unshuffled elements to the frame.
  *
  * Where the reference reads past the end of a corrupt chunk (bitshuffle
  * trusts the block lengths it reads), the decoder reports the bitshuffle
