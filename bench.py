@@ -54,7 +54,8 @@ def log(*args):
     print(*args, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sf, host_frames, device_lists, threads, seconds):
+def cpu_baseline(sf, host_frames, device_lists, threads, seconds,
+                 chunks=None):
     """The reference CPU path (or the oracle port) on a bounded sample: the
     threshold over the sample frames on `threads` host threads, repeated
     until about `seconds` have passed, and the reference's streaming
@@ -91,6 +92,18 @@ def cpu_baseline(sf, host_frames, device_lists, threads, seconds):
             diffs += 1
     value = threshold_fps if label_fps is None else min(threshold_fps,
                                                         label_fps)
+    # the oracle's decompress::image on two of the sample's bitshuffle+LZ4
+    # chunks, one thread (a plain restatement: bitshuffle's SIMD library is
+    # not here), checked against the frames
+    decode_ms, decode_diffs = None, 0
+    if chunks:
+        t0 = time.perf_counter()
+        for f, chunk in enumerate(chunks[:2]):
+            out, message = cpu.decompress_image(chunk, "bslz4", 16, HEIGHT,
+                                                WIDTH)
+            if message or out != host_frames[f].tobytes():
+                decode_diffs += 1
+        decode_ms = (time.perf_counter() - t0) / len(chunks[:2]) * 1e3
     return {
         "value": round(value, 3),
         "unit": "frames/s",
@@ -104,6 +117,9 @@ def cpu_baseline(sf, host_frames, device_lists, threads, seconds):
         "labeller_fps": None if label_fps is None else round(label_fps, 1),
         "signal_pixels_per_pass": int(total // passes),
         "parity_frames": min(n, 8),
+        "decode_port_ms_per_frame": None if decode_ms is None
+        else round(decode_ms, 1),
+        "decode_port_diffs": decode_diffs,
     }, diffs
 
 
@@ -562,12 +578,12 @@ def main() -> int:
         "written_bytes_per_frame": 2 * masked,
     }
     gap_mask.close()
-    decode = driver = None
+    decode = driver = decode_chunks = None
     if args.decode_steps > 0:
-        chunks, decode = measure_decode(sf, series, chunk, frame_bytes,
-                                        args.decode_steps)
+        decode_chunks, decode = measure_decode(sf, series, chunk,
+                                               frame_bytes, args.decode_steps)
         log(f"chunk decode: {decode}")
-        driver = measure_driver(sf, chunks)
+        driver = measure_driver(sf, decode_chunks)
         log(f"driver: {driver}")
     kernel_s = float(np.median(scout_ms)) / 1e3
     threshold_s = float(np.median(np.add(scout_ms, rest_ms))) / 1e3
@@ -607,7 +623,7 @@ def main() -> int:
     series.download(host)
     threads = min(os.cpu_count() or 1, 16)
     baseline, diffs = cpu_baseline(sf, host, device_lists, threads,
-                                   args.cpu_seconds)
+                                   args.cpu_seconds, decode_chunks)
 
     others = {}
     if args.other_steps > 0:
@@ -696,7 +712,8 @@ def main() -> int:
     print(json.dumps(record))
     plan.close()
     series.close()
-    decoded_ok = decode is None or decode["frame_diffs"] == 0
+    decoded_ok = (decode is None or decode["frame_diffs"] == 0) and \
+        baseline["decode_port_diffs"] == 0
     return 0 if diffs == 0 and decoded_ok else 3
 
 

@@ -396,9 +396,6 @@ __global__ __launch_bounds__(64) void zero_status(int32_t *status, int n) {
 // parsed again by its wave in LDS.
 constexpr int kMaxSeq = 256;
 
-// Parts of a batch whose parse and copy passes overlap.
-constexpr uint32_t kParts = 4;
-
 // One LZ4 sequence: where its literals start in the block's LZ4 bytes, how
 // many, the match offset and length (0: the last sequence, literals only).
 __device__ __forceinline__ uint64_t pack_seq(int at, int lit, int off,
@@ -717,10 +714,6 @@ void put_be(uint8_t *p, uint64_t v, int bytes) {
 struct sf_decoder {
   int device = 0;
   hipStream_t stream = nullptr;
-  // the copy pass of part k runs here while part k + 1 is parsed
-  hipStream_t side = nullptr;
-  hipEvent_t parsed[kParts] = {};
-  hipEvent_t copied = nullptr;
   size_t element = 0, expected = 0;
   int max_frames = 0;
   // the indexed batch
@@ -876,12 +869,6 @@ extern "C" int sf_hip_decoder_create(unsigned bit_depth, size_t height,
                   max_frames * sizeof(int32_t));
   for (int k = 0; k < 2 && e == hipSuccess; k++)
     e = hipEventCreate(&d->events[k]);
-  if (e == hipSuccess)
-    e = hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking);
-  for (int k = 0; k < kParts && e == hipSuccess; k++)
-    e = hipEventCreateWithFlags(&d->parsed[k], hipEventDisableTiming);
-  if (e == hipSuccess)
-    e = hipEventCreateWithFlags(&d->copied, hipEventDisableTiming);
   if (e != hipSuccess) {
     sf_hip_decoder_destroy(d);
     return device_fail("sf_hip_decoder_create", e);
@@ -1026,44 +1013,30 @@ extern "C" int sf_hip_decoder_decode(sf_decoder *d,
     // window reads may touch; the planes and 64 bytes for idle lanes
     const uint32_t stage = (d->max_comp + 15 + 64 + 15) / 16 * 16;
     const uint32_t lds = stage + (d->max_bytes + 15) / 16 * 16 + 64;
-    // kParts parts: the copy pass of a part (on the side stream) overlaps
-    // the parse pass of the next (on the decoder's stream), which is bound
-    // by the latency of its loads and leaves the CUs room.
-    const uint32_t total = d->count[kShuffled];
-    const uint32_t parts = total >= kParts * 1024u ? kParts : 1;
-    for (uint32_t k = 0; k < parts; k++) {
-      const uint32_t from = (uint32_t)((uint64_t)total * k / parts);
-      const uint32_t to = (uint32_t)((uint64_t)total * (k + 1) / parts);
-      const uint32_t count = to - from;
-      const Block *b = d->d_blocks + d->first[kShuffled] + from;
-      uint64_t *seqs = d->d_seqs + (size_t)from * kMaxSeq;
-      int32_t *n_seq = d->d_n_seq + from;
-      parse_kernel<<<(count + 63) / 64, 64, 0, d->stream>>>(
-          b, count, chunks, seqs, n_seq, d->d_status);
-      (void)hipEventRecord(d->parsed[k], d->stream);
-      (void)hipStreamWaitEvent(d->side, d->parsed[k], 0);
-      const dim3 grid(count);
-      switch (d->element) {
-      case 1:
-        shuffled_kernel<1><<<grid, 64, lds, d->side>>>(
-            b, chunks, frames, stage, seqs, n_seq, d->d_status);
-        break;
-      case 2:
-        shuffled_kernel<2><<<grid, 64, lds, d->side>>>(
-            b, chunks, frames, stage, seqs, n_seq, d->d_status);
-        break;
-      case 4:
-        shuffled_kernel<4><<<grid, 64, lds, d->side>>>(
-            b, chunks, frames, stage, seqs, n_seq, d->d_status);
-        break;
-      default:
-        shuffled_kernel<8><<<grid, 64, lds, d->side>>>(
-            b, chunks, frames, stage, seqs, n_seq, d->d_status);
-        break;
-      }
+    const Block *b = d->d_blocks + d->first[kShuffled];
+    const dim3 grid(d->count[kShuffled]);
+    parse_kernel<<<(d->count[kShuffled] + 63) / 64, 64, 0, d->stream>>>(
+        b, d->count[kShuffled], chunks, d->d_seqs, d->d_n_seq, d->d_status);
+    const uint64_t *seqs = d->d_seqs;
+    const int32_t *n_seq = d->d_n_seq;
+    switch (d->element) {
+    case 1:
+      shuffled_kernel<1><<<grid, 64, lds, d->stream>>>(
+          b, chunks, frames, stage, seqs, n_seq, d->d_status);
+      break;
+    case 2:
+      shuffled_kernel<2><<<grid, 64, lds, d->stream>>>(
+          b, chunks, frames, stage, seqs, n_seq, d->d_status);
+      break;
+    case 4:
+      shuffled_kernel<4><<<grid, 64, lds, d->stream>>>(
+          b, chunks, frames, stage, seqs, n_seq, d->d_status);
+      break;
+    default:
+      shuffled_kernel<8><<<grid, 64, lds, d->stream>>>(
+          b, chunks, frames, stage, seqs, n_seq, d->d_status);
+      break;
     }
-    (void)hipEventRecord(d->copied, d->side);
-    (void)hipStreamWaitEvent(d->stream, d->copied, 0);
   }
   if (d->count[kLz4])
     lz4_kernel<<<d->count[kLz4], 64, 0, d->stream>>>(
@@ -1181,15 +1154,6 @@ extern "C" void sf_hip_decoder_destroy(sf_decoder *d) {
   for (hipEvent_t ev : d->events)
     if (ev)
       (void)hipEventDestroy(ev);
-  for (hipEvent_t ev : d->parsed)
-    if (ev)
-      (void)hipEventDestroy(ev);
-  if (d->copied)
-    (void)hipEventDestroy(d->copied);
-  if (d->side) {
-    (void)hipStreamSynchronize(d->side);
-    (void)hipStreamDestroy(d->side);
-  }
   if (d->stream)
     (void)hipStreamDestroy(d->stream);
   delete d;
