@@ -321,14 +321,17 @@ int run_batch(sf_driver *d, size_t n) {
     return status;
   };
   for (auto &f : decoded) {
+    // not above the last grouped frame, or the last of the run being built
+    // (a number that came twice)
     const int64_t z = f.first + d->z_offset;
-    if (d->any && z <= d->highest) {
+    const bool late = !numbers.empty() ? z <= numbers.back() + d->z_offset
+                                       : d->any && z <= d->highest;
+    if (late) {
       d->stats.read++;
       d->stats.out_of_order++;
       continue;
     }
-    if (!numbers.empty() &&
-        (f.first != numbers.back() + 1 || (int)slots.size() >= d->chunk)) {
+    if (!numbers.empty() && f.first != numbers.back() + 1) {
       const int status = flush();
       if (status != SF_OK)
         return status;

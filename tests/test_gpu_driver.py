@@ -154,6 +154,21 @@ def test_late_frame_is_dropped(sf, fixture_chunks):
     d.close()
 
 
+def test_a_number_submitted_twice(sf, fixture_chunks):
+    """A frame number that arrives twice in one chunk is grouped once; the
+    second is dropped as late."""
+    stack, chunks = fixture_chunks
+    d = sf.Driver(16, H, W, chunk=6)
+    for z in range(12):
+        d.submit(z, chunks[z])
+        if z == 4:
+            d.submit(4, chunks[4])
+    d.finish()
+    assert d.stats()["out_of_order"] == 1 and d.stats()["failed"] == 0
+    assert same_tables(d.results(), oracle_tables(stack, range(12)))
+    d.close()
+
+
 def test_failures_are_logged_and_leave_a_gap(sf, fixture_chunks):
     """src/find_spots.cc:528-534: a frame that does not decode is logged
     with the reference's message and counted; the rest are grouped."""
