@@ -265,3 +265,17 @@ def test_messages_follow_the_reference():
     bad[17] = 200  # a literal run longer than the block
     assert dec(bytes(bad), "lz4", 16, 2, 4)[1].startswith(
         "lz4 decompression failed with code ")
+
+
+def test_encoder_arguments():
+    """The host encoder refuses a block size that is not a multiple of 8
+    and writes the default block size into the header otherwise."""
+    x = np.arange(100, dtype=np.uint16)
+    with pytest.raises(ValueError, match="multiple of 8"):
+        sf.bslz4_compress(x, 12)
+    chunk = sf.bslz4_compress(x, 0)
+    assert int.from_bytes(chunk[8:12], "big") == 8192
+    assert sf.lib().sf_bslz4_bound(100, 2, 0) >= len(chunk)
+    empty = sf.bslz4_compress(np.zeros(0, np.uint16))
+    assert len(empty) == 12 and int.from_bytes(empty[:8], "big") == 0
+    assert cpu.decompress_image(empty, "bslz4", 16, 0, 0)[1] == ""
