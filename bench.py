@@ -55,7 +55,7 @@ def log(*args):
 
 
 def cpu_baseline(sf, host_frames, device_lists, threads, seconds,
-                 chunks=None):
+                 chunks=None, refl=None):
     """The reference CPU path (or the oracle port) on a bounded sample: the
     threshold over the sample frames on `threads` host threads, repeated
     until about `seconds` have passed, and the reference's streaming
@@ -105,6 +105,16 @@ def cpu_baseline(sf, host_frames, device_lists, threads, seconds,
             if message or out != host_frames[f].tobytes():
                 decode_diffs += 1
         decode_ms = (time.perf_counter() - t0) / len(sample) * 1e3
+    # the reference's own refl::write on the driver's tables: the same bytes
+    # as the library's writer, and its time beside it
+    ref_write_ms, refl_same = None, None
+    if refl is not None and kind == "reference":
+        other = refl["path"] + ".reference"
+        t0 = time.perf_counter()
+        cpu.ref_refl_write(other, refl["spots"], refl["pixels"], WIDTH)
+        ref_write_ms = (time.perf_counter() - t0) * 1e3
+        with open(other, "rb") as a, open(refl["path"], "rb") as b:
+            refl_same = a.read() == b.read()
     return {
         "value": round(value, 3),
         "unit": "frames/s",
@@ -121,6 +131,9 @@ def cpu_baseline(sf, host_frames, device_lists, threads, seconds,
         "decode_port_ms_per_frame": None if decode_ms is None
         else round(decode_ms, 1),
         "decode_port_diffs": decode_diffs,
+        "refl_reference_write_ms": None if ref_write_ms is None
+        else round(ref_write_ms, 1),
+        "refl_identical": refl_same,
     }, diffs
 
 
@@ -381,13 +394,16 @@ def measure_decode(sf, series, chunk: int, frame_bytes: int,
     }
 
 
-def measure_driver(sf, chunks: list, repeats: int = 2) -> dict:
+def measure_driver(sf, chunks: list, repeats: int = 2):
     """The series driver (sf_hip_driver_*) over the compressed chunks from
     host memory: submit by number, decode, threshold and group in order,
     finish with the summary; host clock over the whole run, which includes
     the chunks' copies to the device (PCIe)."""
-    times, lines, spots = [], [], 0
-    for _ in range(repeats):
+    import tempfile
+
+    times, lines, spots, tables, write_s = [], [], 0, None, None
+    path = os.path.join(tempfile.mkdtemp(), "strong.refl")
+    for r in range(repeats):
         t0 = time.perf_counter()
         d = sf.Driver(16, HEIGHT, WIDTH, chunk=len(chunks))
         for z, c in enumerate(chunks):
@@ -395,10 +411,18 @@ def measure_driver(sf, chunks: list, repeats: int = 2) -> dict:
         d.finish()
         times.append(time.perf_counter() - t0)
         lines = d.log().splitlines()
-        spots = len(d.results()[0])
+        tables = d.results()
+        spots = len(tables[0])
+        if r == repeats - 1:  # the .refl writer on the chunk's tables
+            t0 = time.perf_counter()
+            d.write(path)
+            write_s = time.perf_counter() - t0
         d.close()
     t = min(times)
-    return {
+    refl = {"path": path, "spots": tables[0], "pixels": tables[1],
+            "write_ms": round(write_s * 1e3, 1),
+            "bytes": os.path.getsize(path)}
+    return refl, {
         "workload": f"{len(chunks)} bitshuffle+LZ4 chunks of {HEIGHT} x "
                     f"{WIDTH} u16 in host memory through the driver (copy "
                     "to the device, decode, threshold, group, summary)",
@@ -406,6 +430,8 @@ def measure_driver(sf, chunks: list, repeats: int = 2) -> dict:
         "frames_per_s": round(len(chunks) / t, 1),
         "spots": spots,
         "summary": lines[1:3],
+        "refl_write_ms": refl["write_ms"],
+        "refl_bytes": refl["bytes"],
     }
 
 
@@ -579,12 +605,12 @@ def main() -> int:
         "written_bytes_per_frame": 2 * masked,
     }
     gap_mask.close()
-    decode = driver = decode_chunks = None
+    decode = driver = decode_chunks = refl = None
     if args.decode_steps > 0:
         decode_chunks, decode = measure_decode(sf, series, chunk,
                                                frame_bytes, args.decode_steps)
         log(f"chunk decode: {decode}")
-        driver = measure_driver(sf, decode_chunks)
+        refl, driver = measure_driver(sf, decode_chunks)
         log(f"driver: {driver}")
     kernel_s = float(np.median(scout_ms)) / 1e3
     threshold_s = float(np.median(np.add(scout_ms, rest_ms))) / 1e3
@@ -624,7 +650,7 @@ def main() -> int:
     series.download(host)
     threads = min(os.cpu_count() or 1, 16)
     baseline, diffs = cpu_baseline(sf, host, device_lists, threads,
-                                   args.cpu_seconds, decode_chunks)
+                                   args.cpu_seconds, decode_chunks, refl)
 
     others = {}
     if args.other_steps > 0:
@@ -714,7 +740,8 @@ def main() -> int:
     plan.close()
     series.close()
     decoded_ok = (decode is None or decode["frame_diffs"] == 0) and \
-        baseline["decode_port_diffs"] == 0
+        baseline["decode_port_diffs"] == 0 and \
+        baseline["refl_identical"] is not False
     return 0 if diffs == 0 and decoded_ok else 3
 
 
