@@ -190,3 +190,34 @@ def test_failures_are_logged_and_leave_a_gap(sf, fixture_chunks):
     want = oracle_tables(stack, [z for z in range(12) if z not in (3, 7)])
     assert same_tables(d.results(), want)
     d.close()
+
+
+def test_u32_frames_in_every_algorithm(sf):
+    """32-bit frames whose chunks come in all three forms in one batch
+    (bitshuffle+LZ4, plain LZ4, stored: nxmx.cc gives a chunk whose filter
+    was skipped the empty algorithm), against the oracle labeller."""
+    from test_decode import lz4_chunk
+
+    base = fr.u32_frame()
+    h, w = base.shape
+    frames = [np.roll(base, 9 * z, axis=1) for z in range(6)]
+    forms = ["bslz4", "lz4", "", "bslz4", "", "lz4"]
+    d = sf.Driver(32, h, w, chunk=4)
+    for z, (x, form) in enumerate(zip(frames, forms)):
+        if form == "bslz4":
+            data = sf.bslz4_compress(x)
+        elif form == "lz4":
+            data = lz4_chunk(x.tobytes(), 4096)
+        else:
+            data = x.tobytes()
+        d.submit(z, data, form)
+    d.finish()
+    assert d.stats()["failed"] == 0 and d.stats()["grouped"] == 6
+    lab = cpu.Labeller(h, w)
+    for z, x in enumerate(frames):
+        assert lab.add(z, cpu.dext(x)[1]) == 0
+    lab.finish()
+    want = lab.results()
+    assert len(want[0]) > 0
+    assert same_tables(d.results(), want)
+    d.close()
