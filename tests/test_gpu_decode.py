@@ -274,3 +274,16 @@ def test_decoded_fixture_gives_the_published_answer(sf):
     dec.close()
     buf.close()
     frames.close()
+
+
+def test_lz4_whole_frame_blocks(sf):
+    """The HDF5 LZ4 filter's default: the whole frame one LZ4 block, which
+    one wave decodes in global memory (the fixture's 256 x 384 frames,
+    196 KB a block, thousands of sequences)."""
+    stack = fr.nxmx_fixture()[:3]
+    n, h, w = stack.shape
+    chunks = [lz4_chunk(stack[z].tobytes(), h * w * 2, header_block=0)
+              for z in range(n)]
+    got = decode_batch(sf, chunks, "lz4", 16, h, w)
+    assert all(got[z] == stack[z].tobytes() for z in range(n))
+    check(sf, chunks, "lz4", 16, h, w)
