@@ -78,6 +78,7 @@ struct sf_driver {
   uint8_t *d_stage = nullptr;     // the chunks of a batch
   size_t stage_cap = 0;
   std::vector<Pending> waiting;
+  const uint32_t *mask_bits = nullptr; // the detector mask, device bits
   std::string log;
   sf_driver_stats stats{};
   int64_t highest = 0;
@@ -304,6 +305,20 @@ int run_batch(sf_driver *d, size_t n) {
       decoded.push_back({g.second.first[f]->number, g.second.second[f]});
   }
   std::sort(decoded.begin(), decoded.end());
+  if (d->mask_bits && !decoded.empty()) {
+    // the detector mask into the decoded frames, before the threshold
+    std::vector<void *> ptrs;
+    for (auto &f : decoded)
+      ptrs.push_back(d->d_frames + (size_t)f.second * d->frame_bytes);
+    int status = sf_hip_stamp_mask((int)(d->bit_depth / 8), ptrs.data(),
+                                   (int)ptrs.size(), d->height, d->width,
+                                   d->mask_bits, nullptr);
+    if (status == SF_OK)
+      status = sf_hip_synchronize();
+    if (status != SF_OK)
+      return fail(SF_DEVICE_ERROR,
+                  std::string("mask: ") + sf_hip_mask_last_error());
+  }
   if (d->bit_depth != 16 && d->bit_depth != 32) {
     for (auto &f : decoded) // src/find_spots.cc:515-519
       frame_failed(d, f.first,
@@ -408,6 +423,16 @@ extern "C" int sf_hip_driver_submit(sf_driver *d, int64_t number,
   d->stats.dispatched++;
   if (d->waiting.size() >= (size_t)d->chunk)
     return run_batch(d, (size_t)d->chunk);
+  return SF_OK;
+}
+
+extern "C" int sf_hip_driver_set_mask(sf_driver *d, const uint32_t *bits) {
+  if (!d)
+    return fail(SF_BAD_ARGUMENT, "sf_hip_driver_set_mask: bad argument");
+  if (d->bit_depth != 16 && d->bit_depth != 32)
+    return fail(SF_BAD_ARGUMENT,
+                "sf_hip_driver_set_mask: 16- or 32-bit frames only");
+  d->mask_bits = bits;
   return SF_OK;
 }
 

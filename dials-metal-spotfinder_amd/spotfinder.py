@@ -198,6 +198,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_driver_submit", _i, _p, ctypes.c_int64, _p, _sz,
        ctypes.c_char_p)
     fn("sf_hip_driver_finish", _i, _p)
+    fn("sf_hip_driver_set_mask", _i, _p, _p)
     fn("sf_hip_driver_write", _i, _p, ctypes.c_char_p,
        ctypes.POINTER(ReflOptions))
     fn("sf_hip_driver_log", _sz, _p, _p, _sz)
@@ -920,6 +921,14 @@ class Driver:
                 self.handle, number, src.ctypes.data, len(data),
                 algorithm.encode())
         self._status(status, "submit")
+
+    def set_mask(self, mask: "PixelMask | None") -> None:
+        """Stamp this detector mask into every frame before the threshold
+        (the mask must stay open while the driver runs)."""
+        self._mask = mask
+        self._status(lib().sf_hip_driver_set_mask(
+            self.handle, mask.bits.ptr if mask is not None else None),
+            "set_mask")
 
     def finish(self) -> None:
         self._status(lib().sf_hip_driver_finish(self.handle), "finish")

@@ -61,6 +61,12 @@ int sf_hip_driver_submit(sf_driver *driver, int64_t number,
                          const uint8_t *data, size_t size,
                          const char *algorithm);
 
+/* The detector mask (the bit array of sf_hip_stamp_mask, device memory the
+ * caller keeps; NULL: none) stamped into every decoded frame before the
+ * threshold, which the reference leaves out (README.md:227-234). 16- and
+ * 32-bit frames. */
+int sf_hip_driver_set_mask(sf_driver *driver, const uint32_t *bits);
+
 /* Runs what waits, ends the grouping and logs the summary. */
 int sf_hip_driver_finish(sf_driver *driver);
 

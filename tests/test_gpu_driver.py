@@ -221,3 +221,28 @@ def test_u32_frames_in_every_algorithm(sf):
     assert len(want[0]) > 0
     assert same_tables(d.results(), want)
     d.close()
+
+
+def test_detector_mask_before_the_threshold(sf, fixture_chunks):
+    """sf_hip_driver_set_mask: the mask stamped into each decoded frame
+    before the threshold (the step README.md:227-234 says the reference
+    leaves out), against the oracle labeller over the host-stamped
+    frames."""
+    stack, chunks = fixture_chunks
+    rng = np.random.default_rng(21)
+    mask = rng.random((H, W)) < 0.02
+    mask[60:63, :] = True  # rows through planted spots
+    mask[:, 100:104] = True
+    pm = sf.PixelMask(mask)
+    d = sf.Driver(16, H, W, chunk=5)
+    d.set_mask(pm)
+    for z in range(12):
+        d.submit(z, chunks[z])
+    d.finish()
+    stamped = np.where(mask[None], np.uint16(0xFFFF), stack)
+    want = oracle_tables(stamped, range(12))
+    got = d.results()
+    assert same_tables(got, want)
+    assert got[2]["signal_pixels"] < 8928  # the mask took pixels away
+    d.close()
+    pm.close()
