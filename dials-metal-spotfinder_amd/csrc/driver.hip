@@ -310,14 +310,19 @@ int run_batch(sf_driver *d, size_t n) {
     std::vector<void *> ptrs;
     for (auto &f : decoded)
       ptrs.push_back(d->d_frames + (size_t)f.second * d->frame_bytes);
+    // on the decoder's stream, which wrote the frames; waiting for that
+    // stream alone leaves every other stream of the process running
     int status = sf_hip_stamp_mask((int)(d->bit_depth / 8), ptrs.data(),
                                    (int)ptrs.size(), d->height, d->width,
-                                   d->mask_bits, nullptr);
-    if (status == SF_OK)
-      status = sf_hip_synchronize();
+                                   d->mask_bits,
+                                   sf_hip_decoder_stream(d->dec));
     if (status != SF_OK)
       return fail(SF_DEVICE_ERROR,
                   std::string("mask: ") + sf_hip_mask_last_error());
+    status = sf_hip_decoder_wait(d->dec);
+    if (status != SF_OK)
+      return fail(SF_DEVICE_ERROR,
+                  std::string("mask: ") + sf_hip_decode_last_error());
   }
   if (d->bit_depth != 16 && d->bit_depth != 32) {
     for (auto &f : decoded) // src/find_spots.cc:515-519
@@ -426,12 +431,20 @@ extern "C" int sf_hip_driver_submit(sf_driver *d, int64_t number,
   return SF_OK;
 }
 
-extern "C" int sf_hip_driver_set_mask(sf_driver *d, const uint32_t *bits) {
+extern "C" int sf_hip_driver_set_mask(sf_driver *d, const uint32_t *bits,
+                                      size_t height, size_t width) {
   if (!d)
     return fail(SF_BAD_ARGUMENT, "sf_hip_driver_set_mask: bad argument");
   if (d->bit_depth != 16 && d->bit_depth != 32)
     return fail(SF_BAD_ARGUMENT,
                 "sf_hip_driver_set_mask: 16- or 32-bit frames only");
+  // the stamp reads ceil(height * width / 32) words: a mask of another
+  // shape would be read past its end, or mask the wrong pixels
+  if (bits && (height != d->height || width != d->width))
+    return fail(SF_BAD_ARGUMENT,
+                format("sf_hip_driver_set_mask: a %zu x %zu mask for %zu x "
+                       "%zu frames",
+                       height, width, d->height, d->width));
   d->mask_bits = bits;
   return SF_OK;
 }

@@ -85,7 +85,8 @@ class DriverStats(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "batches", "batched_frames", "admission_waits",
-        "max_batches_in_flight", "bytes_in_use", "peak_bytes", "budget")]
+        "max_batches_in_flight", "bytes_in_use", "peak_bytes", "budget",
+        "overflowed")]
 
 
 def _declare(lib: ctypes.CDLL) -> None:
@@ -198,7 +199,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_driver_submit", _i, _p, ctypes.c_int64, _p, _sz,
        ctypes.c_char_p)
     fn("sf_hip_driver_finish", _i, _p)
-    fn("sf_hip_driver_set_mask", _i, _p, _p)
+    fn("sf_hip_driver_set_mask", _i, _p, _p, _sz, _sz)
     fn("sf_hip_driver_write", _i, _p, ctypes.c_char_p,
        ctypes.POINTER(ReflOptions))
     fn("sf_hip_driver_log", _sz, _p, _p, _sz)
@@ -898,6 +899,7 @@ class Driver:
                  two_d: bool = False):
         options = DriverOptions(chunk, z_offset, Options(
             min_spot_size, max_spot_size, max_separation, int(two_d)))
+        self.height, self.width = height, width
         self.handle = lib().sf_hip_driver_create(bit_depth, height, width,
                                                  ctypes.byref(options))
         if not self.handle:
@@ -923,12 +925,22 @@ class Driver:
         self._status(status, "submit")
 
     def set_mask(self, mask: "PixelMask | None") -> None:
-        """Stamp this detector mask into every frame before the threshold
-        (the mask must stay open while the driver runs)."""
+        """Stamp this detector mask into every frame before the threshold.
+        The mask must have the driver's frame shape (ValueError otherwise,
+        before anything is launched) and stay open until finish(): the
+        driver holds a reference to it."""
+        if mask is not None and (mask.height, mask.width) != (self.height,
+                                                                self.width):
+            raise ValueError(
+                f"set_mask: a {mask.height} x {mask.width} mask for "
+                f"{self.height} x {self.width} frames")
+        if mask is None:
+            status = lib().sf_hip_driver_set_mask(self.handle, None, 0, 0)
+        else:
+            status = lib().sf_hip_driver_set_mask(
+                self.handle, mask.bits.ptr, mask.height, mask.width)
+        self._status(status, "set_mask")
         self._mask = mask
-        self._status(lib().sf_hip_driver_set_mask(
-            self.handle, mask.bits.ptr if mask is not None else None),
-            "set_mask")
 
     def finish(self) -> None:
         self._status(lib().sf_hip_driver_finish(self.handle), "finish")

@@ -64,8 +64,12 @@ int sf_hip_driver_submit(sf_driver *driver, int64_t number,
 /* The detector mask (the bit array of sf_hip_stamp_mask, device memory the
  * caller keeps; NULL: none) stamped into every decoded frame before the
  * threshold, which the reference leaves out (README.md:227-234). 16- and
- * 32-bit frames. */
-int sf_hip_driver_set_mask(sf_driver *driver, const uint32_t *bits);
+ * 32-bit frames. height x width is the mask's shape: SF_BAD_ARGUMENT unless
+ * it is the driver's frame shape (nothing is launched then). The driver
+ * keeps the pointer: the mask must stay allocated until
+ * sf_hip_driver_finish() returns or another mask (or NULL) is set. */
+int sf_hip_driver_set_mask(sf_driver *driver, const uint32_t *bits,
+                           size_t height, size_t width);
 
 /* Runs what waits, ends the grouping and logs the summary. */
 int sf_hip_driver_finish(sf_driver *driver);

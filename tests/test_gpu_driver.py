@@ -246,3 +246,27 @@ def test_detector_mask_before_the_threshold(sf, fixture_chunks):
     assert got[2]["signal_pixels"] < 8928  # the mask took pixels away
     d.close()
     pm.close()
+
+
+def test_detector_mask_of_another_shape_is_refused(sf):
+    """A mask whose shape is not the driver's frame shape is refused before
+    anything is launched: by the binding (ValueError) and by the C entry
+    point itself (SF_BAD_ARGUMENT), for a smaller mask (whose bit array
+    the stamp would read past) and for one with the same pixel count but
+    another width (which would mask the wrong pixels)."""
+    d = sf.Driver(16, H, W, chunk=5)
+    for shape in ((H // 2, W), (W, H)):
+        pm = sf.PixelMask(np.zeros(shape, dtype=bool))
+        with pytest.raises(ValueError):
+            d.set_mask(pm)
+        status = sf.lib().sf_hip_driver_set_mask(d.handle, pm.bits.ptr,
+                                                 shape[0], shape[1])
+        assert status == sf.SF_BAD_ARGUMENT, shape
+        assert b"mask for" in sf.lib().sf_hip_driver_last_error()
+        pm.close()
+    # a mask of the right shape, then none, are accepted
+    pm = sf.PixelMask(np.zeros((H, W), dtype=bool))
+    d.set_mask(pm)
+    d.set_mask(None)
+    d.close()
+    pm.close()
