@@ -153,6 +153,7 @@ struct sf_server {
   std::string error;
   // statistics
   uint64_t n_batches = 0, batched_frames = 0, admission_waits = 0;
+  uint64_t overflowed = 0; // frames whose list overflowed (labelled empty)
   int max_inflight = 0;
 };
 
@@ -618,6 +619,10 @@ void series_loop(sf_server *s, Series *se) {
       if (!failed && status == SF_OK) {
         for (int f : job.frames) {
           const Request &r = b->requests[f];
+          // more signal pixels than the slot's list holds (the -2 of
+          // src/dext_gpu.hh:148-151): the labeller took the frame as empty
+          if (b->h_found[f] > b->h_desc[f].capacity)
+            s->overflowed++;
           se->frames.push_back(r.frame);
           se->counts.push_back(b->h_found[f]);
           se->mask_ms.push_back(ms_between(r.arrival, b->masked_at));
@@ -864,6 +869,7 @@ int sf_hip_server_stats(sf_server *s, sf_server_stats *stats) {
   stats->bytes_in_use = s->used;
   stats->peak_bytes = s->peak;
   stats->budget = s->budget;
+  stats->overflowed = s->overflowed;
   return SF_OK;
 }
 

@@ -250,6 +250,11 @@ typedef struct sf_server_stats {
   uint64_t bytes_in_use;          /* slot bytes allocated */
   uint64_t peak_bytes;
   uint64_t budget;
+  /* frames with more signal pixels than their list holds (the -2 of
+   * dext_gpu.hh:148-151): their count is reported as found (above the
+   * slot's capacity) and the labeller takes them as empty frames, as the
+   * driver does not group a frame that failed */
+  uint64_t overflowed;
 } sf_server_stats;
 /* hbm_budget: bytes of slots the server may hold; max_batch_frames: frames
  * per launch; keep_lists: keep every frame's ordered list on the host (for
@@ -276,7 +281,9 @@ int sf_hip_server_drain(sf_server *server);
 int sf_hip_server_close(sf_server *server, int series);
 /* The series' labeller, for sf_hip_labeller_results and friends. */
 sf_labeller *sf_hip_server_labeller(sf_server *server, int series);
-/* Per completed frame of a series, in frame order: number, signal count,
+/* Per completed frame of a series, in frame order: number, signal count
+ * (above sf_hip_server_slot_bytes' list capacity, height * width *
+ * pixel_bytes / 16, when the frame overflowed and was labelled empty),
  * arrival-to-ordered-list latency and arrival-to-labelled latency (ms). Any
  * output may be NULL; *n is the number of frames completed. */
 int sf_hip_server_frames(sf_server *server, int series, int64_t *frames,

@@ -42,3 +42,35 @@ def test_backend_name_without_device():
     import spotfinder
 
     assert spotfinder.backend() == "HIP"
+
+
+@pytest.mark.skipif(not os.path.exists(LIBRARY), reason="library not built")
+def test_find_refuses_frames_over_int32_max_pixels():
+    """gpu::find's bound (src/dext_cuda.cu:588-593): -1 for a frame of more
+    than INT32_MAX pixels, checked before anything touches the device or the
+    image (a 16-byte buffer stands in for a 65536 x 65536 frame), with
+    *found cleared. Also -1 for a null image or a zero dimension."""
+    lib = ctypes.CDLL(LIBRARY)
+    tiny = (ctypes.c_uint8 * 16)()
+    out = (ctypes.c_uint8 * 16)()
+    found = ctypes.c_size_t(123)
+    for name in ("sf_hip_find_u16", "sf_hip_find_u32"):
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                       ctypes.c_void_p, ctypes.c_size_t,
+                       ctypes.POINTER(ctypes.c_size_t)]
+        for h, w in ((65536, 65536), (2, 0x40000000), (0x7fffffff, 2),
+                     (1 << 40, 1)):
+            found.value = 123
+            assert fn(ctypes.addressof(tiny), h, w, ctypes.addressof(out), 1,
+                      ctypes.byref(found)) == -1, (name, h, w)
+            assert found.value == 0
+        for h, w in ((0, 16), (16, 0)):
+            assert fn(ctypes.addressof(tiny), h, w, ctypes.addressof(out), 1,
+                      ctypes.byref(found)) == -1
+        assert fn(None, 4, 4, ctypes.addressof(out), 1,
+                  ctypes.byref(found)) == -1
+    lib.sf_hip_last_error.restype = ctypes.c_char_p
+    assert b"INT32_MAX" in lib.sf_hip_last_error() or b"null" in \
+        lib.sf_hip_last_error()

@@ -130,3 +130,46 @@ def test_submit_rules(sf):
     server.close()
     with pytest.raises(ValueError):
         sf.Server(0)
+
+
+def test_too_dense_frame_is_counted_and_labelled_empty(sf):
+    """A frame with more signal pixels than its slot's list holds (the -2
+    of src/dext_gpu.hh:148-151) is counted in the server's stats, reported
+    with its count above the list's capacity, and labelled as an empty
+    frame: the tables equal the oracle Labeller's over the same frames with
+    that frame's list empty."""
+    h, w, n, dense = 96, 128, 6, 3
+    rng = np.random.default_rng(3)
+    host = np.zeros((n, h, w), dtype=np.uint16)
+    for k in range(n):
+        host[k] = rng.poisson(2.0, size=(h, w))
+        for y, x in ((20, 30), (60, 90)):  # a spot a frame, 2 frames deep
+            host[k, y - 1:y + 2, x - 1:x + 2] += 400
+    host[dense] = rng.integers(0, 5000, size=(h, w))
+    capacity = h * w * 2 // 16
+    assert cpu.dext(host[dense])[2] > capacity
+    buf = sf.DeviceBuffer(host.nbytes)
+    buf.upload(host)
+    server = sf.Server(1 << 26, max_batch=4, keep_lists=True)
+    sid = server.open(2, h, w)
+    for k in range(n):
+        server.submit(sid, 10 + k, buf.ptr + host[0].nbytes * k)
+    server.close_series(sid)
+    frames = server.frames(sid)
+    stats = server.stats()
+    got = server.results(sid)
+    server.close()
+    buf.close()
+    assert stats["overflowed"] == 1
+    assert list(frames["frame"]) == list(range(10, 10 + n))
+    assert int(frames["count"][dense]) > capacity
+    oracle = cpu.Labeller(h, w)
+    for k in range(n):
+        lst = cpu.dext(host[k])[1] if k != dense else cpu.dext(host[0])[1][:0]
+        if k != dense:
+            assert int(frames["count"][k]) == len(lst)
+        assert oracle.add(10 + k, lst) == 0
+    oracle.finish()
+    want = oracle.results()
+    assert same_tables(got, want), (got[2], want[2])
+    assert want[2]["accepted"] > 0
