@@ -1043,4 +1043,5 @@ hipError_t launch_batch(int pixel_bytes, bool any_vec, bool any_scalar,
 }
 
 void ensure_device_tables() { ensure_tables(); }
+void set_last_error(const std::string &message) { g_last_error = message; }
 } // namespace sf

@@ -127,6 +127,34 @@ int sf_hip_find_u32(const uint32_t *image, size_t height, size_t width,
  * first n entries (n <= found) into out. */
 int sf_hip_take_found(sf_signal_pixel *out, size_t n);
 
+/* ---- the threshold with no allocation --------------------------------------
+ * No counterpart in the reference: this is for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc. It is sf_hip_find_u16 / sf_hip_find_u32
+ * (same kernels, same list, same count) on a frame already in device memory,
+ * with every buffer beyond its inputs and outputs in one static device
+ * array of SF_FIND_STATIC_SCRATCH_BYTES per device, held until the process
+ * ends. It calls no hipMalloc, hipFree or other allocating routine.
+ *   pixel_bytes 2 or 4; frame: height x width device pixels; out: device
+ *   list of `capacity` entries; found: a device uint32 that receives the
+ *   number of signal pixels found. Asynchronous on `stream` (a hipStream_t;
+ *   NULL: the null stream). The list is written, ascending by index, when
+ *   the count is at most min(capacity, height * width * pixel_bytes / 16);
+ *   a larger count is the SF_OVERFLOW of find() and leaves the list
+ *   unwritten.
+ * Returns SF_OK once enqueued, SF_BAD_ARGUMENT (message in
+ * sf_hip_last_error) for a null pointer, a zero size or capacity, another
+ * pixel type, more than INT32_MAX pixels, or a frame whose tables exceed the
+ * array (a 4362 x 4148 frame fits at either pixel type), SF_DEVICE_ERROR
+ * when a launch fails. The first call on a device looks the array up and
+ * uploads two small tables (synchronous work: make it outside a capture).
+ * Calls on one device share its array: calls that overlap there (other
+ * streams, other threads) race, and must be serialised by the caller. */
+#define SF_FIND_STATIC_SCRATCH_BYTES ((size_t)96 << 20)
+int sf_hip_find_static(int pixel_bytes, const void *frame, size_t height,
+                       size_t width, sf_signal_pixel *out, size_t capacity,
+                       uint32_t *found, void *stream);
+
 /* ---- device-resident batches (no reference counterpart: the reference
  * launches one frame at a time, dext_cuda.cu:603-656) ----------------------
  * A plan fixes a list of frame shapes of one pixel type (2 or 4 bytes) and
