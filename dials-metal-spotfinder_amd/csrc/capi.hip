@@ -36,6 +36,9 @@ hipError_t launch_sparse_threshold(const FrameDesc *, int, uint32_t,
 hipError_t launch_sparse_order(const FrameDesc *, int, uint32_t, hipStream_t);
 hipError_t launch_sparse(const FrameDesc *, int, uint32_t, uint32_t,
                          hipStream_t);
+hipError_t launch_sparse_pipeline(const FrameDesc *, int, uint32_t, uint32_t,
+                                  const FrameDesc *, int, uint32_t,
+                                  hipStream_t, hipEvent_t *);
 hipError_t launch_probe(int, const FrameDesc *, int, uint32_t, int,
                         hipStream_t);
 int probe_shapes();
@@ -52,7 +55,7 @@ int fail(const char *what, hipError_t err) {
   return SF_DEVICE_ERROR;
 }
 
-int bad(const char *what) {
+int bad(const std::string &what) {
   g_last_error = what;
   return SF_BAD_ARGUMENT;
 }
@@ -156,6 +159,15 @@ struct sf_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_form = -1;
+  // pipelined steps: whether the bound frames have been scouted by a step
+  // (and await their erosion, Poisson pass and ordering), the captured step
+  // this plan owns and the plans it was captured for, and an event for the
+  // waits between the two plans' streams
+  bool scouted = false;
+  hipGraph_t pipe_graph = nullptr;
+  hipGraphExec_t pipe_exec = nullptr;
+  sf_plan *pipe_cur = nullptr, *pipe_next = nullptr;
+  hipEvent_t handoff = nullptr;
 };
 
 namespace {
@@ -185,9 +197,22 @@ uint32_t strips_for(const sf_plan *p, uint32_t width, int form) {
   return (width + ow - 1) / ow;
 }
 
+void drop_pipe_graph(sf_plan *p) {
+  if (p->pipe_exec)
+    (void)hipGraphExecDestroy(p->pipe_exec);
+  if (p->pipe_graph)
+    (void)hipGraphDestroy(p->pipe_graph);
+  p->pipe_exec = nullptr;
+  p->pipe_graph = nullptr;
+  p->pipe_cur = p->pipe_next = nullptr;
+}
+
 void plan_free(sf_plan *p) {
   if (p->stream)
     (void)hipStreamSynchronize(p->stream);
+  drop_pipe_graph(p);
+  if (p->handoff)
+    (void)hipEventDestroy(p->handoff);
   if (p->exec)
     (void)hipGraphExecDestroy(p->exec);
   if (p->graph)
@@ -425,6 +450,8 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
     ok = ok && hipEventCreate(&e) == hipSuccess;
   ok = ok && hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming) ==
                  hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&p->handoff, hipEventDisableTiming) ==
+                 hipSuccess;
   if (!ok) {
     g_last_error = "sf_hip_plan_create: device allocation failed";
     plan_free(p);
@@ -455,6 +482,8 @@ int sf_hip_plan_set_form(sf_plan *p, int form) {
   p->form = form;
   layout(p);
   p->bound.clear(); // force a descriptor upload
+  p->scouted = false;
+  drop_pipe_graph(p);
   return SF_OK;
 }
 
@@ -463,6 +492,8 @@ int sf_hip_plan_set_grid(sf_plan *p, int blocks) {
     return bad("sf_hip_plan_set_grid: bad argument");
   p->blocks = blocks;
   p->graph_form = -1;
+  p->scouted = false;
+  drop_pipe_graph(p);
   return SF_OK;
 }
 
@@ -512,6 +543,8 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
     p->graph = nullptr;
   }
   p->graph_form = -1;
+  p->scouted = false;
+  drop_pipe_graph(p);
   if (kernel == kSweep && need_bands > p->band_capacity) {
     uint32_t *grown = nullptr;
     SF_CHECK(hipMalloc(&grown, sizeof(uint32_t) * need_bands),
@@ -583,6 +616,8 @@ int sf_hip_plan_set_tile(sf_plan *p, int shape) {
     p->graph = nullptr;
   }
   p->graph_form = -1;
+  p->scouted = false;
+  drop_pipe_graph(p);
   p->tile = shape;
   layout(p);
   p->bound.clear();
@@ -607,6 +642,7 @@ int sf_hip_plan_run(sf_plan *p, const void *const *frames) {
   const int up = upload_desc(p, frames);
   if (up != SF_OK)
     return up;
+  p->scouted = false;
   return enqueue(p, p->timing);
 }
 
@@ -616,6 +652,7 @@ int sf_hip_plan_bind(sf_plan *p, const void *const *frames) {
   for (int f = 0; f < p->n; f++)
     if (!frames[f])
       return bad("sf_hip_plan_bind: a frame pointer is null");
+  p->scouted = false; // the tables no longer describe the bound frames
   return upload_desc(p, frames);
 }
 
@@ -719,6 +756,129 @@ int sf_hip_plan_last_pass_times(sf_plan *p, double *first_ms,
 }
 
 void *sf_hip_plan_stream(sf_plan *p) { return p ? (void *)p->stream : nullptr; }
+
+} // extern "C"
+
+namespace {
+
+// Checks the pair of a pipelined step and returns the stream it runs on:
+// the current plan's (the next plan's when there is none).
+int pipeline_check(sf_plan *cur, sf_plan *next, const char *what,
+                   hipStream_t *stream, bool running = true) {
+  if ((!cur && !next) || cur == next)
+    return bad(std::string(what) + ": needs one plan or two different ones");
+  for (sf_plan *p : {cur, next}) {
+    if (p && (p->pixel_bytes != 2 || kernel_for(p, p->form) != kSparse))
+      return bad(std::string(what) +
+                 ": pipelined steps take 16-bit plans of form 0");
+  }
+  // a capture runs nothing: what it needs is checked when it is replayed
+  if (running && cur && !cur->scouted)
+    return bad(std::string(what) +
+               ": the current plan's frames were not scouted by a step");
+  if (running && next && next->bound.size() != (size_t)next->n)
+    return bad(std::string(what) + ": the next plan has no frames bound");
+  *stream = cur ? cur->stream : next->stream;
+  return SF_OK;
+}
+
+// The other plan's stream must be done with what the step overwrites (its
+// tables and descriptors) before the step, and its own later work must
+// follow the step.
+int pipeline_wait_other(sf_plan *next, hipStream_t s) {
+  if (next && next->stream != s) {
+    SF_CHECK(hipEventRecord(next->handoff, next->stream), "hipEventRecord");
+    SF_CHECK(hipStreamWaitEvent(s, next->handoff, 0), "hipStreamWaitEvent");
+  }
+  return SF_OK;
+}
+
+int pipeline_after(sf_plan *cur, sf_plan *next, hipStream_t s) {
+  sf_plan *owner = cur ? cur : next;
+  if (next && next->stream != s) {
+    SF_CHECK(hipEventRecord(owner->handoff, s), "hipEventRecord");
+    SF_CHECK(hipStreamWaitEvent(next->stream, owner->handoff, 0),
+             "hipStreamWaitEvent");
+  }
+  if (cur)
+    cur->scouted = false;
+  if (next)
+    next->scouted = true;
+  return SF_OK;
+}
+
+int pipeline_enqueue(sf_plan *cur, sf_plan *next, hipStream_t s,
+                     bool timing) {
+  sf_plan *owner = cur ? cur : next;
+  // events as the plan's pass times read them: first mixed launch, second,
+  // ordering
+  hipEvent_t marks[4] = {owner->ev[0], owner->ev[1], owner->ev[3],
+                         owner->ev[4]};
+  SF_CHECK(sf::launch_sparse_pipeline(
+               cur ? cur->d_desc : nullptr, cur ? cur->n : 0,
+               cur ? cur->items : 0, cur ? cur->max_cell_rows : 0,
+               next ? next->d_desc : nullptr, next ? next->n : 0,
+               next ? next->items : 0, s, timing ? marks : nullptr),
+           "launching a pipelined step");
+  return SF_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int sf_hip_plan_pipeline(sf_plan *cur, sf_plan *next) {
+  hipStream_t s = nullptr;
+  int status = pipeline_check(cur, next, "sf_hip_plan_pipeline", &s);
+  if (status != SF_OK)
+    return status;
+  if ((status = pipeline_wait_other(next, s)) != SF_OK)
+    return status;
+  sf_plan *owner = cur ? cur : next;
+  if ((status = pipeline_enqueue(cur, next, s, owner->timing)) != SF_OK)
+    return status;
+  return pipeline_after(cur, next, s);
+}
+
+int sf_hip_plan_capture_pipeline(sf_plan *cur, sf_plan *next) {
+  hipStream_t s = nullptr;
+  const int status = pipeline_check(cur, next, "sf_hip_plan_capture_pipeline",
+                                    &s, false);
+  if (status != SF_OK)
+    return status;
+  sf_plan *owner = cur ? cur : next;
+  SF_CHECK(hipStreamSynchronize(s), "hipStreamSynchronize");
+  drop_pipe_graph(owner);
+  SF_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal),
+           "hipStreamBeginCapture");
+  const int enqueued = pipeline_enqueue(cur, next, s, false);
+  hipGraph_t graph = nullptr;
+  const hipError_t end = hipStreamEndCapture(s, &graph);
+  if (enqueued != SF_OK)
+    return enqueued;
+  SF_CHECK(end, "hipStreamEndCapture");
+  owner->pipe_graph = graph;
+  SF_CHECK(hipGraphInstantiate(&owner->pipe_exec, graph, nullptr, nullptr, 0),
+           "hipGraphInstantiate");
+  owner->pipe_cur = cur;
+  owner->pipe_next = next;
+  return SF_OK;
+}
+
+int sf_hip_plan_replay_pipeline(sf_plan *cur, sf_plan *next) {
+  hipStream_t s = nullptr;
+  int status = pipeline_check(cur, next, "sf_hip_plan_replay_pipeline", &s);
+  if (status != SF_OK)
+    return status;
+  sf_plan *owner = cur ? cur : next;
+  if (!owner->pipe_exec || owner->pipe_cur != cur || owner->pipe_next != next)
+    return bad("sf_hip_plan_replay_pipeline: no step captured for these "
+               "plans");
+  if ((status = pipeline_wait_other(next, s)) != SF_OK)
+    return status;
+  SF_CHECK(hipGraphLaunch(owner->pipe_exec, s), "hipGraphLaunch");
+  return pipeline_after(cur, next, s);
+}
 
 // ---------------------------------------------------------------------------
 // Device, memory, policy.

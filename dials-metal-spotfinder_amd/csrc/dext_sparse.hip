@@ -96,6 +96,20 @@ __device__ __forceinline__ void scout_row(const uint32_t (&d)[4],
   }
 }
 
+// The dispersion test (src/dext.cc:97-109) from window sums of p = v + 1
+// over the window's unmasked pixels (s = sum p, q = sum p^2, n7 their
+// count): the sums of v follow exactly, modulo 2^32 as the reference's
+// 32-bit accumulator wraps. In float, in the reference's order.
+__device__ __forceinline__ bool dispersed(uint32_t s, uint32_t q, uint32_t n7,
+                                          const float *lut) {
+  const uint32_t sv = s - n7;
+  const uint32_t qv = q - 2u * sv - n7;
+  const float fm = (float)n7, fs = (float)sv, fq = (float)qv;
+  const float lhs = fm * fq - fs * fs - fs * (fm - 1.0f);
+  const float rhs = fs * 6.0f * lut[n7];
+  return lhs > rhs;
+}
+
 // Scout one item: cell columns 64 * strip + lane, cell rows
 // [16 * band, 16 * band + 16). VEC: the frame's rows are 4-byte aligned and
 // at least 8 pixels wide, so every lane reads its 8 columns as one 16-byte
@@ -172,6 +186,57 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
       load_row.template operator()<CHECK>(y + r, rows[r]);
   };
 
+  // Whether full cell row i's cell (lane's column j), whose probe is masked,
+  // holds no eroded pixel (src/dext.cc:117-142): E(x) = 1 needs mask0 on
+  // every in-frame pixel of x's 5x5 window, so one unmasked pixel z with
+  // mask0(z) = 0 clears E within two rows and columns of it. The points
+  // tried lie on the probe's column pc (every column of the cell is within
+  // two of it): row 5i+1 (else 5i) and row 5i+3 (else 5i+4), where
+  // unmasked; their 7x7 windows have the probe window's columns, so the
+  // lane's eight loaded columns serve. The cell is free when each of its
+  // rows that holds an unmasked pixel is within two rows of a point whose
+  // test is negative. Rows 5i-3 .. 5i+7 are read again, once.
+  const auto eroded_free = [&](int i) -> bool {
+    uint32_t ws[4] = {0, 0, 0, 0}, wq[4] = {0, 0, 0, 0}, wn[4] = {0, 0, 0, 0};
+    uint32_t um = 0, zc = 0; // per cell row: unmasked pixel / at column pc
+    const int pk = probe_at >> 1, half = 16 * (probe_at & 1);
+#pragma unroll 1
+    for (int o = -3; o <= 7; o++) {
+      uint32_t d[4];
+      load_row.template operator()<true>(5 * i + o, d);
+      uint32_t s_ = 0, q_ = 0, n_ = 0, c_ = 0;
+      scout_row(d, m, w5, one, s_, q_, n_, c_);
+      // windows of the points at cell rows 0, 1, 3, 4 (slots 0..3)
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int r = t < 2 ? t : t + 1;
+        if (o >= r - 3 && o <= r + 3) {
+          ws[t] += s_;
+          wq[t] += q_;
+          wn[t] += n_;
+        }
+      }
+      if (o >= 0 && o <= 4) {
+        const uint32_t w = pk == 0   ? d[0]
+                           : pk == 1 ? d[1]
+                           : pk == 2 ? d[2]
+                                     : d[3];
+        um |= (c_ != 0 ? 1u : 0u) << o;
+        zc |= (((w >> half) & 0xFFFFu) < 0xFFFEu ? 1u : 0u) << o;
+      }
+    }
+    uint32_t covered = 0;
+    // top point: row 1, else row 0; bottom point: row 3, else row 4
+    const int top = (zc >> 1) & 1u ? 1 : (zc & 1u) ? 0 : -1;
+    const int bottom = (zc >> 3) & 1u ? 2 : (zc >> 4) & 1u ? 3 : -1;
+    const uint32_t rows_near[4] = {0x07u, 0x0Fu, 0x1Eu, 0x1Cu};
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      if ((t == top || t == bottom) && !dispersed(ws[t], wq[t], wn[t], lut))
+        covered |= rows_near[t];
+    return (um & ~covered) == 0;
+  };
+
   const int i0 = (int)band * kBandCells, i1 = min(i0 + kBandCells, ncr);
   // rows 5 i0 - 1 and 5 i0: the first window's top, carried like a tail
   uint32_t ts = 0, tq = 0, tn = 0, top5 = 0, unused = 0;
@@ -220,17 +285,18 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
                                     : probe_row[3];
     const uint32_t pv = (word >> (16 * (probe_at & 1))) & 0xFFFFu;
     const bool masked = pv >= 0xFFFEu;
-    // the dispersion test (src/dext.cc:97-109): sums of v from sums of v + 1
-    const uint32_t sv = s - n7;
-    const uint32_t qv = q - 2u * sv - n7;
-    const float fm = (float)n7, fs = (float)sv, fq = (float)qv;
-    const float lhs = fm * fq - fs * fs - fs * (fm - 1.0f);
-    const float rhs = fs * 6.0f * lut[n7];
-    const bool active = live && (masked || lhs > rhs) && n5 > 0;
+    const bool active = live && (masked || dispersed(s, q, n7, lut)) && n5 > 0;
+    // A full cell whose probe lies in a gap (0xFFFF: a module gap that cuts
+    // the cell) usually holds no eroded pixel: such cells are checked after
+    // the sweep, when its registers are free. (Cells whose probe is a hot
+    // pixel, 0xFFFE, are scattered one to a wave: re-reading their windows
+    // would cost more than their erosion, so they stay.)
+    const bool full = 5 * i + 4 < height && 5 * j + 4 < width;
     const uint64_t bits = ballot(active);
+    const uint64_t check = ballot(active && pv == 0xFFFFu && full);
     if (lane == 0) {
-      fd.act[(size_t)i * fd.n_strips + strip] = bits;
       masks[i - i0] = bits;
+      masks[kBandCells + i - i0] = check;
     }
   };
   // two row sets in turn: one loads while the other is summed
@@ -252,6 +318,19 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
     sweep.template operator()<false>();
   else
     sweep.template operator()<true>();
+  // drop the checked cells that hold no eroded pixel, then publish the bits
+  for (int i = i0; i < i1; i++) {
+    uint64_t bits = masks[i - i0];
+    const uint64_t check = masks[kBandCells + i - i0];
+    if (check) { // uniform, and rare
+      const bool mine = (check >> lane) & 1u;
+      bits &= ~ballot(eroded_free(i) && mine);
+      if (lane == 0)
+        masks[i - i0] = bits;
+    }
+    if (lane == 0)
+      fd.act[(size_t)i * fd.n_strips + strip] = bits;
+  }
   // the item's active cells, in its own segment of the list
   uint32_t total = 0;
   const uint32_t local = band * fd.n_strips + strip;
@@ -270,35 +349,49 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
 // frame, copied into registers (the passes store through pointers the
 // compiler cannot tell apart from the descriptor), and its index within the
 // frame; false past the end.
-__device__ __forceinline__ bool block_item(const FrameDesc *frames,
-                                           int n_frames, uint32_t items,
-                                           FrameDesc &fd, uint32_t &local) {
+__device__ __forceinline__ bool wave_item(const FrameDesc *frames,
+                                          int n_frames, uint32_t first,
+                                          uint32_t end, uint32_t block,
+                                          FrameDesc &fd, uint32_t &local) {
   // the wave's index, as a scalar: everything derived from it is uniform
   const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane(
-      (int)(4 * xcd_local_item(blockIdx.x, gridDim.x) + (threadIdx.x >> 6)));
-  if (item >= items)
+      (int)(first + 4 * block + (threadIdx.x >> 6)));
+  if (item >= end)
     return false;
   fd = frames[frame_of_item(frames, n_frames, item)];
   local = item - fd.item_begin;
   return true;
 }
 
+__device__ __forceinline__ bool block_item(const FrameDesc *frames,
+                                           int n_frames, uint32_t items,
+                                           FrameDesc &fd, uint32_t &local) {
+  return wave_item(frames, n_frames, 0, items,
+                   xcd_local_item(blockIdx.x, gridDim.x), fd, local);
+}
+
+// A scout work item on this wave: its frame's lattice, strip and band.
+__device__ __forceinline__ void scout_wave(const FrameDesc &fd, uint32_t local,
+                                           const float *lut, uint64_t *masks) {
+  const uint32_t strip = local % fd.n_strips, band = local / fd.n_strips;
+  if (fd.vec && fd.width >= 8)
+    scout_item<true>(fd, strip, band, lut, masks);
+  else
+    scout_item<false>(fd, strip, band, lut, masks);
+}
+
 __global__ __launch_bounds__(256) void scout_kernel(const FrameDesc *frames,
                                                     int n_frames,
                                                     uint32_t items) {
   __shared__ float lut[64];
-  __shared__ uint64_t masks[4][kBandCells];
+  // per wave: each cell row's active cells, then the cells to check
+  __shared__ uint64_t masks[4][2 * kBandCells];
   fill_lut(lut);
   __syncthreads();
   FrameDesc fd;
   uint32_t local;
-  if (!block_item(frames, n_frames, items, fd, local))
-    return;
-  const uint32_t strip = local % fd.n_strips, band = local / fd.n_strips;
-  if (fd.vec && fd.width >= 8)
-    scout_item<true>(fd, strip, band, lut, masks[threadIdx.x >> 6]);
-  else
-    scout_item<false>(fd, strip, band, lut, masks[threadIdx.x >> 6]);
+  if (block_item(frames, n_frames, items, fd, local))
+    scout_wave(fd, local, lut, masks[threadIdx.x >> 6]);
 }
 
 // ---------------------------------------------------------------------------
@@ -697,6 +790,94 @@ __global__ __launch_bounds__(256) void poisson_kernel(const FrameDesc *frames,
     poisson_item(fd, local, slots[wave], stash[wave]);
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined launches. The scout streams every pixel from HBM with VALU to
+// spare; erosion and Poisson read little but keep the VALU busy. A chunk's
+// scout therefore runs beside the previous chunk's erosion (one launch) and
+// beside its Poisson pass (the next launch): half the scout items in each.
+// Blocks of the two kinds alternate in the launch, so that every CU holds
+// both. No block of a launch waits for another: each launch's kinds work on
+// different chunks, whose tables are apart.
+// ---------------------------------------------------------------------------
+struct Side {
+  const FrameDesc *frames; // descriptors of the side's chunk
+  int n_frames;
+  uint32_t first, end;     // its work items in this launch
+  uint32_t blocks;         // blocks of four items
+};
+
+// Block `block` of a mixed launch: 1 for a scout block, 0 for a sparse
+// block, with its index among its kind; -1 for the grid's padding. Blocks are dealt to the eight XCDs round-robin, and
+// each XCD draws HBM through its own share of the fabric: so each XCD takes
+// an eighth of each kind (contiguous runs: neighbouring strips share an
+// L2), the two kinds spread through its blocks in proportion. The grid is
+// mixed_grid() blocks.
+__device__ __forceinline__ int mixed_role(uint32_t block,
+                                          uint32_t scout_blocks,
+                                          uint32_t sparse_blocks,
+                                          uint32_t &index) {
+  const uint32_t x = block % 8, k = block / 8;
+  const uint32_t s0 = scout_blocks * x / 8, s1 = scout_blocks * (x + 1) / 8;
+  const uint32_t e0 = sparse_blocks * x / 8, e1 = sparse_blocks * (x + 1) / 8;
+  const uint32_t ns = s1 - s0, n = ns + (e1 - e0);
+  index = 0;
+  if (k >= n)
+    return -1;
+  // scout blocks among this XCD's first k and first k + 1
+  const uint32_t before = (uint32_t)((uint64_t)k * ns / n);
+  const bool scout = (uint32_t)((uint64_t)(k + 1) * ns / n) > before;
+  index = scout ? s0 + before : e0 + (k - before);
+  return scout ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void scout_erode_kernel(Side scout,
+                                                          Side sparse) {
+  __shared__ float lut[64];
+  __shared__ uint64_t masks[4][2 * kBandCells];
+  fill_lut(lut);
+  __syncthreads();
+  uint32_t index;
+  const int role = mixed_role(blockIdx.x, scout.blocks, sparse.blocks, index);
+  FrameDesc fd;
+  uint32_t local;
+  if (role < 0)
+    return;
+  if (role == 1) {
+    if (wave_item(scout.frames, scout.n_frames, scout.first, scout.end,
+                  index, fd, local))
+      scout_wave(fd, local, lut, masks[threadIdx.x >> 6]);
+  } else if (wave_item(sparse.frames, sparse.n_frames, sparse.first,
+                       sparse.end, index, fd, local)) {
+    erode_item(fd, local, lut);
+  }
+}
+
+// (Four blocks a CU, the scout's occupancy: 128 VGPRs at most.)
+__global__ __launch_bounds__(256, 4) void scout_poisson_kernel(Side scout,
+                                                            Side sparse) {
+  __shared__ float lut[64];
+  __shared__ uint64_t masks[4][2 * kBandCells];
+  __shared__ uint64_t slots[4][64];
+  __shared__ Stash stash[4];
+  fill_lut(lut);
+  __syncthreads();
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t index;
+  const int role = mixed_role(blockIdx.x, scout.blocks, sparse.blocks, index);
+  FrameDesc fd;
+  uint32_t local;
+  if (role < 0)
+    return;
+  if (role == 1) {
+    if (wave_item(scout.frames, scout.n_frames, scout.first, scout.end,
+                  index, fd, local))
+      scout_wave(fd, local, lut, masks[wave]);
+  } else if (wave_item(sparse.frames, sparse.n_frames, sparse.first,
+                       sparse.end, index, fd, local)) {
+    poisson_item(fd, local, slots[wave], stash[wave]);
+  }
+}
+
 // Inclusive sum of x over the lanes up to this one (DPP: within rows of 16,
 // then row 0's and row 1's totals broadcast upwards).
 __device__ __forceinline__ uint32_t wave_prefix(uint32_t x) {
@@ -954,6 +1135,63 @@ hipError_t launch_sparse_order(const FrameDesc *frames, int n_frames,
     tag_kernel<<<dim3((max_cell_rows + 3) / 4, n_frames), 256, 0, stream>>>(
         frames);
   place_kernel<<<dim3(64, n_frames), 256, 0, stream>>>(frames);
+  return hipGetLastError();
+}
+
+// Blocks of a mixed launch: on each XCD its eighth of both kinds.
+uint32_t mixed_grid(uint32_t scout_blocks, uint32_t sparse_blocks) {
+  return 8 * ((scout_blocks + 7) / 8 + (sparse_blocks + 7) / 8);
+}
+
+// One pipelined step on `stream`: the threshold of `cur` (n_cur frames,
+// scouted by the previous step) finished -- erosion, Poisson, ordering --
+// while `next` (n_next frames) is cleared and scouted. Either side may be
+// empty. `marks` (optional, four events): before the first mixed launch,
+// after it, after the second, after the ordering.
+hipError_t launch_sparse_pipeline(const FrameDesc *cur, int n_cur,
+                                  uint32_t cur_items,
+                                  uint32_t cur_max_cell_rows,
+                                  const FrameDesc *next, int n_next,
+                                  uint32_t next_items, hipStream_t stream,
+                                  hipEvent_t *marks) {
+  hipError_t e;
+  if (n_next > 0)
+    zero_kernel<<<n_next, 256, 0, stream>>>(next);
+  if (n_next <= 0)
+    next_items = 0;
+  if (n_cur <= 0)
+    cur_items = 0;
+  // the scout's items in two halves of whole blocks
+  const uint32_t half = ((next_items / 2) + 3) & ~3u;
+  const uint32_t split = half < next_items ? half : next_items;
+  const uint32_t sparse_blocks = (cur_items + 3) / 4;
+  const Side a_scout{next, n_next, 0, split, (split + 3) / 4};
+  const Side b_scout{next, n_next, split, next_items,
+                     (next_items - split + 3) / 4};
+  const Side sparse{cur, n_cur, 0, cur_items, sparse_blocks};
+  if (marks && (e = hipEventRecord(marks[0], stream)) != hipSuccess)
+    return e;
+  const uint32_t grid_a = a_scout.blocks + sparse_blocks
+                              ? mixed_grid(a_scout.blocks, sparse_blocks)
+                              : 0;
+  if (grid_a)
+    scout_erode_kernel<<<grid_a, 256, 0, stream>>>(a_scout, sparse);
+  if (marks && (e = hipEventRecord(marks[1], stream)) != hipSuccess)
+    return e;
+  const uint32_t grid_b = b_scout.blocks + sparse_blocks
+                              ? mixed_grid(b_scout.blocks, sparse_blocks)
+                              : 0;
+  if (grid_b)
+    scout_poisson_kernel<<<grid_b, 256, 0, stream>>>(b_scout, sparse);
+  if (marks && (e = hipEventRecord(marks[2], stream)) != hipSuccess)
+    return e;
+  if (n_cur > 0) {
+    e = launch_sparse_order(cur, n_cur, cur_max_cell_rows, stream);
+    if (e != hipSuccess)
+      return e;
+  }
+  if (marks && (e = hipEventRecord(marks[3], stream)) != hipSuccess)
+    return e;
   return hipGetLastError();
 }
 

@@ -141,6 +141,9 @@ def _declare(lib: ctypes.CDLL) -> None:
        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
        ctypes.POINTER(ctypes.c_double))
     fn("sf_hip_plan_stream", _p, _p)
+    for name in ("sf_hip_plan_pipeline", "sf_hip_plan_capture_pipeline",
+                 "sf_hip_plan_replay_pipeline"):
+        fn(name, _i, _p, _p)
     fn("sf_hip_synth_frames", _i, _p, _i, _u32, _u32, _u32, ctypes.c_uint64,
        ctypes.c_double, _i, ctypes.c_double, _p, _u32, ctypes.c_double)
     fn("sf_hip_labeller_create", _p, _sz, _sz, ctypes.POINTER(Options))
@@ -489,6 +492,30 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def _handle(plan):
+    return plan.handle if plan is not None else None
+
+
+def pipeline(current: "Plan | None", next_plan: "Plan | None") -> None:
+    """One pipelined step (sf_hip_plan_pipeline): finish `current` (scouted
+    by the step before) while scouting `next_plan` (frames bound); either
+    may be None."""
+    _check(lib().sf_hip_plan_pipeline(_handle(current), _handle(next_plan)),
+           "pipeline")
+
+
+def capture_pipeline(current: "Plan | None", next_plan: "Plan | None") -> None:
+    _check(lib().sf_hip_plan_capture_pipeline(_handle(current),
+                                              _handle(next_plan)),
+           "capture_pipeline")
+
+
+def replay_pipeline(current: "Plan | None", next_plan: "Plan | None") -> None:
+    _check(lib().sf_hip_plan_replay_pipeline(_handle(current),
+                                             _handle(next_plan)),
+           "replay_pipeline")
 
 
 # --- 3-D grouping and centroids (dials_spots::Labeller) ----------------------

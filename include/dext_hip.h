@@ -222,6 +222,29 @@ int sf_hip_plan_last_pass_times(sf_plan *plan, double *first_ms,
 /* The plan's stream, as an opaque handle (hipStream_t). */
 void *sf_hip_plan_stream(sf_plan *plan);
 
+/* Pipelined steps over a series of chunks (16-bit plans of form 0; no
+ * reference counterpart). The sparse threshold's first pass (the scout)
+ * streams every pixel from HBM; its later passes (erosion, Poisson test)
+ * read little and keep the vector units busy. A step finishes the chunk
+ * bound to `current` -- scouted by the step before -- and scouts the chunk
+ * bound to `next`, in two launches that each mix half of next's scout with
+ * one of current's passes, then orders current's lists. Afterwards
+ * current's counts and lists are those of run(), and next awaits the step
+ * that finishes it. Either plan may be NULL: (NULL, first) only scouts,
+ * (last, NULL) only finishes. SF_BAD_ARGUMENT if current was not scouted by
+ * a step since its frames were bound, if next has no frames bound, or for a
+ * plan of another pixel type or form. The step runs on current's stream
+ * (next's when current is NULL), after next's stream has finished its work;
+ * next's stream then waits for the step. */
+int sf_hip_plan_pipeline(sf_plan *current, sf_plan *next);
+/* The same step captured into a hipGraph that `current` (or `next` when
+ * current is NULL) owns; replay_pipeline launches it for the same pair,
+ * reading whatever frames are bound to the two plans at that time, with the
+ * same checks as sf_hip_plan_pipeline (a capture runs nothing and needs no
+ * frames). Capture again after either plan changes form. */
+int sf_hip_plan_capture_pipeline(sf_plan *current, sf_plan *next);
+int sf_hip_plan_replay_pipeline(sf_plan *current, sf_plan *next);
+
 /* ---- 3-D grouping and centroids on the device
  * (dials_spots::Labeller, src/dials_spots.hh:108-158) -----------------------
  * Same options, same output: the Spot and Pixel tables and Counts come out

@@ -120,6 +120,16 @@ def ref_lib() -> ctypes.CDLL:
             fn = getattr(lib, name)
             fn.restype = ctypes.c_longlong
             fn.argtypes = [_ptr, _c_size, _c_size, _c_size, ctypes.c_int]
+        for name in ("ref_dext_cycle_u16", "ref_dext_cycle_u32"):
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_longlong
+            fn.argtypes = [_ptr, _c_size, _c_size, _c_size, _c_size,
+                           ctypes.c_int]
+        lib.ref_pipeline_u16.restype = ctypes.c_int
+        lib.ref_pipeline_u16.argtypes = [
+            _ptr, _c_size, _c_size, _c_size, ctypes.c_int, ctypes.c_char_p,
+            ctypes.POINTER(_c_size), ctypes.POINTER(_c_size),
+            ctypes.POINTER(_c_size)]
         lib.ref_by_index.argtypes = [_ptr, _c_size]
         lib.ref_labeller_new.restype = _ptr
         lib.ref_labeller_new.argtypes = [_c_size, _c_size, _c_size, _c_size,
@@ -194,6 +204,30 @@ def ref_dext_many(frames: np.ndarray, threads: int) -> int:
     lib = ref_lib()
     fn = lib.ref_dext_many_u16 if frames.dtype == np.uint16 else lib.ref_dext_many_u32
     return int(fn(_address(frames), n, height, width, threads))
+
+
+def ref_dext_cycle(frames: np.ndarray, jobs: int, threads: int) -> int:
+    """`jobs` runs of the reference's dext over a (n, h, w) stack (frame
+    j % n), on `threads` host threads; total found."""
+    frames = np.ascontiguousarray(frames)
+    n, height, width = frames.shape
+    lib = ref_lib()
+    fn = (lib.ref_dext_cycle_u16 if frames.dtype == np.uint16
+          else lib.ref_dext_cycle_u32)
+    return int(fn(_address(frames), n, height, width, jobs, threads))
+
+
+def ref_pipeline(frames: np.ndarray, threads: int, path: str):
+    """The reference's whole CPU path over a (n, h, w) u16 stack: dext on
+    `threads` workers, the Labeller in frame order, refl::write to `path`.
+    Returns (status, signal pixels, spots, pixels)."""
+    frames = np.ascontiguousarray(frames, dtype=np.uint16)
+    n, height, width = frames.shape
+    signal, spots, pixels = _c_size(0), _c_size(0), _c_size(0)
+    status = ref_lib().ref_pipeline_u16(
+        _address(frames), n, height, width, threads, path.encode(),
+        ctypes.byref(signal), ctypes.byref(spots), ctypes.byref(pixels))
+    return status, signal.value, spots.value, pixels.value
 
 
 def oracle_dext_many(frames: np.ndarray, threads: int) -> int:

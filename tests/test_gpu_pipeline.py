@@ -1,0 +1,173 @@
+"""Pipelined steps (sf_hip_plan_pipeline / capture / replay): a chunk's
+scout runs beside the previous chunk's erosion and Poisson passes. The lists
+each step finishes must equal the oracle's (and run()'s) byte for byte,
+whatever the order of plans, shapes and rebinding; the labeller fed from
+the finished plans must give the oracle's tables."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import frames as fr
+from oracle import cpu
+
+pytestmark = pytest.mark.gpu
+
+
+def chunk_frames(sf, shapes, seed, density=30000.0):
+    """Device frames of the measured series' value model, one per shape;
+    returns (buffers, pointers, host frames)."""
+    bufs, ptrs, hosts = [], [], []
+    for k, (h, w) in enumerate(shapes):
+        b = sf.DeviceBuffer(2 * h * w)
+        spots = fr.diffraction_spots(seed + k, h, w, 1,
+                                     density * h * w / (4362 * 4148))
+        sf.synth_frames(b, 2, h, w, 1, seed + k, 0.3, True, 0.0005, spots)
+        host = np.zeros((h, w), np.uint16)
+        b.download(host)
+        bufs.append(b)
+        ptrs.append(b.ptr)
+        hosts.append(host)
+    return bufs, ptrs, hosts
+
+
+def check(plan, hosts, what):
+    plan.wait()
+    counts = plan.counts()
+    for f, host in enumerate(hosts):
+        want = cpu.dext(host, capacity=host.size // 8)[1]
+        got = plan.output(f, int(counts[f]))
+        assert got.tobytes() == want.tobytes(), (what, f, len(got),
+                                                 len(want))
+
+
+SHAPES_A = [(216, 206), (327, 311), (101, 3), (436, 414)]
+SHAPES_B = [(300, 517), (64, 64), (560, 1040)]
+
+
+def test_eager_steps_over_chunks_of_other_shapes(sf):
+    chunks = [chunk_frames(sf, SHAPES_A if c % 2 == 0 else SHAPES_B, 10 * c)
+              for c in range(4)]
+    plans = [sf.Plan(2, SHAPES_A), sf.Plan(2, SHAPES_B)]
+    plans[0].bind(chunks[0][1])
+    sf.pipeline(None, plans[0])
+    for c in range(1, 4):
+        cur, nxt = plans[(c - 1) % 2], plans[c % 2]
+        nxt.bind(chunks[c][1])
+        sf.pipeline(cur, nxt)
+        check(cur, chunks[c - 1][2], ("chunk", c - 1))
+    sf.pipeline(plans[1], None)
+    check(plans[1], chunks[3][2], ("chunk", 3))
+    for p in plans:
+        p.close()
+    for bufs, _, _ in chunks:
+        for b in bufs:
+            b.close()
+
+
+def test_replayed_steps_and_the_labeller(sf):
+    """Two captured steps (P0 -> P1 and P1 -> P0) replayed over six chunks
+    of one shape list, frames rebound before each replay, eager work between
+    replays; the finished plans feed one device labeller, whose tables equal
+    the oracle Labeller's over the oracle's lists."""
+    shapes = [(256, 384)] * 3
+    chunks = [chunk_frames(sf, shapes, 100 + 10 * c, density=60000.0)
+              for c in range(6)]
+    p = [sf.Plan(2, shapes), sf.Plan(2, shapes)]
+    p[0].bind(chunks[0][1])
+    sf.pipeline(None, p[0])
+    p[1].bind(chunks[1][1])
+    sf.capture_pipeline(p[0], p[1])
+    lab = sf.Labeller(256, 384)
+    oracle = cpu.Labeller(256, 384)
+    for c in range(6):
+        cur, nxt = p[c % 2], p[(c + 1) % 2]
+        if c + 1 < 6:
+            if c == 1:  # the second pair's graph, captured on first use
+                nxt.bind(chunks[c + 1][1])
+                sf.capture_pipeline(cur, nxt)
+            nxt.bind(chunks[c + 1][1])
+            sf.replay_pipeline(cur, nxt)
+        else:
+            sf.pipeline(cur, None)
+        check(cur, chunks[c][2], ("chunk", c))
+        lab.add_plan(cur, 3 * c)
+        for k, host in enumerate(chunks[c][2]):
+            assert oracle.add(3 * c + k, cpu.dext(host)[1]) == 0
+        assert sf.find(chunks[c][2][0])[0] == 0  # eager work in between
+    lab.finish()
+    oracle.finish()
+    got, want = lab.results(), oracle.results()
+    assert got[2] == want[2]
+    assert got[0].tobytes() == want[0].tobytes()
+    assert got[1].tobytes() == want[1].tobytes()
+    assert want[2]["accepted"] > 10
+    lab.close()
+    for q in p:
+        q.close()
+    for bufs, _, _ in chunks:
+        for b in bufs:
+            b.close()
+
+
+def test_full_size_steps_equal_run(sf):
+    """Two 8-frame chunks of 4362 x 4148 frames of the measured series
+    through pipelined steps: every list equals the plan's own run() on the
+    same frames, byte for byte."""
+    h, w, n = 4362, 4148, 16
+    buf = sf.DeviceBuffer(2 * h * w * n)
+    spots = fr.diffraction_spots(5, h, w, n, 1500.0)
+    sf.synth_frames(buf, 2, h, w, n, 5, 0.3, True, 0.0005, spots)
+    ptr = [buf.ptr + 2 * h * w * f for f in range(n)]
+    ref = sf.Plan(2, [(h, w)] * 8)
+    want = []
+    for c in range(2):
+        ref.run(ptr[8 * c:8 * c + 8])
+        ref.wait()
+        counts = ref.counts()
+        want.append([ref.output(k, int(counts[k])) for k in range(8)])
+    ref.close()
+    p = [sf.Plan(2, [(h, w)] * 8), sf.Plan(2, [(h, w)] * 8)]
+    p[0].bind(ptr[:8])
+    sf.pipeline(None, p[0])
+    p[1].bind(ptr[8:])
+    sf.pipeline(p[0], p[1])
+    sf.pipeline(p[1], None)
+    for c in range(2):
+        p[c].wait()
+        counts = p[c].counts()
+        for k in range(8):
+            got = p[c].output(k, int(counts[k]))
+            assert got.tobytes() == want[c][k].tobytes(), (c, k)
+        assert len(want[c][0]) > 20000
+    for q in p:
+        q.close()
+    buf.close()
+
+
+def test_refusals(sf):
+    a, b = sf.Plan(2, [(64, 64)]), sf.Plan(2, [(64, 64)])
+    c32 = sf.Plan(4, [(64, 64)])
+    buf = sf.DeviceBuffer(4 * 64 * 64)
+    with pytest.raises(ValueError):
+        sf.pipeline(None, None)
+    with pytest.raises(ValueError):
+        sf.pipeline(a, b)  # a was not scouted
+    with pytest.raises(ValueError):
+        sf.pipeline(None, a)  # a has no frames bound
+    a.bind([buf.ptr])
+    with pytest.raises(ValueError):
+        sf.pipeline(a, a)
+    c32.bind([buf.ptr])
+    with pytest.raises(ValueError):
+        sf.pipeline(None, c32)
+    sf.pipeline(None, a)
+    with pytest.raises(ValueError):
+        sf.replay_pipeline(a, None)  # nothing captured
+    a.bind([buf.ptr])  # rebinding drops the scout
+    with pytest.raises(ValueError):
+        sf.pipeline(a, None)
+    for q in (a, b, c32):
+        q.close()
+    buf.close()
