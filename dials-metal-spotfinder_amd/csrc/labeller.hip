@@ -335,7 +335,9 @@ __global__ __launch_bounds__(kTileThreads) void number_runs(
           run_idx[r] = idx;
           run_slot[r] = node_slot[g];
           parent[r] = g < a.n_carry ? carry_root[g] : r;
-          last_slot[r] = 0;
+          // the last frame (slot) this run alone reaches, raised to its
+          // component's by resolve_runs
+          last_slot[r] = node_slot[g];
         }
         if (tail)
           run_last[r] = idx;
@@ -448,28 +450,286 @@ __device__ __forceinline__ void unite_row(uint32_t *parent,
   }
 }
 
-// The up and previous-frame edges (src/dials_spots.cc:106-137), a run at a
-// time: a run joins each run above it on its frame, and each run on the same
-// row of the frame before when the frame numbers are consecutive and not
-// two_d. Runs in run order, a contiguous range of them to each XCD, so that
-// the run tables they read are mostly in that XCD's L2.
-__global__ __launch_bounds__(256) void unite_runs(
+// The up and previous-frame edges are src/dials_spots.cc:106-137's: a run
+// joins each run above it on its frame, and each run on the same row of the
+// frame before when the frame numbers are consecutive and not two_d.
+// ---------------------------------------------------------------------------
+// The same unions in two passes, with few global atomics. Cut the chunk into
+// tiles of rows x frames: an up edge stays on its frame and a
+// previous-frame edge on its row, so most edges join two runs of one tile.
+// A workgroup takes a tile, loads its runs' columns and rows into LDS and
+// unites them there; then one pass over all runs does, on the global
+// forest, the edges the tiles could not: up edges from a tile's first row,
+// previous-frame edges from a tile's first frame (to the carried frontier
+// too), and every edge of a tile whose runs overflow LDS. Roots still always
+// point to the lower run: local ids follow run order (slot, then row, then
+// column), and the global pass hooks the larger root under the smaller.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kTileRuns = 5632;   // runs a tile unites in LDS
+constexpr uint32_t kTileFrames = 32;   // chunk frames a tile spans at most
+constexpr uint32_t kTileMaxRows = 120; // rows a tile spans at most
+constexpr uint32_t kTileMinRows = 8;
+
+// The chunk's tiling, the same in both passes: frame groups of
+// kTileFrames (or fewer), and rows per tile such that a tile is expected to
+// hold three quarters of kTileRuns.
+struct Tiling {
+  uint32_t frames, rows, bands, groups;
+};
+__device__ __forceinline__ Tiling tiling_for(const LabelArgs &a,
+                                             uint32_t runs) {
+  Tiling t;
+  const uint32_t height = a.pixels / a.width;
+  const uint32_t nf = a.n_frames ? a.n_frames : 1u;
+  t.frames = min(nf, kTileFrames);
+  const uint64_t per_row = (uint64_t)(runs ? runs : 1u) * t.frames / nf;
+  const uint64_t want = (uint64_t)kTileRuns * 3 / 4 * height /
+                        (per_row ? per_row : 1u);
+  t.rows = (uint32_t)min<uint64_t>(
+      min<uint64_t>(max<uint64_t>(want, kTileMinRows), kTileMaxRows), height);
+  t.bands = (height + t.rows - 1) / t.rows;
+  t.groups = (nf + t.frames - 1) / t.frames;
+  return t;
+}
+
+__device__ __forceinline__ uint32_t find_local(uint32_t *lp, uint32_t i) {
+  uint32_t p = lp[i];
+  while (p != i) {
+    const uint32_t gp = lp[p];
+    if (gp != p)
+      lp[i] = gp;
+    i = p;
+    p = gp;
+  }
+  return i;
+}
+
+__device__ void unite_local(uint32_t *lp, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = find_local(lp, a);
+    b = find_local(lp, b);
+    if (a == b)
+      return;
+    if (a > b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    const uint32_t old = atomicCAS(&lp[b], b, a);
+    if (old == b)
+      return;
+    b = old;
+  }
+}
+
+// Inclusive sum over the wave's lanes.
+__device__ __forceinline__ uint32_t wave_inclusive(uint32_t x) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+    if (lane >= (uint32_t)d)
+      x += y;
+  }
+  return x;
+}
+
+// One tile per block: block b is band b % bands (rows [band * rows, ..))
+// of frame group b / bands (slots 1 + group * frames .. ). Its runs united in
+// LDS, then each run's parent set to its local root. A tile that cannot be
+// held (more runs than kTileRuns, a frame wider than 65,536) is marked in
+// spill and left to the global pass. The grid is an upper bound on the
+// tiles; blocks past them return.
+__global__ __launch_bounds__(1024) void unite_tiles(
+    LabelArgs a, const ChunkDesc *frames, const uint32_t *run_idx,
+    const uint32_t *run_last, const uint32_t *run_rows,
+    const uint32_t *n_runs, uint32_t *parent, uint32_t *last_slot,
+    uint32_t *spill) {
+  __shared__ uint32_t lp[kTileRuns];
+  __shared__ uint32_t cols[kTileRuns];  // first column | last column << 16
+  __shared__ uint8_t row_of_run[kTileRuns];
+  __shared__ uint16_t lrows[kTileFrames * (kTileMaxRows + 1)];
+  __shared__ uint32_t slot_first[kTileFrames + 1], slot_off[kTileFrames + 1];
+  __shared__ uint32_t linked[kTileFrames];
+  __shared__ uint32_t total;
+  const uint32_t height = a.pixels / a.width, stride = height + 1;
+  const Tiling t = tiling_for(a, *n_runs);
+  if (blockIdx.x >= t.bands * t.groups)
+    return;
+  const uint32_t band = blockIdx.x % t.bands, group = blockIdx.x / t.bands;
+  const uint32_t y0 = band * t.rows, y1 = min(y0 + t.rows, height);
+  const uint32_t s_lo = 1 + group * t.frames;
+  const uint32_t nslots = min(t.frames, a.n_frames + 1 - s_lo);
+  if (threadIdx.x < 64) { // each slot's runs in the tile, as local ranges
+    const uint32_t lane = lane_id();
+    uint32_t first = 0, count = 0;
+    if (lane < nslots) {
+      const uint32_t s = s_lo + lane;
+      first = run_rows[(size_t)s * stride + y0];
+      count = run_rows[(size_t)s * stride + y1] - first;
+      linked[lane] = frames[s - 1].linked;
+    }
+    const uint32_t incl = wave_inclusive(count);
+    if (lane < nslots) {
+      slot_first[lane] = first;
+      slot_off[lane] = incl - count;
+    }
+    if (lane == 63)
+      total = incl;
+  }
+  __syncthreads();
+  const uint32_t n = total;
+  if (n > kTileRuns || a.width > 65536u) {
+    if (threadIdx.x == 0)
+      spill[blockIdx.x] = 1;
+    return;
+  }
+  if (threadIdx.x == 0)
+    spill[blockIdx.x] = 0;
+  // the local slot of local run i: the last slot whose offset is at most i
+  const auto slot_of = [&](uint32_t i) {
+    uint32_t lo = 0, hi = nslots - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (slot_off[mid] <= i)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    return lo;
+  };
+  const uint32_t trows = y1 - y0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t ls = slot_of(i);
+    const uint32_t r = slot_first[ls] + (i - slot_off[ls]);
+    const uint32_t idx = run_idx[r], last = run_last[r];
+    const uint32_t y = row_of(a, idx);
+    const uint32_t base = y * a.width;
+    lp[i] = i;
+    cols[i] = (idx - base) | ((last - base) << 16);
+    row_of_run[i] = (uint8_t)(y - y0);
+  }
+  for (uint32_t e = threadIdx.x; e < nslots * (trows + 1); e += blockDim.x) {
+    const uint32_t ls = e / (trows + 1), ry = e % (trows + 1);
+    lrows[ls * (kTileMaxRows + 1) + ry] =
+        (uint16_t)(slot_off[ls] +
+                   (run_rows[(size_t)(s_lo + ls) * stride + y0 + ry] -
+                    slot_first[ls]));
+  }
+  __syncthreads();
+  // The neighbours of local run i (columns lo .. hi) among the runs
+  // [from, to) of one row -- disjoint, in column order -- are those that
+  // share a column with it: a range [first, end) of that row.
+  const auto span = [&](uint32_t from, uint32_t to, uint32_t lo, uint32_t hi,
+                        uint32_t &end) {
+    if (to - from > 8) { // skip the runs that end left of lo
+      uint32_t l = from, h = to;
+      while (l < h) {
+        const uint32_t mid = (l + h) >> 1;
+        if ((cols[mid] >> 16) < lo)
+          l = mid + 1;
+        else
+          h = mid;
+      }
+      from = l;
+    }
+    while (from < to && (cols[from] >> 16) < lo)
+      from++;
+    end = from;
+    while (end < to && (cols[end] & 0xFFFFu) <= hi)
+      end++;
+    return from;
+  };
+  // Two passes, so that runs of one large component do not all race to
+  // hook one root: first each run points at its lowest neighbour (every
+  // neighbour comes earlier in local order), a plain store; then each
+  // further edge is united, and most of them already share a root.
+  const auto neighbours = [&](uint32_t i, uint32_t &up0, uint32_t &up1,
+                              uint32_t &pv0, uint32_t &pv1) {
+    const uint32_t ls = slot_of(i), ry = row_of_run[i];
+    const uint32_t c = cols[i], lo = c & 0xFFFFu, hi = c >> 16;
+    const uint16_t *rows = lrows + ls * (kTileMaxRows + 1);
+    up0 = up1 = pv0 = pv1 = 0;
+    if (ry > 0)
+      up0 = span(rows[ry - 1], rows[ry], lo, hi, up1);
+    if (!a.two_d && ls > 0 && linked[ls]) {
+      const uint16_t *prev = lrows + (ls - 1) * (kTileMaxRows + 1);
+      pv0 = span(prev[ry], prev[ry + 1], lo, hi, pv1);
+    }
+  };
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    uint32_t up0, up1, pv0, pv1;
+    neighbours(i, up0, up1, pv0, pv1);
+    uint32_t low = i;
+    if (up1 > up0)
+      low = up0;
+    if (pv1 > pv0 && pv0 < low)
+      low = pv0;
+    lp[i] = low;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    uint32_t up0, up1, pv0, pv1;
+    neighbours(i, up0, up1, pv0, pv1);
+    const uint32_t low = (up1 > up0 && !(pv1 > pv0 && pv0 < up0)) ? up0
+                         : (pv1 > pv0)                            ? pv0
+                                                                  : i;
+    for (uint32_t u = up0; u < up1; u++)
+      if (u != low)
+        unite_local(lp, i, u);
+    for (uint32_t u = pv0; u < pv1; u++)
+      if (u != low)
+        unite_local(lp, i, u);
+  }
+  __syncthreads();
+  // local roots (every path compressed), the last slot each local component
+  // reaches (in `cols`, free now), then each run's parent and, for local
+  // roots, that last slot for resolve_runs to raise the component's by
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    uint32_t root = i;
+    while (lp[root] != root)
+      root = lp[root];
+    lp[i] = root;
+    cols[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    atomicMax(&cols[lp[i]], s_lo + slot_of(i));
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t root = lp[i];
+    const uint32_t s = slot_of(i), q = slot_of(root);
+    const uint32_t g = slot_first[s] + (i - slot_off[s]);
+    parent[g] = slot_first[q] + (root - slot_off[q]);
+    last_slot[g] = root == i ? cols[i] : 0u;
+  }
+}
+
+// The edges the tiles left, on the global forest (a run at a time): up edges from a tile's first row, previous-frame edges from a
+// tile's first frame (which includes those to the carried frontier), and
+// every edge of a spilled tile.
+__global__ __launch_bounds__(256) void unite_cross(
     LabelArgs a, const ChunkDesc *frames, const uint32_t *run_idx,
     const uint32_t *run_last, const uint32_t *run_slot,
-    const uint32_t *run_rows, const uint32_t *n_runs, uint32_t *parent) {
+    const uint32_t *run_rows, const uint32_t *n_runs, const uint32_t *spill,
+    uint32_t *parent) {
   const uint32_t stride = a.pixels / a.width + 1;
   const uint32_t runs = *n_runs;
+  const Tiling t = tiling_for(a, runs);
   for (uint32_t r = xcd_local_item(blockIdx.x, gridDim.x) * blockDim.x +
                     threadIdx.x + a.n_carry;
        r < runs; r += gridDim.x * blockDim.x) {
-    const int k = (int)run_slot[r] - 1;
+    const int k = (int)run_slot[r] - 1; // chunk frame
     const uint32_t idx = run_idx[r], last = run_last[r];
     const uint32_t y = row_of(a, idx);
-    const uint32_t *rows = run_rows + (size_t)(k + 1) * stride;
-    if (y > 0)
-      unite_row(parent, run_idx, run_last, r, rows[y - 1], rows[y],
+    const uint32_t group = (uint32_t)k / t.frames;
+    const bool spilled = spill[group * t.bands + y / t.rows] != 0;
+    const uint32_t *mine = run_rows + (size_t)(k + 1) * stride;
+    if (y > 0 && (spilled || y % t.rows == 0))
+      unite_row(parent, run_idx, run_last, r, mine[y - 1], mine[y],
                 idx - a.width, last - a.width);
-    if (!a.two_d && frames[k].linked) {
+    if (!a.two_d && frames[k].linked &&
+        (spilled || (uint32_t)k % t.frames == 0)) {
       const uint32_t *prev = run_rows + (size_t)k * stride;
       unite_row(parent, run_idx, run_last, r, prev[y], prev[y + 1], idx,
                 last);
@@ -481,7 +741,8 @@ __global__ __launch_bounds__(256) void unite_runs(
 // whose first pixel is the component's first pixel) with a read-only walk.
 // (Writing roots back into `parent` while other threads still walk it would
 // let a path-halving write replace a finished root with an intermediate
-// ancestor.) A run lies on one frame: one last_slot update per run.
+// ancestor.) Each run then raises its root's last slot by what it carries
+// (one atomic per tile-local component, not per run).
 __global__ void resolve_runs(const uint32_t *parent, const uint32_t *run_slot,
                              const uint32_t *n_runs, uint32_t *root,
                              uint32_t *last_slot) {
@@ -492,8 +753,12 @@ __global__ void resolve_runs(const uint32_t *parent, const uint32_t *run_slot,
     while (parent[p] != p)
       p = parent[p];
     root[r] = p;
-    if (run_slot[r])
-      atomicMax(&last_slot[p], run_slot[r]);
+    // last_slot holds what run r contributes: its own slot, or (for a root
+    // of a component united in a tile) that component's last slot, or 0;
+    // only roots are raised, so a non-root's entry is read undisturbed
+    const uint32_t last = last_slot[r];
+    if (p != r && last)
+      atomicMax(&last_slot[p], last);
   }
 }
 
@@ -640,76 +905,6 @@ __device__ __forceinline__ void for_members(uint32_t s0, uint32_t n,
   }
 }
 
-// One component by one thread: the reference's emit() (src/dials_spots.cc:
-// 221-360) operation for operation, every sum in member order.
-__device__ int emit_sequential(const LabelArgs &a, uint32_t s0, uint32_t n,
-                               const int32_t *mem_frame,
-                               const uint32_t *mem_index,
-                               const uint32_t *mem_value, sf_spot &spot) {
-  spot.n_signal = n;
-  double sum = 0.0, sum_sq = 0.0;
-  double weighted[3] = {0.0, 0.0, 0.0};
-  int32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
-  uint32_t peak_value = 0;
-  double peak[3] = {0.0, 0.0, 0.0};
-  for_members(s0, n, mem_frame, mem_index, mem_value,
-              [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
-                const uint32_t row = row_of(a, idx);
-                const int32_t x = (int32_t)(idx - row * a.width);
-                const int32_t y = (int32_t)row;
-                if (k == 0) {
-                  x0 = x;
-                  x1 = x + 1;
-                  y0 = y;
-                  y1 = y + 1;
-                  z0 = z;
-                  z1 = z + 1;
-                } else {
-                  x0 = x < x0 ? x : x0;
-                  x1 = x + 1 > x1 ? x + 1 : x1;
-                  y0 = y < y0 ? y : y0;
-                  y1 = y + 1 > y1 ? y + 1 : y1;
-                  z0 = z < z0 ? z : z0;
-                  z1 = z + 1 > z1 ? z + 1 : z1;
-                }
-                const double value = (double)v;
-                const double position[3] = {(double)x + 0.5, (double)y + 0.5,
-                                            (double)z + 0.5};
-                sum += value;
-                sum_sq += value * value;
-                for (int axis = 0; axis < 3; axis++)
-                  weighted[axis] += value * position[axis];
-                if (v > peak_value) {
-                  peak_value = v;
-                  peak[0] = position[0];
-                  peak[1] = position[1];
-                  peak[2] = position[2];
-                }
-              });
-  spot.bbox[0] = x0;
-  spot.bbox[1] = x1;
-  spot.bbox[2] = y0;
-  spot.bbox[3] = y1;
-  spot.bbox[4] = z0;
-  spot.bbox[5] = z1;
-  return finish_spot(
-      a, spot, sum, sum_sq, weighted, peak,
-      [&](const double (&centre)[3], double (&delta_sq)[3]) {
-        for_members(s0, n, mem_frame, mem_index, mem_value,
-                    [&](uint32_t, uint32_t idx, uint32_t v, int32_t z) {
-                      const uint32_t row = row_of(a, idx);
-                      const double value = (double)v;
-                      const double position[3] = {
-                          (double)(idx - row * a.width) + 0.5,
-                          (double)row + 0.5, (double)z + 0.5};
-                      for (int axis = 0; axis < 3; axis++) {
-                        const double delta = position[axis] - centre[axis];
-                        delta_sq[axis] += value * delta * delta;
-                      }
-                    });
-      });
-}
-
 // The closed members' frame, index and value in emission order, gathered
 // once, one thread per node: node g of run r is member
 // (first member of r's sorted place) + (g - r's head).
@@ -753,59 +948,78 @@ __device__ void store_tallies(const uint32_t (&tally)[4], uint32_t *rows) {
     rows[4 * blockIdx.x + threadIdx.x] = block_tally[threadIdx.x];
 }
 
-// Components of up to this many members are emitted one to a lane.
-constexpr uint32_t kLaneMembers = 32;
+// What the centroid pass knows of a component before its last arithmetic:
+// the sums over its members in member order (src/dials_spots.cc:233-296),
+// its variance sums (src/dials_spots.cc:304-316), its bounding box, peak
+// and size, and whether it was rejected for its size.
+struct CompSums {
+  double sum, sum_sq;
+  double weighted[3];
+  double peak[3];
+  double delta_sq[3];
+  int32_t bbox[6];
+  uint32_t n;
+  int32_t kind; // kTooSmall or kTooLarge, or -1 to be finished
+};
 
-// The closed components, one to a lane, in emission order: too small, too
-// large, or at most kLaneMembers members and emitted by the reference's own
-// arithmetic (emit_sequential). Larger ones are listed for finalise_wide.
-__global__ __launch_bounds__(256) void finalise_narrow(
-    LabelArgs a, const uint32_t *starts, const uint32_t *n_starts,
-    const uint32_t *n_closed, const int32_t *mem_frame,
-    const uint32_t *mem_index, const uint32_t *mem_value, SpotOut *out,
-    unsigned long long *spot_pack, uint32_t *wide, uint32_t *n_wide,
-    uint32_t *tallies) {
-  const uint32_t comps = *n_starts;
-  const uint32_t end_all = *n_closed;
-  const uint32_t lane = lane_id();
-  uint32_t tally[4] = {0, 0, 0, 0};
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
-       c += gridDim.x * blockDim.x) {
-    const uint32_t s0 = starts[c];
-    const uint32_t n = (c + 1 < comps ? starts[c + 1] : end_all) - s0;
-    const bool later =
-        n >= a.min_size && n <= a.max_size && n > kLaneMembers;
-    const uint64_t mask = ballot(later);
-    if (mask) {
-      const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
-      uint32_t base = 0;
-      if (lane == leader)
-        base = atomicAdd(n_wide, (uint32_t)__builtin_popcountll(mask));
-      base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
-      if (later)
-        wide[base + count_below(mask)] = c;
-    }
-    if (later)
-      continue;
-    SpotOut r;
-    memset(&r, 0, sizeof(r));
-    int kind;
-    if (n < a.min_size)
-      kind = kTooSmall;
-    else if (n > a.max_size)
-      kind = kTooLarge;
-    else
-      kind = emit_sequential(a, s0, n, mem_frame, mem_index, mem_value,
-                             r.spot);
-    if (kind == kAccepted) {
-      r.accepted = 1;
-      r.members = n;
-    }
-    out[c] = r;
-    spot_pack[c] = ((unsigned long long)r.accepted << 32) | r.members;
-    tally[kind]++;
+// The sums of the reference's emit() (src/dials_spots.cc:233-296) for one
+// lane, operation for operation: sums, bounding box and peak in member
+// order.
+__device__ void sequential_sums(const LabelArgs &a, uint32_t s0, uint32_t n,
+                                const int32_t *mem_frame,
+                                const uint32_t *mem_index,
+                                const uint32_t *mem_value, CompSums &cs) {
+  double sum = 0.0, sum_sq = 0.0;
+  double weighted[3] = {0.0, 0.0, 0.0};
+  int32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
+  uint32_t peak_value = 0;
+  double peak[3] = {0.0, 0.0, 0.0};
+  for_members(s0, n, mem_frame, mem_index, mem_value,
+              [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
+                const uint32_t row = row_of(a, idx);
+                const int32_t x = (int32_t)(idx - row * a.width);
+                const int32_t y = (int32_t)row;
+                if (k == 0) {
+                  x0 = x;
+                  x1 = x + 1;
+                  y0 = y;
+                  y1 = y + 1;
+                  z0 = z;
+                  z1 = z + 1;
+                } else {
+                  x0 = x < x0 ? x : x0;
+                  x1 = x + 1 > x1 ? x + 1 : x1;
+                  y0 = y < y0 ? y : y0;
+                  y1 = y + 1 > y1 ? y + 1 : y1;
+                  z0 = z < z0 ? z : z0;
+                  z1 = z + 1 > z1 ? z + 1 : z1;
+                }
+                const double value = (double)v;
+                const double position[3] = {(double)x + 0.5, (double)y + 0.5,
+                                            (double)z + 0.5};
+                sum += value;
+                sum_sq += value * value;
+                for (int axis = 0; axis < 3; axis++)
+                  weighted[axis] += value * position[axis];
+                if (v > peak_value) {
+                  peak_value = v;
+                  peak[0] = position[0];
+                  peak[1] = position[1];
+                  peak[2] = position[2];
+                }
+              });
+  cs.sum = sum;
+  cs.sum_sq = sum_sq;
+  for (int axis = 0; axis < 3; axis++) {
+    cs.weighted[axis] = weighted[axis];
+    cs.peak[axis] = peak[axis];
   }
-  store_tallies(tally, tallies);
+  cs.bbox[0] = x0;
+  cs.bbox[1] = x1;
+  cs.bbox[2] = y0;
+  cs.bbox[3] = y1;
+  cs.bbox[4] = z0;
+  cs.bbox[5] = z1;
 }
 
 // Variance terms staged per round, and the stride of each axis's row (two
@@ -813,39 +1027,45 @@ __global__ __launch_bounds__(256) void finalise_narrow(
 constexpr uint32_t kTermRound = 256;
 constexpr uint32_t kTermStride = kTermRound + 2;
 
-// One wave per listed component, with the result the sequential emit()
-// gives bit for bit:
+// One wave per closed component, in emission order: everything of the
+// reference's emit() (src/dials_spots.cc:221-330) that reads the members,
+// bit for bit:
 //  * bbox, peak (first strict maximum in member order) and the sums are
 //    taken across the wave. The reference sums in double in member order;
 //    when every term and partial sum is an integer (or half-integer) below
 //    2^53 -- always so for 16-bit frames -- those double sums are exact and
 //    equal the sums of the same integers taken in any order, which is what
-//    the lanes and the DPP reduction add here. Otherwise one lane runs the
-//    sequential emit();
+//    the lanes and the DPP reduction add here. Otherwise one lane takes the
+//    sums in member order;
 //  * the variance terms v (p - c)^2 are formed in parallel and staged in
 //    LDS, then lanes 0, 1 and 2 each add one axis's terms in member order,
-//    as src/dials_spots.cc:304-316 adds them.
-__global__ __launch_bounds__(256) void finalise_wide(
+//    as src/dials_spots.cc:304-316 adds them, the next terms read ahead of
+//    the adds.
+// The arithmetic that follows the sums is finish_components'.
+__global__ __launch_bounds__(256) void component_sums(
     LabelArgs a, const uint32_t *starts, const uint32_t *n_starts,
     const uint32_t *n_closed, const int32_t *mem_frame,
-    const uint32_t *mem_index, const uint32_t *mem_value,
-    const uint32_t *wide, const uint32_t *n_wide, SpotOut *out,
-    unsigned long long *spot_pack, uint32_t *tallies) {
+    const uint32_t *mem_index, const uint32_t *mem_value, CompSums *sums) {
   __shared__ double terms[4][3 * kTermStride];
-  const uint32_t count = *n_wide;
   const uint32_t comps = *n_starts;
   const uint32_t end_all = *n_closed;
   const uint32_t lane = lane_id();
   double *mine = terms[threadIdx.x >> 6];
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  uint32_t tally[4] = {0, 0, 0, 0};
-  for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-       w < count; w += waves) {
-    const uint32_t c = wide[w];
+  for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       c < comps; c += waves) {
     const uint32_t s0 = starts[c];
     const uint32_t n = (c + 1 < comps ? starts[c + 1] : end_all) - s0;
-    SpotOut r;
-    memset(&r, 0, sizeof(r));
+    CompSums cs;
+    memset(&cs, 0, sizeof(cs));
+    cs.n = n;
+    if (n < a.min_size || n > a.max_size) {
+      cs.kind = n < a.min_size ? kTooSmall : kTooLarge;
+      if (lane == 0)
+        sums[c] = cs;
+      continue;
+    }
+    cs.kind = -1;
     // the first round's members in registers: lane + 64 j, j < 4
     uint32_t m_idx[4], m_val[4];
     int32_t m_z[4];
@@ -856,7 +1076,7 @@ __global__ __launch_bounds__(256) void finalise_wide(
       m_val[j] = k < n ? mem_value[s0 + k] : 0;
       m_z[j] = k < n ? mem_frame[s0 + k] : 0;
     }
-    // pass 1: bbox, peak, sums, bounds
+    // bbox, peak, sums, and the bounds of the sums' exactness
     uint32_t x0 = 0xFFFFFFFFu, x1 = 0, y0 = 0xFFFFFFFFu, y1 = 0;
     int32_t z0 = INT32_MAX, z1 = INT32_MIN;
     double sv = 0.0, svv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
@@ -899,7 +1119,6 @@ __global__ __launch_bounds__(256) void finalise_wide(
     zabs = wave_max_u32(zabs);
     const uint32_t vtop = wave_max_u32(vmax);
     const uint32_t ktop = wave_min_u32(vmax == vtop ? kmax : 0xFFFFFFFFu);
-    sf_spot &spot = r.spot;
     // Exactness of the reference's double sums: every term and partial
     // sum below 2^52 (with margin) in magnitude.
     const double bound = 4503599627370496.0; // 2^52
@@ -907,97 +1126,131 @@ __global__ __launch_bounds__(256) void finalise_wide(
     const double reach =
         (double)max(max(a.width, a.pixels / a.width), zabs) * 2.0 + 1.0;
     const bool exact = cnt * top * top < bound && cnt * top * reach < bound;
-    int kind;
-    if (!exact) {
-      kind = kAccepted;
-      if (lane == 0)
-        kind = emit_sequential(a, s0, n, mem_frame, mem_index, mem_value,
-                               spot);
-      kind = __builtin_amdgcn_readlane(kind, 0);
-    } else {
-      const double sum = wave_sum_f64(sv);
-      const double sum_sq = wave_sum_f64(svv);
-      const double weighted[3] = {wave_sum_f64(sx) * 0.5,
-                                  wave_sum_f64(sy) * 0.5,
-                                  wave_sum_f64(sz) * 0.5};
-      spot.n_signal = n;
-      spot.bbox[0] = (int32_t)x0;
-      spot.bbox[1] = (int32_t)x1;
-      spot.bbox[2] = (int32_t)y0;
-      spot.bbox[3] = (int32_t)y1;
-      spot.bbox[4] = z0;
-      spot.bbox[5] = z1;
-      double peak[3] = {0.0, 0.0, 0.0};
+    if (exact) {
+      cs.sum = wave_sum_f64(sv);
+      cs.sum_sq = wave_sum_f64(svv);
+      cs.weighted[0] = wave_sum_f64(sx) * 0.5;
+      cs.weighted[1] = wave_sum_f64(sy) * 0.5;
+      cs.weighted[2] = wave_sum_f64(sz) * 0.5;
+      cs.bbox[0] = (int32_t)x0;
+      cs.bbox[1] = (int32_t)x1;
+      cs.bbox[2] = (int32_t)y0;
+      cs.bbox[3] = (int32_t)y1;
+      cs.bbox[4] = z0;
+      cs.bbox[5] = z1;
       if (vtop > 0) {
         const uint32_t idx = mem_index[s0 + ktop];
         const uint32_t y = row_of(a, idx);
-        peak[0] = (double)(idx - y * a.width) + 0.5;
-        peak[1] = (double)y + 0.5;
-        peak[2] = (double)mem_frame[s0 + ktop] + 0.5;
+        cs.peak[0] = (double)(idx - y * a.width) + 0.5;
+        cs.peak[1] = (double)y + 0.5;
+        cs.peak[2] = (double)mem_frame[s0 + ktop] + 0.5;
       }
-      kind = finish_spot(
-          a, spot, sum, sum_sq, weighted, peak,
-          [&](const double (&centre)[3], double (&delta_sq)[3]) {
-            double acc = 0.0; // lane a < 3: axis a, in member order
-            for (uint32_t k0 = 0; k0 < n; k0 += kTermRound) {
+    } else {
+      if (lane == 0)
+        sequential_sums(a, s0, n, mem_frame, mem_index, mem_value, cs);
+      cs.sum = readlane_f64(cs.sum, 0);
+      for (int axis = 0; axis < 3; axis++)
+        cs.weighted[axis] = readlane_f64(cs.weighted[axis], 0);
+    }
+    if (cs.sum > 0.0) {
+      // the centroid as finish_spot forms it; then the terms
+      double centre[3];
+      for (int axis = 0; axis < 3; axis++)
+        centre[axis] = cs.weighted[axis] / cs.sum;
+      double acc = 0.0; // lane a < 3: axis a, in member order
+      for (uint32_t k0 = 0; k0 < n; k0 += kTermRound) {
 #pragma unroll
-              for (int j = 0; j < 4; j++) {
-                const uint32_t k = k0 + lane + 64u * j;
-                uint32_t idx = m_idx[j], v = m_val[j];
-                int32_t z = m_z[j];
-                if (k0 > 0) {
-                  idx = k < n ? mem_index[s0 + k] : 0;
-                  v = k < n ? mem_value[s0 + k] : 0;
-                  z = k < n ? mem_frame[s0 + k] : 0;
-                }
-                const uint32_t y = row_of(a, idx);
-                const double value = (double)v;
-                const double position[3] = {
-                    (double)(idx - y * a.width) + 0.5, (double)y + 0.5,
-                    (double)z + 0.5};
-                for (int axis = 0; axis < 3; axis++) {
-                  const double delta = position[axis] - centre[axis];
-                  mine[axis * kTermStride + lane + 64u * j] =
-                      value * delta * delta;
-                }
-              }
-              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-              __builtin_amdgcn_wave_barrier();
-              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-              if (lane < 3) {
-                // eight terms read ahead of their eight dependent adds
-                const uint32_t m = min(kTermRound, n - k0);
-                const double *row = mine + lane * kTermStride;
-                uint32_t i = 0;
-                for (; i + 8 <= m; i += 8) {
-                  double r8[8];
+        for (int j = 0; j < 4; j++) {
+          const uint32_t k = k0 + lane + 64u * j;
+          uint32_t idx = m_idx[j], v = m_val[j];
+          int32_t z = m_z[j];
+          if (k0 > 0) {
+            idx = k < n ? mem_index[s0 + k] : 0;
+            v = k < n ? mem_value[s0 + k] : 0;
+            z = k < n ? mem_frame[s0 + k] : 0;
+          }
+          const uint32_t y = row_of(a, idx);
+          const double value = (double)v;
+          const double position[3] = {(double)(idx - y * a.width) + 0.5,
+                                      (double)y + 0.5, (double)z + 0.5};
+          for (int axis = 0; axis < 3; axis++) {
+            const double delta = position[axis] - centre[axis];
+            mine[axis * kTermStride + lane + 64u * j] = value * delta * delta;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < 3) {
+          // eight terms read ahead of the eight dependent adds before them
+          const uint32_t m = min(kTermRound, n - k0);
+          const double *row = mine + lane * kTermStride;
+          const uint32_t full = m & ~7u;
+          double cur[8];
 #pragma unroll
-                  for (int q = 0; q < 8; q++)
-                    r8[q] = row[i + q];
+          for (int q = 0; q < 8; q++)
+            cur[q] = full ? row[q] : 0.0;
+          uint32_t i = 0;
+          for (; i < full; i += 8) {
+            double next[8];
+            const bool more = i + 8 < full;
 #pragma unroll
-                  for (int q = 0; q < 8; q++)
-                    acc += r8[q];
-                }
-                for (; i < m; i++)
-                  acc += row[i];
-              }
-              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-              __builtin_amdgcn_wave_barrier();
-              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
-            for (int axis = 0; axis < 3; axis++)
-              delta_sq[axis] = readlane_f64(acc, axis);
-          });
+            for (int q = 0; q < 8; q++)
+              next[q] = more ? row[i + 8 + q] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+              acc += cur[q];
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+              cur[q] = next[q];
+          }
+          for (; i < m; i++)
+            acc += row[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      for (int axis = 0; axis < 3; axis++)
+        cs.delta_sq[axis] = readlane_f64(acc, axis);
+    }
+    if (lane == 0)
+      sums[c] = cs;
+  }
+}
+
+// The rest of emit() (src/dials_spots.cc:297-360) one component a lane:
+// centroid, variance, the single-frame z rule, the separation test; the
+// spot row and the tallies.
+__global__ __launch_bounds__(256) void finish_components(
+    LabelArgs a, const uint32_t *n_starts, const CompSums *sums,
+    SpotOut *out, unsigned long long *spot_pack, uint32_t *tallies) {
+  const uint32_t comps = *n_starts;
+  uint32_t tally[4] = {0, 0, 0, 0};
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
+       c += gridDim.x * blockDim.x) {
+    const CompSums cs = sums[c];
+    SpotOut r;
+    memset(&r, 0, sizeof(r));
+    int kind = cs.kind;
+    if (kind < 0) {
+      sf_spot &spot = r.spot;
+      spot.n_signal = cs.n;
+      for (int q = 0; q < 6; q++)
+        spot.bbox[q] = cs.bbox[q];
+      kind = finish_spot(a, spot, cs.sum, cs.sum_sq, cs.weighted, cs.peak,
+                         [&](const double (&)[3], double (&delta_sq)[3]) {
+                           for (int axis = 0; axis < 3; axis++)
+                             delta_sq[axis] = cs.delta_sq[axis];
+                         });
     }
     if (kind == kAccepted) {
       r.accepted = 1;
-      r.members = n;
+      r.members = cs.n;
     }
-    if (lane == 0) {
-      out[c] = r;
-      spot_pack[c] = ((unsigned long long)r.accepted << 32) | r.members;
-      tally[kind]++;
-    }
+    out[c] = r;
+    spot_pack[c] = ((unsigned long long)r.accepted << 32) | r.members;
+    tally[kind]++;
   }
   store_tallies(tally, tallies);
 }
@@ -1132,6 +1385,147 @@ __global__ __launch_bounds__(64) void add_signal_count(
     counts[0] += total;
 }
 
+// ---------------------------------------------------------------------------
+// A stable LSD radix sort of (key, value) pairs, for the component sort's
+// 32-bit keys: three passes of 9-bit digits. A pass is a histogram per tile
+// of 4096 pairs, one block turning the histograms into each tile's first
+// place per digit, and a scatter that ranks each tile's pairs per digit in
+// input order (a wave's 64 pairs by ballot matches of their digits, the
+// waves of a tile and the rounds of a wave in order).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSortBits = 9;
+constexpr uint32_t kSortBuckets = 1u << kSortBits;
+constexpr uint32_t kSortTile = 4096; // pairs a block takes: 4 waves x 1024
+
+__global__ __launch_bounds__(256) void sort_histogram(const uint32_t *keys,
+                                                      uint32_t n, int shift,
+                                                      uint32_t *hist,
+                                                      uint32_t tiles) {
+  __shared__ uint32_t counts[kSortBuckets];
+  for (uint32_t d = threadIdx.x; d < kSortBuckets; d += blockDim.x)
+    counts[d] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (uint32_t k = threadIdx.x; k < kSortTile; k += blockDim.x)
+    if (base + k < n)
+      atomicAdd(&counts[(keys[base + k] >> shift) & (kSortBuckets - 1)], 1u);
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < kSortBuckets; d += blockDim.x)
+    hist[d * tiles + blockIdx.x] = counts[d];
+}
+
+// Each digit's row of hist (digit-major, tiles per digit) becomes each
+// tile's first place within the digit, and the digit's total goes to
+// totals[d]: a wave per digit.
+__global__ __launch_bounds__(256) void sort_offsets(uint32_t *hist,
+                                                    uint32_t tiles,
+                                                    uint32_t *totals) {
+  const uint32_t lane = lane_id();
+  const uint32_t d = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (d >= kSortBuckets)
+    return;
+  uint32_t run = 0;
+  for (uint32_t t0 = 0; t0 < tiles; t0 += 64) {
+    const uint32_t t = t0 + lane;
+    const uint32_t v = t < tiles ? hist[d * tiles + t] : 0u;
+    const uint32_t incl = wave_inclusive(v);
+    if (t < tiles)
+      hist[d * tiles + t] = run + incl - v;
+    run += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+  }
+  if (lane == 0)
+    totals[d] = run;
+}
+
+// The lanes of this round with the same digit as this lane (all lanes
+// active; `live` marks the lanes that hold a pair).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t digit,
+                                                uint64_t live) {
+  uint64_t mask = live;
+#pragma unroll
+  for (uint32_t b = 0; b < kSortBits; b++) {
+    const uint64_t set = ballot((digit >> b) & 1u);
+    mask &= ((digit >> b) & 1u) ? set : ~set;
+  }
+  return mask;
+}
+
+__global__ __launch_bounds__(256) void sort_scatter(
+    const uint32_t *keys_in, const uint32_t *values_in, uint32_t n,
+    int shift, const uint32_t *offsets, const uint32_t *totals,
+    uint32_t tiles, uint32_t *keys_out, uint32_t *values_out) {
+  __shared__ uint32_t start[4][kSortBuckets]; // per wave and digit
+  __shared__ uint32_t base[kSortBuckets];     // each digit's first place
+  __shared__ uint32_t wave_sum[4];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  { // the digits' first places: an exclusive prefix of their totals, two
+    // digits a thread
+    const uint32_t v0 = totals[2 * threadIdx.x], v1 = totals[2 * threadIdx.x + 1];
+    const uint32_t incl = wave_inclusive(v0 + v1);
+    if (lane == 63)
+      wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; w++)
+      before += wave_sum[w];
+    base[2 * threadIdx.x] = before + incl - (v0 + v1);
+    base[2 * threadIdx.x + 1] = before + incl - v1;
+  }
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t d = lane; d < kSortBuckets; d += 64)
+    start[wave][d] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t first = blockIdx.x * kSortTile + wave * (kSortTile / 4);
+  // this wave's pairs per digit
+  for (uint32_t r = 0; r < kSortTile / 4; r += 64) {
+    const uint32_t i = first + r + lane;
+    const bool live = i < n;
+    const uint32_t digit =
+        live ? (keys_in[i] >> shift) & (kSortBuckets - 1) : 0u;
+    const uint64_t peers = digit_peers(digit, ballot(live));
+    if (live && (peers & below) == 0)
+      start[wave][digit] += (uint32_t)__builtin_popcountll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  __syncthreads();
+  // each wave's first place per digit: the tile's, after earlier waves'
+  for (uint32_t d = threadIdx.x; d < kSortBuckets; d += blockDim.x) {
+    uint32_t run = base[d] + offsets[d * tiles + blockIdx.x];
+    for (uint32_t w = 0; w < 4; w++) {
+      const uint32_t c = start[w][d];
+      start[w][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t r = 0; r < kSortTile / 4; r += 64) {
+    const uint32_t i = first + r + lane;
+    const bool live = i < n;
+    const uint32_t key = live ? keys_in[i] : 0u;
+    const uint32_t value = live ? values_in[i] : 0u;
+    const uint32_t digit = (key >> shift) & (kSortBuckets - 1);
+    const uint64_t peers = digit_peers(digit, ballot(live));
+    const uint32_t at =
+        start[wave][digit] + (uint32_t)__builtin_popcountll(peers & below);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (live) {
+      keys_out[at] = key;
+      values_out[at] = value;
+      if ((peers & below) == 0)
+        start[wave][digit] += (uint32_t)__builtin_popcountll(peers);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 } // namespace sf
 
 // ---------------------------------------------------------------------------
@@ -1221,6 +1615,7 @@ struct sf_labeller {
   sf::DevArray<uint32_t> run_head, run_idx, run_last, run_slot, parent, root,
       last_slot, pos_of;
   sf::DevArray<uint32_t> run_rows; // per (slot, row): first run
+  sf::DevArray<uint32_t> tile_spill; // per tile of unite_tiles: left over
   sf::DevArray<uint64_t> keys, keys_sorted;
   sf::DevArray<uint32_t> values, order, open_len, open_at;
   sf::DevArray<unsigned long long> run_pack, run_rank;
@@ -1228,6 +1623,8 @@ struct sf_labeller {
   sf::DevArray<uint32_t> starts, comp_mem;
   sf::DevArray<uint32_t> tallies; // per centroid-pass block: kinds emitted
   sf::DevArray<sf::SpotOut> results;
+  sf::DevArray<sf::CompSums> comp_sums;
+  sf::DevArray<uint32_t> sort_hist; // per digit and tile of the run sort
   sf::DevArray<unsigned long long> spot_pack, spot_rank;
   sf::DevArray<uint8_t> temp;
   sf::DevArray<sf::ChunkDesc> chunk;
@@ -1421,20 +1818,11 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(l->carry_value[outside].ensure(n), "alloc");
   LAB_CHECK(l->carry_root[outside].ensure(n), "alloc");
   LAB_CHECK(l->chunk.ensure(std::max<uint32_t>(n_frames, 1)), "alloc");
-  // Sort keys: 32 bits when (last slot, root run) fit, else 64. The runs
-  // are at most the nodes.
-  int root_bits = 1, slot_bits = 1;
-  while (root_bits < 32 && (1ull << root_bits) < (uint64_t)n)
-    root_bits++;
-  while ((1ull << slot_bits) < (uint64_t)n_frames + 2)
-    slot_bits++;
-  const bool narrow = root_bits + slot_bits <= 32;
   uint32_t *keys32 = reinterpret_cast<uint32_t *>(l->keys.ptr);
   uint32_t *keys32_sorted = reinterpret_cast<uint32_t *>(l->keys_sorted.ptr);
-  // d_scalars: components, closed members, closed runs, components left to
-  // the wave-per-component pass, runs
+  // d_scalars: components, closed members, closed runs, (unused), runs
   uint32_t *d_comps = l->d_scalars, *d_closed_mem = l->d_scalars + 1,
-           *d_closed_runs = l->d_scalars + 2, *n_wide = l->d_scalars + 3,
+           *d_closed_runs = l->d_scalars + 2,
            *n_runs = l->d_scalars + 4;
   LAB_CHECK(hipMemsetAsync(l->d_scalars, 0, 8 * sizeof(uint32_t), s),
             "memset");
@@ -1478,14 +1866,38 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
         l->run_rows.ptr);
   }
   // the run count is on the device: bounded grids that stride (a multiple of
-  // 8 blocks for the XCD mapping of unite_runs)
+  // 8 blocks for the XCD mapping of unite_cross)
   const uint32_t run_grid = std::min<uint32_t>((grid + 7) & ~7u, 4096);
-  sf::unite_runs<<<run_grid, 256, 0, s>>>(
-      a, l->chunk.ptr, l->run_idx.ptr, l->run_last.ptr, l->run_slot.ptr,
-      l->run_rows.ptr, n_runs, l->parent.ptr);
+  {
+    // tiles: at most ceil(height / kTileMinRows) bands of every frame group
+    const uint32_t height = (uint32_t)l->height;
+    const uint32_t bands =
+        (height + sf::kTileMinRows - 1) / sf::kTileMinRows;
+    const uint32_t groups =
+        std::max<uint32_t>(1, (n_frames + sf::kTileFrames - 1) /
+                                  sf::kTileFrames);
+    LAB_CHECK(l->tile_spill.ensure((size_t)bands * groups), "alloc");
+    sf::unite_tiles<<<bands * groups, 1024, 0, s>>>(
+        a, l->chunk.ptr, l->run_idx.ptr, l->run_last.ptr, l->run_rows.ptr,
+        n_runs, l->parent.ptr, l->last_slot.ptr, l->tile_spill.ptr);
+    sf::unite_cross<<<run_grid, 256, 0, s>>>(
+        a, l->chunk.ptr, l->run_idx.ptr, l->run_last.ptr, l->run_slot.ptr,
+        l->run_rows.ptr, n_runs, l->tile_spill.ptr, l->parent.ptr);
+  }
   sf::resolve_runs<<<run_grid, 256, 0, s>>>(l->parent.ptr, l->run_slot.ptr,
                                             n_runs, l->root.ptr,
                                             l->last_slot.ptr);
+  LAB_CHECK(hipGetLastError(), "label kernels");
+  // the keys, the sort and the scans take their sizes from the host
+  LAB_CHECK(hipEventSynchronize(l->runs_ready), "waiting for the run count");
+  const uint32_t runs = l->h_scalars[1];
+  // Sort keys: 32 bits when (last slot, root run) fit, else 64.
+  int root_bits = 1, slot_bits = 1;
+  while (root_bits < 32 && (1ull << root_bits) < (uint64_t)runs)
+    root_bits++;
+  while ((1ull << slot_bits) < (uint64_t)n_frames + 2)
+    slot_bits++;
+  const bool narrow = root_bits + slot_bits <= 32;
   if (narrow)
     sf::make_keys<uint32_t><<<run_grid, 256, 0, s>>>(
         a, l->root.ptr, l->last_slot.ptr, l->run_head.ptr, n_runs, keys32,
@@ -1494,10 +1906,6 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
     sf::make_keys<uint64_t><<<run_grid, 256, 0, s>>>(
         a, l->root.ptr, l->last_slot.ptr, l->run_head.ptr, n_runs,
         l->keys.ptr, l->values.ptr, l->open_len.ptr, 32);
-  LAB_CHECK(hipGetLastError(), "label kernels");
-  // the sort and the scans take their sizes from the host
-  LAB_CHECK(hipEventSynchronize(l->runs_ready), "waiting for the run count");
-  const uint32_t runs = l->h_scalars[1];
   // temp storage: the largest of the rocPRIM calls
   size_t need = 0, bytes = 0;
   if (narrow)
@@ -1523,10 +1931,37 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   const uint32_t rgrid = grid_for(runs);
   bytes = l->temp.cap;
   if (narrow) {
-    LAB_CHECK(sort32(l->temp.ptr, bytes, keys32, keys32_sorted,
-                     l->values.ptr, l->order.ptr, runs,
-                     root_bits + slot_bits, s),
-              "sorting components");
+    if (root_bits + slot_bits <= 3 * (int)sf::kSortBits) {
+      // three 9-bit passes: keys32 -> keys32_sorted -> keys32 ->
+      // keys32_sorted, the values likewise (the inputs are dead after)
+      const uint32_t sort_tiles = (runs + sf::kSortTile - 1) / sf::kSortTile;
+      LAB_CHECK(l->sort_hist.ensure((size_t)sf::kSortBuckets *
+                                    (std::max<uint32_t>(sort_tiles, 1) + 1)),
+                "alloc");
+      uint32_t *kin = keys32, *kout = keys32_sorted;
+      uint32_t *vin = l->values.ptr, *vout = l->order.ptr;
+      for (int pass = 0; pass < 3; pass++) {
+        const int shift = pass * (int)sf::kSortBits;
+        if (sort_tiles) {
+          sf::sort_histogram<<<sort_tiles, 256, 0, s>>>(
+              kin, runs, shift, l->sort_hist.ptr, sort_tiles);
+          uint32_t *totals =
+              l->sort_hist.ptr + (size_t)sf::kSortBuckets * sort_tiles;
+          sf::sort_offsets<<<sf::kSortBuckets / 4, 256, 0, s>>>(
+              l->sort_hist.ptr, sort_tiles, totals);
+          sf::sort_scatter<<<sort_tiles, 256, 0, s>>>(
+              kin, vin, runs, shift, l->sort_hist.ptr, totals, sort_tiles,
+              kout, vout);
+        }
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+      }
+    } else {
+      LAB_CHECK(sort32(l->temp.ptr, bytes, keys32, keys32_sorted,
+                       l->values.ptr, l->order.ptr, runs,
+                       root_bits + slot_bits, s),
+                "sorting components");
+    }
     // closed keys stay below 2^(root_bits + slot_bits) - 1, so the open key
     // (all ones) sorts last on those bits too
     sf::mark_runs<uint32_t><<<rgrid, 256, 0, s>>>(
@@ -1556,21 +1991,19 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
       a, d_closed_runs, l->run_of.ptr, l->pos_of.ptr, l->run_head.ptr,
       l->run_rank.ptr, l->node_frame.ptr, l->node_index.ptr,
       l->node_value.ptr, mem_frame, mem_index, mem_value);
-  // components past kLaneMembers members go to the wave-per-component pass,
-  // listed in `order` (no longer needed once the members are gathered); each
-  // pass loops over components on a bounded grid
-  uint32_t *wide = l->order.ptr;
-  const uint32_t narrow_grid = std::min<uint32_t>(rgrid, 1024);
-  const uint32_t wide_grid = std::min<uint32_t>(rgrid, 4096);
-  LAB_CHECK(l->tallies.ensure(4 * (narrow_grid + wide_grid)), "alloc");
-  sf::finalise_narrow<<<narrow_grid, 256, 0, s>>>(
+  // the centroid pass: a wave per component for what reads the members,
+  // then a lane per component for the rest; each loops over components on a
+  // bounded grid
+  const uint32_t sums_grid = std::min<uint32_t>(rgrid, 4096);
+  const uint32_t finish_grid = std::min<uint32_t>(rgrid, 1024);
+  LAB_CHECK(l->comp_sums.ensure(n), "alloc");
+  LAB_CHECK(l->tallies.ensure(4 * finish_grid), "alloc");
+  sf::component_sums<<<sums_grid, 256, 0, s>>>(
       a, l->comp_mem.ptr, d_comps, d_closed_mem, mem_frame, mem_index,
-      mem_value, l->results.ptr, l->spot_pack.ptr, wide, n_wide,
+      mem_value, l->comp_sums.ptr);
+  sf::finish_components<<<finish_grid, 256, 0, s>>>(
+      a, d_comps, l->comp_sums.ptr, l->results.ptr, l->spot_pack.ptr,
       l->tallies.ptr);
-  sf::finalise_wide<<<wide_grid, 256, 0, s>>>(
-      a, l->comp_mem.ptr, d_comps, d_closed_mem, mem_frame, mem_index,
-      mem_value, wide, n_wide, l->results.ptr, l->spot_pack.ptr,
-      l->tallies.ptr + 4 * narrow_grid);
   bytes = l->temp.cap;
   // components are at most the runs
   LAB_CHECK(rocprim::exclusive_scan(l->temp.ptr, bytes, l->spot_pack.ptr,
@@ -1583,7 +2016,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
       l->comp_mem.ptr, l->spot_rank.ptr, l->node_frame.ptr,
       l->node_index.ptr, l->node_value.ptr, l->d_totals, l->spots.ptr,
       l->pixels.ptr);
-  sf::close_chunk<<<1, 256, 0, s>>>(l->tallies.ptr, narrow_grid + wide_grid,
+  sf::close_chunk<<<1, 256, 0, s>>>(l->tallies.ptr, finish_grid,
                                     d_comps, l->spot_rank.ptr, l->results.ptr,
                                     l->d_counts, l->d_totals, l->hd_totals);
   bytes = l->temp.cap;
@@ -1808,6 +2241,8 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   l->comp_mem.release();
   l->tallies.release();
   l->results.release();
+  l->comp_sums.release();
+  l->sort_hist.release();
   l->spot_pack.release();
   l->spot_rank.release();
   l->temp.release();
