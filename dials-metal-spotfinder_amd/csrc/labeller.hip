@@ -1026,6 +1026,7 @@ __device__ void sequential_sums(const LabelArgs &a, uint32_t s0, uint32_t n,
 // doubles over, so the three chains read distinct banks).
 constexpr uint32_t kTermRound = 256;
 constexpr uint32_t kTermStride = kTermRound + 2;
+static_assert(kTermRound % 16 == 0, "the variance sums take sixteen terms at a time");
 
 // One wave per closed component, in emission order: everything of the
 // reference's emit() (src/dials_spots.cc:221-330) that reads the members,
@@ -1052,20 +1053,27 @@ __global__ __launch_bounds__(256) void component_sums(
   const uint32_t lane = lane_id();
   double *mine = terms[threadIdx.x >> 6];
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-       c < comps; c += waves) {
-    const uint32_t s0 = starts[c];
-    const uint32_t n = (c + 1 < comps ? starts[c + 1] : end_all) - s0;
-    CompSums cs;
-    memset(&cs, 0, sizeof(cs));
-    cs.n = n;
+  uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // each component's bounds are read one component ahead
+  uint32_t next_s0 = c < comps ? starts[c] : 0u;
+  uint32_t next_end = c + 1 < comps ? starts[c + 1] : end_all;
+  for (; c < comps; c += waves) {
+    const uint32_t s0 = next_s0;
+    const uint32_t n = next_end - s0;
+    if (c + waves < comps) {
+      next_s0 = starts[c + waves];
+      next_end = c + waves + 1 < comps ? starts[c + waves + 1] : end_all;
+    }
+    // fields are stored as soon as they are known (by lane 0, or lane
+    // `axis` for the variance sums); a rejected size needs only n and kind
+    CompSums *out = sums + c;
     if (n < a.min_size || n > a.max_size) {
-      cs.kind = n < a.min_size ? kTooSmall : kTooLarge;
-      if (lane == 0)
-        sums[c] = cs;
+      if (lane == 0) {
+        out->n = n;
+        out->kind = n < a.min_size ? kTooSmall : kTooLarge;
+      }
       continue;
     }
-    cs.kind = -1;
     // the first round's members in registers: lane + 64 j, j < 4
     uint32_t m_idx[4], m_val[4];
     int32_t m_z[4];
@@ -1080,7 +1088,8 @@ __global__ __launch_bounds__(256) void component_sums(
     uint32_t x0 = 0xFFFFFFFFu, x1 = 0, y0 = 0xFFFFFFFFu, y1 = 0;
     int32_t z0 = INT32_MAX, z1 = INT32_MIN;
     double sv = 0.0, svv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-    uint32_t vmax = 0, kmax = 0xFFFFFFFFu, zabs = 0;
+    uint32_t vmax = 0, kmax = 0xFFFFFFFFu, zabs = 0, imax = 0;
+    int32_t zmax = 0;
     auto visit = [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
       const uint32_t y = row_of(a, idx);
       const uint32_t x = idx - y * a.width;
@@ -1100,6 +1109,8 @@ __global__ __launch_bounds__(256) void component_sums(
       if (v > vmax) { // lanes see members in order, so first per lane
         vmax = v;
         kmax = k;
+        imax = idx;
+        zmax = z;
       }
     };
 #pragma unroll
@@ -1126,38 +1137,59 @@ __global__ __launch_bounds__(256) void component_sums(
     const double reach =
         (double)max(max(a.width, a.pixels / a.width), zabs) * 2.0 + 1.0;
     const bool exact = cnt * top * top < bound && cnt * top * reach < bound;
+    double sum, weighted[3];
     if (exact) {
-      cs.sum = wave_sum_f64(sv);
-      cs.sum_sq = wave_sum_f64(svv);
-      cs.weighted[0] = wave_sum_f64(sx) * 0.5;
-      cs.weighted[1] = wave_sum_f64(sy) * 0.5;
-      cs.weighted[2] = wave_sum_f64(sz) * 0.5;
-      cs.bbox[0] = (int32_t)x0;
-      cs.bbox[1] = (int32_t)x1;
-      cs.bbox[2] = (int32_t)y0;
-      cs.bbox[3] = (int32_t)y1;
-      cs.bbox[4] = z0;
-      cs.bbox[5] = z1;
-      if (vtop > 0) {
-        const uint32_t idx = mem_index[s0 + ktop];
+      sum = wave_sum_f64(sv);
+      const double sum_sq = wave_sum_f64(svv);
+      weighted[0] = wave_sum_f64(sx) * 0.5;
+      weighted[1] = wave_sum_f64(sy) * 0.5;
+      weighted[2] = wave_sum_f64(sz) * 0.5;
+      double peak[3] = {0.0, 0.0, 0.0};
+      if (vtop > 0) { // member ktop was lane (ktop mod 64)'s
+        const int owner = (int)(ktop & 63u);
+        const uint32_t idx =
+            (uint32_t)__builtin_amdgcn_readlane((int)imax, owner);
+        const int32_t z = __builtin_amdgcn_readlane(zmax, owner);
         const uint32_t y = row_of(a, idx);
-        cs.peak[0] = (double)(idx - y * a.width) + 0.5;
-        cs.peak[1] = (double)y + 0.5;
-        cs.peak[2] = (double)mem_frame[s0 + ktop] + 0.5;
+        peak[0] = (double)(idx - y * a.width) + 0.5;
+        peak[1] = (double)y + 0.5;
+        peak[2] = (double)z + 0.5;
+      }
+      if (lane == 0) {
+        out->sum = sum;
+        out->sum_sq = sum_sq;
+        for (int axis = 0; axis < 3; axis++) {
+          out->weighted[axis] = weighted[axis];
+          out->peak[axis] = peak[axis];
+        }
+        out->bbox[0] = (int32_t)x0;
+        out->bbox[1] = (int32_t)x1;
+        out->bbox[2] = (int32_t)y0;
+        out->bbox[3] = (int32_t)y1;
+        out->bbox[4] = z0;
+        out->bbox[5] = z1;
+        out->n = n;
+        out->kind = -1;
       }
     } else {
-      if (lane == 0)
+      CompSums cs; // lane 0's
+      if (lane == 0) {
+        memset(&cs, 0, sizeof(cs));
         sequential_sums(a, s0, n, mem_frame, mem_index, mem_value, cs);
-      cs.sum = readlane_f64(cs.sum, 0);
+        cs.n = n;
+        cs.kind = -1;
+        *out = cs;
+      }
+      sum = readlane_f64(cs.sum, 0);
       for (int axis = 0; axis < 3; axis++)
-        cs.weighted[axis] = readlane_f64(cs.weighted[axis], 0);
+        weighted[axis] = readlane_f64(cs.weighted[axis], 0);
     }
-    if (cs.sum > 0.0) {
+    double acc = 0.0; // lane a < 3: axis a's variance sum, in member order
+    if (sum > 0.0) {
       // the centroid as finish_spot forms it; then the terms
       double centre[3];
       for (int axis = 0; axis < 3; axis++)
-        centre[axis] = cs.weighted[axis] / cs.sum;
-      double acc = 0.0; // lane a < 3: axis a, in member order
+        centre[axis] = weighted[axis] / sum;
       for (uint32_t k0 = 0; k0 < n; k0 += kTermRound) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -1182,40 +1214,41 @@ __global__ __launch_bounds__(256) void component_sums(
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lane < 3) {
-          // eight terms read ahead of the eight dependent adds before them
-          const uint32_t m = min(kTermRound, n - k0);
+          // Sixteen terms at a time, two sets of eight: each set is read
+          // while the other is added. The round's slots past the last
+          // member hold +0.0 (a zero value times a finite square), and
+          // adding +0.0 to the sum, which is never -0.0, leaves it
+          // unchanged, so the last sixteen need no tail.
+          const uint32_t m = (min(kTermRound, n - k0) + 15u) & ~15u;
           const double *row = mine + lane * kTermStride;
-          const uint32_t full = m & ~7u;
-          double cur[8];
+          double p[8], q[8];
 #pragma unroll
-          for (int q = 0; q < 8; q++)
-            cur[q] = full ? row[q] : 0.0;
-          uint32_t i = 0;
-          for (; i < full; i += 8) {
-            double next[8];
-            const bool more = i + 8 < full;
+          for (int t = 0; t < 8; t++)
+            p[t] = row[t];
+          for (uint32_t i = 0; i < m; i += 16) {
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-              next[q] = more ? row[i + 8 + q] : 0.0;
+            for (int t = 0; t < 8; t++)
+              q[t] = row[i + 8 + t];
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-              acc += cur[q];
+            for (int t = 0; t < 8; t++)
+              acc += p[t];
+            if (i + 16 < m) {
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-              cur[q] = next[q];
+              for (int t = 0; t < 8; t++)
+                p[t] = row[i + 16 + t];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+              acc += q[t];
           }
-          for (; i < m; i++)
-            acc += row[i];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      for (int axis = 0; axis < 3; axis++)
-        cs.delta_sq[axis] = readlane_f64(acc, axis);
     }
-    if (lane == 0)
-      sums[c] = cs;
+    if (lane < 3)
+      out->delta_sq[lane] = acc;
   }
 }
 
@@ -1388,14 +1421,14 @@ __global__ __launch_bounds__(64) void add_signal_count(
 // ---------------------------------------------------------------------------
 // A stable LSD radix sort of (key, value) pairs, for the component sort's
 // 32-bit keys: three passes of 9-bit digits. A pass is a histogram per tile
-// of 4096 pairs, one block turning the histograms into each tile's first
-// place per digit, and a scatter that ranks each tile's pairs per digit in
-// input order (a wave's 64 pairs by ballot matches of their digits, the
-// waves of a tile and the rounds of a wave in order).
+// of 1024 pairs, a wave per digit turning the histograms into each tile's
+// first place within the digit, and a scatter that ranks each tile's pairs
+// per digit in input order (a wave's 64 pairs by ballot matches of their
+// digits, the waves of a tile and the rounds of a wave in order).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSortBits = 9;
 constexpr uint32_t kSortBuckets = 1u << kSortBits;
-constexpr uint32_t kSortTile = 4096; // pairs a block takes: 4 waves x 1024
+constexpr uint32_t kSortTile = 1024; // pairs a block takes: 4 waves x 256
 
 __global__ __launch_bounds__(256) void sort_histogram(const uint32_t *keys,
                                                       uint32_t n, int shift,
