@@ -30,8 +30,11 @@
 //            become the next chunk's frontier, each pointing at its
 //            component's first pixel.
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sf_device.hh"
@@ -1418,6 +1421,37 @@ __global__ __launch_bounds__(64) void add_signal_count(
     counts[0] += total;
 }
 
+// A plan's lists as add_plan takes them: each frame's list (its count
+// entries, none if the list overflowed its capacity) copied to dst +
+// dst_offset on the plan's own stream, and the counts to device memory and
+// to host-mapped memory, so that the plan may be rebound and run again at
+// once while the labeller reads the copy later.
+struct SnapFrame {
+  const sf_signal_pixel *src;
+  const uint32_t *count;
+  uint32_t capacity, pad;
+  uint64_t dst_offset;
+};
+
+__global__ __launch_bounds__(256) void snapshot_lists(
+    const SnapFrame *frames, uint32_t n_frames, sf_signal_pixel *dst,
+    uint32_t *counts, uint32_t *host_counts) {
+  for (uint32_t k = blockIdx.y; k < n_frames; k += gridDim.y) {
+    const SnapFrame f = frames[k];
+    const uint32_t c = *f.count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      counts[k] = c;
+      host_counts[k] = c;
+    }
+    const uint32_t take = c <= f.capacity ? c : 0u;
+    const uint4 *from = reinterpret_cast<const uint4 *>(f.src);
+    uint4 *to = reinterpret_cast<uint4 *>(dst + f.dst_offset);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < take;
+         i += gridDim.x * blockDim.x)
+      to[i] = from[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // A stable LSD radix sort of (key, value) pairs, for the component sort's
 // 32-bit keys: three passes of 9-bit digits. A pass is a histogram per tile
@@ -1621,7 +1655,32 @@ template <typename T> struct DevArray {
 struct sf_labeller {
   size_t height = 0, width = 0;
   sf_options options{};
+  int device = 0;
   hipStream_t stream = nullptr;
+  // add_plan's chunks: the plan's lists are copied on the plan's stream to
+  // one of two snapshot sides (snapshot_lists), and a worker thread labels
+  // the chunk once the copy is done, so add_plan returns without waiting
+  // for the plan or the labelling. One chunk is handed over at a time.
+  struct Job {
+    std::vector<sf::ChunkDesc> frames;
+    int side = 0;
+  };
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv;
+  bool job_ready = false, busy = false, stopping = false;
+  Job job;
+  int deferred_status = 0; // the first failure of a handed-over chunk
+  std::string deferred_error;
+  sf::DevArray<sf_signal_pixel> snap_list[2];
+  sf::DevArray<sf::SnapFrame> snap_table[2];
+  sf::DevArray<uint32_t> snap_counts[2];
+  sf::SnapFrame *h_snap_table[2] = {nullptr, nullptr}; // pinned staging
+  uint32_t *h_snap_counts[2] = {nullptr, nullptr};     // host-mapped
+  uint32_t *hd_snap_counts[2] = {nullptr, nullptr};
+  size_t snap_frames_cap[2] = {0, 0};
+  hipEvent_t snap_ready[2] = {nullptr, nullptr}, snap_free[2] = {nullptr, nullptr};
+  int snap_side = 0;
   hipEvent_t done = nullptr;
   bool started = false;
   int64_t last_frame = 0;
@@ -2081,8 +2140,73 @@ int plan_lists(sf_plan *plan, std::vector<const sf_signal_pixel *> &lists,
 
 namespace {
 
-// Processes the frames added from the host since the last flush.
+// Waits until the worker has labelled (enqueued) every handed-over chunk;
+// reports the first failure among them, once, as this thread's error.
+int drain(sf_labeller *l) {
+  std::unique_lock<std::mutex> lock(l->mu);
+  l->cv.wait(lock, [&] { return !l->job_ready && !l->busy; });
+  const int status = l->deferred_status;
+  if (status != SF_OK) {
+    g_label_error = l->deferred_error;
+    l->deferred_status = SF_OK;
+  }
+  return status;
+}
+
+// The worker's part of a handed-over chunk: once the snapshot is copied,
+// the chunk is labelled from it like any other, and the side is marked
+// free for the snapshot after next once the labelling has read it.
+int label_job(sf_labeller *l, sf_labeller::Job &job) {
+  const int side = job.side;
+  LAB_CHECK(hipEventSynchronize(l->snap_ready[side]),
+            "waiting for the plan's lists");
+  std::vector<uint32_t> counts(job.frames.size());
+  for (size_t k = 0; k < counts.size(); k++)
+    counts[k] = l->h_snap_counts[side][k];
+  const int status = run_chunk(l, job.frames, counts, false, nullptr);
+  LAB_CHECK(hipEventRecord(l->snap_free[side], l->stream), "event");
+  return status;
+}
+
+void worker_loop(sf_labeller *l) {
+  (void)hipSetDevice(l->device);
+  std::unique_lock<std::mutex> lock(l->mu);
+  for (;;) {
+    l->cv.wait(lock, [&] { return l->job_ready || l->stopping; });
+    if (!l->job_ready)
+      return; // stopping, and nothing handed over
+    sf_labeller::Job job = std::move(l->job);
+    l->job_ready = false;
+    l->busy = true;
+    lock.unlock();
+    const int status = label_job(l, job);
+    lock.lock();
+    if (status != SF_OK && l->deferred_status == SF_OK) {
+      l->deferred_status = status;
+      l->deferred_error = g_label_error;
+    }
+    l->busy = false;
+    l->cv.notify_all();
+  }
+}
+
+void stop_worker(sf_labeller *l) {
+  if (!l->worker.joinable())
+    return;
+  {
+    std::lock_guard<std::mutex> lock(l->mu);
+    l->stopping = true;
+  }
+  l->cv.notify_all();
+  l->worker.join();
+}
+
+// Processes the frames added from the host since the last flush (after
+// every handed-over chunk).
 int flush_pending(sf_labeller *l) {
+  const int drained = drain(l);
+  if (drained != SF_OK)
+    return drained;
   if (l->pending_frames.empty())
     return SF_OK;
   LAB_CHECK(hipStreamSynchronize(l->stream), "waiting for the labeller");
@@ -2206,6 +2330,13 @@ sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
                  hipSuccess;
   ok = ok && hipEventCreateWithFlags(&l->runs_ready, hipEventDisableTiming) ==
                  hipSuccess;
+  ok = ok && hipGetDevice(&l->device) == hipSuccess;
+  for (int i = 0; i < 2; i++) {
+    ok = ok && hipEventCreateWithFlags(&l->snap_ready[i],
+                                       hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&l->snap_free[i],
+                                       hipEventDisableTiming) == hipSuccess;
+  }
   ok = ok && hipMalloc(&l->d_scalars, 8 * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipMalloc(&l->d_counts, 6 * sizeof(unsigned long long)) ==
                  hipSuccess;
@@ -2238,8 +2369,25 @@ sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
 void sf_hip_labeller_destroy(sf_labeller *l) {
   if (!l)
     return;
+  stop_worker(l); // labels what was handed over first
   if (l->stream)
     (void)hipStreamSynchronize(l->stream);
+  for (int i = 0; i < 2; i++) {
+    // the plans' streams may still be copying into a side
+    if (l->snap_ready[i]) {
+      (void)hipEventSynchronize(l->snap_ready[i]);
+      (void)hipEventDestroy(l->snap_ready[i]);
+    }
+    if (l->snap_free[i])
+      (void)hipEventDestroy(l->snap_free[i]);
+    l->snap_list[i].release();
+    l->snap_table[i].release();
+    l->snap_counts[i].release();
+    if (l->h_snap_table[i])
+      (void)hipHostFree(l->h_snap_table[i]);
+    if (l->h_snap_counts[i])
+      (void)hipHostFree(l->h_snap_counts[i]);
+  }
   for (int i = 0; i < 2; i++) {
     l->carry_frame[i].release();
     l->carry_index[i].release();
@@ -2343,7 +2491,7 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
     g_label_error = "sf_hip_labeller_add_plan: bad argument";
     return SF_BAD_ARGUMENT;
   }
-  int status = flush_pending(l);
+  int status = flush_pending(l); // also waits for the last handed-over chunk
   if (status != SF_OK)
     return status;
   if (l->started && first_frame <= l->last_frame) {
@@ -2357,23 +2505,82 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
   status = sf::plan_lists(plan, lists, count_ptrs, caps, &producer);
   if (status != SF_OK)
     return status;
-  std::vector<uint32_t> counts(lists.size());
-  status = sf_hip_plan_counts(plan, counts.data());
-  if (status != SF_OK)
-    return status;
-  std::vector<sf::ChunkDesc> frames(lists.size());
-  for (size_t k = 0; k < frames.size(); k++) {
-    frames[k].list = lists[k];
-    frames[k].count = count_ptrs[k];
+  const uint32_t n_frames = (uint32_t)lists.size();
+  if (n_frames == 0)
+    return SF_OK;
+  // the snapshot side: its last reader (the chunk before last) is enqueued
+  const int side = l->snap_side;
+  l->snap_side ^= 1;
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < n_frames; k++)
+    total += caps[k];
+  LAB_CHECK(l->snap_list[side].ensure(std::max<uint64_t>(total, 1)),
+            "snapshot alloc");
+  LAB_CHECK(l->snap_table[side].ensure(n_frames), "snapshot alloc");
+  LAB_CHECK(l->snap_counts[side].ensure(n_frames), "snapshot alloc");
+  if (n_frames > l->snap_frames_cap[side]) {
+    if (l->h_snap_table[side])
+      (void)hipHostFree(l->h_snap_table[side]);
+    if (l->h_snap_counts[side])
+      (void)hipHostFree(l->h_snap_counts[side]);
+    l->h_snap_table[side] = nullptr;
+    l->h_snap_counts[side] = nullptr;
+    l->snap_frames_cap[side] = 0;
+    LAB_CHECK(hipHostMalloc(&l->h_snap_table[side],
+                            n_frames * sizeof(sf::SnapFrame)),
+              "snapshot staging");
+    LAB_CHECK(hipHostMalloc(&l->h_snap_counts[side],
+                            n_frames * sizeof(uint32_t), hipHostMallocMapped),
+              "snapshot counts");
+    LAB_CHECK(hipHostGetDevicePointer(
+                  reinterpret_cast<void **>(&l->hd_snap_counts[side]),
+                  l->h_snap_counts[side], 0),
+              "snapshot counts");
+    l->snap_frames_cap[side] = n_frames;
+  }
+  uint64_t at = 0;
+  std::vector<sf::ChunkDesc> frames(n_frames);
+  for (uint32_t k = 0; k < n_frames; k++) {
+    sf::SnapFrame &f = l->h_snap_table[side][k];
+    f.src = lists[k];
+    f.count = count_ptrs[k];
+    f.capacity = caps[k];
+    f.pad = 0;
+    f.dst_offset = at;
+    frames[k].list = l->snap_list[side].ptr + at;
+    frames[k].count = l->snap_counts[side].ptr + k;
     frames[k].capacity = caps[k];
     frames[k].number = (int32_t)(first_frame + (int64_t)k);
-    frames[k].linked = k > 0 || (l->started && first_frame == l->last_frame + 1);
+    frames[k].linked =
+        k > 0 || (l->started && first_frame == l->last_frame + 1);
+    at += caps[k];
   }
-  status = run_chunk(l, frames, counts, false, producer);
-  if (status != SF_OK)
-    return status;
+  // on the plan's stream: after the run that wrote the lists, and before
+  // anything enqueued there later
+  LAB_CHECK(hipStreamWaitEvent(producer, l->snap_free[side], 0),
+            "event wait");
+  LAB_CHECK(hipMemcpyAsync(l->snap_table[side].ptr, l->h_snap_table[side],
+                           n_frames * sizeof(sf::SnapFrame),
+                           hipMemcpyHostToDevice, producer),
+            "snapshot table");
+  sf::snapshot_lists<<<dim3(32, std::min<uint32_t>(n_frames, 65535)), 256, 0,
+                       producer>>>(l->snap_table[side].ptr, n_frames,
+                                   l->snap_list[side].ptr,
+                                   l->snap_counts[side].ptr,
+                                   l->hd_snap_counts[side]);
+  LAB_CHECK(hipGetLastError(), "snapshot");
+  LAB_CHECK(hipEventRecord(l->snap_ready[side], producer), "event");
+  {
+    std::lock_guard<std::mutex> lock(l->mu);
+    l->job.frames = std::move(frames);
+    l->job.side = side;
+    l->job_ready = true;
+    if (!l->worker.joinable())
+      l->worker = std::thread(worker_loop, l);
+  }
+  l->cv.notify_all();
   l->started = true;
-  l->last_frame = first_frame + (int64_t)frames.size() - 1;
+  l->last_frame = first_frame + (int64_t)n_frames - 1;
   return SF_OK;
 }
 
@@ -2404,6 +2611,9 @@ int sf_hip_labeller_finish(sf_labeller *l) {
 int sf_hip_labeller_wait(sf_labeller *l) {
   if (!l)
     return SF_BAD_ARGUMENT;
+  const int drained = drain(l);
+  if (drained != SF_OK)
+    return drained;
   LAB_CHECK(hipStreamSynchronize(l->stream), "hipStreamSynchronize");
   l->n_carry = *l->h_scalars;
   return SF_OK;

@@ -264,7 +264,12 @@ int sf_hip_labeller_add(sf_labeller *labeller, int64_t frame,
                         const sf_signal_pixel *signal, size_t n);
 /* Add every frame of a plan's last run, as frames first_frame,
  * first_frame + 1, ... straight from the device lists (no host copy). A
- * frame whose list overflowed (-2) is added empty. */
+ * frame whose list overflowed (-2) is added empty. Returns without waiting
+ * for the plan: the lists are copied on the plan's stream (so the plan may be
+ * rebound and run again at once) and a worker thread labels them after the
+ * copy. A failure of that labelling is returned by the next call on the
+ * labeller (add, add_plan, flush, finish, wait or a query). Holds twice the
+ * plan's list capacity in device memory for the copies. */
 int sf_hip_labeller_add_plan(sf_labeller *labeller, sf_plan *plan,
                              int64_t first_frame);
 /* Label what has been added so far (queries do this themselves). */
