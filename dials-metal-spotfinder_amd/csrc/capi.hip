@@ -31,11 +31,16 @@ hipError_t launch_plain(int, const FrameDesc *, int, uint32_t, uint32_t,
 void sparse_layout(uint32_t, uint32_t, FrameDesc &);
 size_t sparse_bytes(uint32_t, uint32_t, uint32_t);
 void sparse_carve(uint8_t *, FrameDesc &);
+hipError_t launch_sparse32_threshold(const FrameDesc *, int, uint32_t,
+                                     hipStream_t, hipEvent_t *);
 hipError_t launch_sparse_threshold(const FrameDesc *, int, uint32_t,
                                    hipStream_t, hipEvent_t *);
 hipError_t launch_sparse_order(const FrameDesc *, int, uint32_t, hipStream_t);
 hipError_t launch_sparse(const FrameDesc *, int, uint32_t, uint32_t,
                          hipStream_t);
+hipError_t launch_sparse32_pipeline(const FrameDesc *, int, uint32_t,
+                                    uint32_t, const FrameDesc *, int, uint32_t,
+                                    hipStream_t, hipEvent_t *);
 hipError_t launch_sparse_pipeline(const FrameDesc *, int, uint32_t, uint32_t,
                                   const FrameDesc *, int, uint32_t,
                                   hipStream_t, hipEvent_t *);
@@ -178,7 +183,9 @@ enum Kernel { kSweep, kPlain, kProbe, kSparse };
 Kernel kernel_for(const sf_plan *p, int form) {
   if (form == 1)
     return kPlain;
-  if (form == 0 && p->pixel_bytes == 2)
+  // the sparse form at 32 bits addresses a frame's bytes with 32-bit
+  // offsets: frames below 2^29 pixels (the sweep takes larger ones)
+  if (form == 0 && (p->pixel_bytes == 2 || p->max_pixels < (1u << 29)))
     return kSparse;
   if (form == 3 && p->pixel_bytes == 2)
     return kProbe;
@@ -297,7 +304,9 @@ int upload_desc(sf_plan *p, const void *const *frames) {
     sf::FrameDesc &d = p->desc[f];
     d.pixels = frames[f];
     const uintptr_t at = reinterpret_cast<uintptr_t>(frames[f]);
-    d.vec = sparse ? (at % 4 == 0 && d.width % 2 == 0 ? 1u : 0u)
+    d.vec = sparse ? (at % 4 == 0 && (p->pixel_bytes == 4 || d.width % 2 == 0)
+                          ? 1u
+                          : 0u)
                    : (at % 8 == 0 && d.width % (uint32_t)k == 0 ? 1u : 0u);
     p->any_vec |= d.vec != 0;
     p->any_scalar |= d.vec == 0;
@@ -322,10 +331,16 @@ int enqueue(sf_plan *p, bool timing) {
   if (kernel == kSparse) { // clears its own counters and orders its lists
     // ev[0] after the clearing kernel, ev[1] after the scout, ev[2] = ev[3]
     // after the Poisson pass, ev[4] after the ordering
-    SF_CHECK(sf::launch_sparse_threshold(p->d_desc, p->n, p->items,
-                                         p->stream,
-                                         timing ? p->ev : nullptr),
-             "launching the sparse threshold");
+    if (p->pixel_bytes == 4)
+      SF_CHECK(sf::launch_sparse32_threshold(p->d_desc, p->n, p->items,
+                                             p->stream,
+                                             timing ? p->ev : nullptr),
+               "launching the sparse threshold");
+    else
+      SF_CHECK(sf::launch_sparse_threshold(p->d_desc, p->n, p->items,
+                                           p->stream,
+                                           timing ? p->ev : nullptr),
+               "launching the sparse threshold");
     if (timing)
       for (int e = 2; e <= 3; e++)
         SF_CHECK(hipEventRecord(p->ev[e], p->stream), "hipEventRecord");
@@ -408,8 +423,7 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
     // device buffer does (src/dext_cuda.cu:614-616).
     p->capacity.push_back((uint32_t)(px * (size_t)pixel_bytes / 16));
     staging += p->capacity.back();
-    if (pixel_bytes == 2)
-      sparse += sf::sparse_bytes(heights[f], widths[f], p->capacity.back());
+    sparse += sf::sparse_bytes(heights[f], widths[f], p->capacity.back());
     cells += (size_t)heights[f] * ((widths[f] + 63) / 64);
     // enough band totals for the shortest band any form or tile uses
     int shortest_tile = sf::probe_tile_height(0);
@@ -432,6 +446,10 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
                  hipSuccess;
   ok = ok && hipMalloc(&p->d_out, sizeof(sf_signal_pixel) *
                                       (staging ? staging : 1)) == hipSuccess;
+  // the sparse form's tables (u32 frames of 2^29 pixels and more never
+  // take that form)
+  if (pixel_bytes == 4 && p->max_pixels >= (1u << 29))
+    sparse = 0;
   if (sparse)
     ok = ok && hipMalloc(&p->d_sparse, sparse) == hipSuccess;
   ok = ok && hipMalloc(&p->d_cell_count, cells) == hipSuccess;
@@ -768,10 +786,13 @@ int pipeline_check(sf_plan *cur, sf_plan *next, const char *what,
   if ((!cur && !next) || cur == next)
     return bad(std::string(what) + ": needs one plan or two different ones");
   for (sf_plan *p : {cur, next}) {
-    if (p && (p->pixel_bytes != 2 || kernel_for(p, p->form) != kSparse))
+    if (p && kernel_for(p, p->form) != kSparse)
       return bad(std::string(what) +
-                 ": pipelined steps take 16-bit plans of form 0");
+                 ": pipelined steps take plans of the sparse form (form 0)");
   }
+  if (cur && next && cur->pixel_bytes != next->pixel_bytes)
+    return bad(std::string(what) +
+               ": the two plans' pixels differ in size");
   // a capture runs nothing: what it needs is checked when it is replayed
   if (running && cur && !cur->scouted)
     return bad(std::string(what) +
@@ -814,11 +835,12 @@ int pipeline_enqueue(sf_plan *cur, sf_plan *next, hipStream_t s,
   // ordering
   hipEvent_t marks[4] = {owner->ev[0], owner->ev[1], owner->ev[3],
                          owner->ev[4]};
-  SF_CHECK(sf::launch_sparse_pipeline(
-               cur ? cur->d_desc : nullptr, cur ? cur->n : 0,
-               cur ? cur->items : 0, cur ? cur->max_cell_rows : 0,
-               next ? next->d_desc : nullptr, next ? next->n : 0,
-               next ? next->items : 0, s, timing ? marks : nullptr),
+  const auto launch = owner->pixel_bytes == 4 ? sf::launch_sparse32_pipeline
+                                               : sf::launch_sparse_pipeline;
+  SF_CHECK(launch(cur ? cur->d_desc : nullptr, cur ? cur->n : 0,
+                  cur ? cur->items : 0, cur ? cur->max_cell_rows : 0,
+                  next ? next->d_desc : nullptr, next ? next->n : 0,
+                  next ? next->items : 0, s, timing ? marks : nullptr),
            "launching a pipelined step");
   return SF_OK;
 }

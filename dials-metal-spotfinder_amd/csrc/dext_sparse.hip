@@ -50,8 +50,6 @@ __device__ __forceinline__ uint32_t udot2(uint32_t a, uint32_t b, uint32_t c) {
                                 __builtin_bit_cast(us2, b), c, false);
 }
 
-constexpr int kBandCells = 16; // cell rows a scout item covers
-constexpr int kSegment = 64 * kBandCells; // list entries an item owns
 
 // sqrt(2 (n - 1)) for n = 2..49 as the reference rounds it (src/dext.cc:108);
 // n = 0, 1 give +inf so that "lhs > rhs" is false there (0 * inf is NaN).
@@ -343,24 +341,6 @@ __device__ void scout_item(const FrameDesc &fd, uint32_t strip, uint32_t band,
   }
   if (lane == 0)
     fd.icount[local] = total;
-}
-
-// Work item `item` of a launch (4 a block, XCD-local runs of blocks): its
-// frame, copied into registers (the passes store through pointers the
-// compiler cannot tell apart from the descriptor), and its index within the
-// frame; false past the end.
-__device__ __forceinline__ bool wave_item(const FrameDesc *frames,
-                                          int n_frames, uint32_t first,
-                                          uint32_t end, uint32_t block,
-                                          FrameDesc &fd, uint32_t &local) {
-  // the wave's index, as a scalar: everything derived from it is uniform
-  const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane(
-      (int)(first + 4 * block + (threadIdx.x >> 6)));
-  if (item >= end)
-    return false;
-  fd = frames[frame_of_item(frames, n_frames, item)];
-  local = item - fd.item_begin;
-  return true;
 }
 
 __device__ __forceinline__ bool block_item(const FrameDesc *frames,
@@ -799,37 +779,6 @@ __global__ __launch_bounds__(256) void poisson_kernel(const FrameDesc *frames,
 // both. No block of a launch waits for another: each launch's kinds work on
 // different chunks, whose tables are apart.
 // ---------------------------------------------------------------------------
-struct Side {
-  const FrameDesc *frames; // descriptors of the side's chunk
-  int n_frames;
-  uint32_t first, end;     // its work items in this launch
-  uint32_t blocks;         // blocks of four items
-};
-
-// Block `block` of a mixed launch: 1 for a scout block, 0 for a sparse
-// block, with its index among its kind; -1 for the grid's padding. Blocks are dealt to the eight XCDs round-robin, and
-// each XCD draws HBM through its own share of the fabric: so each XCD takes
-// an eighth of each kind (contiguous runs: neighbouring strips share an
-// L2), the two kinds spread through its blocks in proportion. The grid is
-// mixed_grid() blocks.
-__device__ __forceinline__ int mixed_role(uint32_t block,
-                                          uint32_t scout_blocks,
-                                          uint32_t sparse_blocks,
-                                          uint32_t &index) {
-  const uint32_t x = block % 8, k = block / 8;
-  const uint32_t s0 = scout_blocks * x / 8, s1 = scout_blocks * (x + 1) / 8;
-  const uint32_t e0 = sparse_blocks * x / 8, e1 = sparse_blocks * (x + 1) / 8;
-  const uint32_t ns = s1 - s0, n = ns + (e1 - e0);
-  index = 0;
-  if (k >= n)
-    return -1;
-  // scout blocks among this XCD's first k and first k + 1
-  const uint32_t before = (uint32_t)((uint64_t)k * ns / n);
-  const bool scout = (uint32_t)((uint64_t)(k + 1) * ns / n) > before;
-  index = scout ? s0 + before : e0 + (k - before);
-  return scout ? 1 : 0;
-}
-
 __global__ __launch_bounds__(256) void scout_erode_kernel(Side scout,
                                                           Side sparse) {
   __shared__ float lut[64];
@@ -1053,6 +1002,15 @@ __global__ void zero_kernel(const FrameDesc *frames) {
 }
 
 } // namespace
+
+// The clearing kernel of the sparse forms on its own (the u32 form's
+// threshold starts with it).
+hipError_t launch_sparse_zero(const FrameDesc *frames, int n_frames,
+                              hipStream_t stream) {
+  if (n_frames > 0)
+    zero_kernel<<<n_frames, 256, 0, stream>>>(frames);
+  return hipGetLastError();
+}
 
 // Items of a frame for the scout: 64-cell strips x 16-cell-row bands.
 void sparse_layout(uint32_t height, uint32_t width, FrameDesc &d) {

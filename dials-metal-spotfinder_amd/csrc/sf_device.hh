@@ -152,4 +152,62 @@ __device__ __forceinline__ uint32_t xcd_local_item(uint32_t block,
   return (block % 8) * per + block / 8;
 }
 
+// The sparse forms' scout items (dext_sparse.hip, dext_sparse32.hip): 16
+// cell rows by 64 cell columns, each listing its active cells in its own
+// segment of the frame's list.
+constexpr int kBandCells = 16;
+constexpr int kSegment = 64 * kBandCells;
+
+// Work item `item` of a launch (4 a block, XCD-local runs of blocks): its
+// frame, copied into registers (the passes store through pointers the
+// compiler cannot tell apart from the descriptor), and its index within the
+// frame; false past the end.
+__device__ __forceinline__ bool wave_item(const FrameDesc *frames,
+                                          int n_frames, uint32_t first,
+                                          uint32_t end, uint32_t block,
+                                          FrameDesc &fd, uint32_t &local) {
+  // the wave's index, as a scalar: everything derived from it is uniform
+  const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane(
+      (int)(first + 4 * block + (threadIdx.x >> 6)));
+  if (item >= end)
+    return false;
+  fd = frames[frame_of_item(frames, n_frames, item)];
+  local = item - fd.item_begin;
+  return true;
+}
+
+// One chunk's part of a mixed launch of the sparse forms (a scout of one
+// chunk beside the erosion or Poisson pass of another).
+struct Side {
+  const FrameDesc *frames; // descriptors of the side's chunk
+  int n_frames;
+  uint32_t first, end;     // its work items in this launch
+  uint32_t blocks;         // blocks of four items
+};
+
+// Block `block` of a mixed launch: 1 for a scout block, 0 for a sparse
+// block, with its index among its kind; -1 for the grid's padding. Blocks
+// are dealt to the eight XCDs round-robin, and each XCD draws HBM through
+// its own share of the fabric: so each XCD takes
+// an eighth of each kind (contiguous runs: neighbouring strips share an
+// L2), the two kinds spread through its blocks in proportion. The grid is
+// mixed_grid() blocks.
+__device__ __forceinline__ int mixed_role(uint32_t block,
+                                          uint32_t scout_blocks,
+                                          uint32_t sparse_blocks,
+                                          uint32_t &index) {
+  const uint32_t x = block % 8, k = block / 8;
+  const uint32_t s0 = scout_blocks * x / 8, s1 = scout_blocks * (x + 1) / 8;
+  const uint32_t e0 = sparse_blocks * x / 8, e1 = sparse_blocks * (x + 1) / 8;
+  const uint32_t ns = s1 - s0, n = ns + (e1 - e0);
+  index = 0;
+  if (k >= n)
+    return -1;
+  // scout blocks among this XCD's first k and first k + 1
+  const uint32_t before = (uint32_t)((uint64_t)k * ns / n);
+  const bool scout = (uint32_t)((uint64_t)(k + 1) * ns / n) > before;
+  index = scout ? s0 + before : e0 + (k - before);
+  return scout ? 1 : 0;
+}
+
 } // namespace sf

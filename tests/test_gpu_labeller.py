@@ -227,3 +227,33 @@ def test_chunks_past_the_merge_sort_limit(sf):
     assert expected[2]["accepted"] > 1000
     plan.close()
     buf.close()
+
+
+def test_plan_rerun_and_closed_right_after_add_plan(sf):
+    """add_plan returns before the chunk is labelled: the plan is run again
+    on other frames at once, added again, then closed before the labeller
+    has finished. The tables still equal the oracle's over both chunks."""
+    h, w, n = 256, 300, 8
+    buf = sf.DeviceBuffer(2 * h * w * n)
+    spots = fr.diffraction_spots(11, h, w, n, 1500.0 * 60)
+    sf.synth_frames(buf, 2, h, w, n, 11, 0.3, True, 0.0005, spots)
+    host = np.zeros((n, h, w), dtype=np.uint16)
+    buf.download(host)
+    oracle = cpu.Labeller(h, w)
+    for z in range(n):
+        assert oracle.add(z, cpu.dext(host[z])[1]) == 0
+    oracle.finish()
+    expected = oracle.results()
+    lab = sf.Labeller(h, w)
+    plan = sf.Plan(2, [(h, w)] * 4)
+    for c in range(2):
+        plan.run([buf.ptr + 2 * h * w * (4 * c + k) for k in range(4)])
+        lab.add_plan(plan, 4 * c)
+    plan.close()
+    lab.finish()
+    got = lab.results()
+    assert got[2] == expected[2], (got[2], expected[2])
+    assert same_tables(got, expected)
+    assert expected[2]["accepted"] > 3
+    lab.close()
+    buf.close()

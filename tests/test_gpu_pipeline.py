@@ -32,6 +32,24 @@ def chunk_frames(sf, shapes, seed, density=30000.0):
     return bufs, ptrs, hosts
 
 
+def chunk_frames32(sf, shapes, seed, density=30000.0):
+    """Device u32 frames of the 32-bit series' value model (Poisson 50,
+    peaks up to 10^6, gaps, hot pixels), one per shape."""
+    bufs, ptrs, hosts = [], [], []
+    for k, (h, w) in enumerate(shapes):
+        b = sf.DeviceBuffer(4 * h * w)
+        spots = fr.diffraction_spots(seed + k, h, w, 1,
+                                     density * h * w / (4362 * 4148),
+                                     peak=1.0e6, min_peak=100.0)
+        sf.synth_frames(b, 4, h, w, 1, seed + k, 50.0, True, 0.0005, spots)
+        host = np.zeros((h, w), np.uint32)
+        b.download(host)
+        bufs.append(b)
+        ptrs.append(b.ptr)
+        hosts.append(host)
+    return bufs, ptrs, hosts
+
+
 def check(plan, hosts, what):
     plan.wait()
     counts = plan.counts()
@@ -64,6 +82,66 @@ def test_eager_steps_over_chunks_of_other_shapes(sf):
     for bufs, _, _ in chunks:
         for b in bufs:
             b.close()
+
+
+def test_u32_steps_over_chunks_of_other_shapes(sf):
+    """The 32-bit form's pipelined steps (its scout beside its erosion and
+    Poisson passes) over chunks of two shape lists in turn, eager and
+    replayed: every list equals the oracle's."""
+    chunks = [chunk_frames32(sf, SHAPES_A if c % 2 == 0 else SHAPES_B,
+                             500 + 10 * c) for c in range(4)]
+    plans = [sf.Plan(4, SHAPES_A), sf.Plan(4, SHAPES_B)]
+    plans[0].bind(chunks[0][1])
+    sf.pipeline(None, plans[0])
+    for c in range(1, 4):
+        cur, nxt = plans[(c - 1) % 2], plans[c % 2]
+        nxt.bind(chunks[c][1])
+        sf.pipeline(cur, nxt)
+        check(cur, chunks[c - 1][2], ("u32 chunk", c - 1))
+    sf.pipeline(plans[1], None)
+    check(plans[1], chunks[3][2], ("u32 chunk", 3))
+    for p in plans:
+        p.close()
+    for bufs, _, _ in chunks:
+        for b in bufs:
+            b.close()
+
+
+def test_u32_full_size_steps_equal_run(sf):
+    """Two 4-frame chunks of 4362 x 4148 u32 frames through pipelined steps
+    (graphs replayed): every list equals the plan's own run() on the same
+    frames, byte for byte."""
+    h, w, n = 4362, 4148, 8
+    buf = sf.DeviceBuffer(4 * h * w * n)
+    spots = fr.diffraction_spots(7, h, w, n, 1500.0, peak=1.0e6,
+                                 min_peak=100.0)
+    sf.synth_frames(buf, 4, h, w, n, 7, 50.0, True, 0.0005, spots)
+    ptr = [buf.ptr + 4 * h * w * f for f in range(n)]
+    ref = sf.Plan(4, [(h, w)] * 4)
+    want = []
+    for c in range(2):
+        ref.run(ptr[4 * c:4 * c + 4])
+        ref.wait()
+        counts = ref.counts()
+        want.append([ref.output(k, int(counts[k])) for k in range(4)])
+    ref.close()
+    p = [sf.Plan(4, [(h, w)] * 4), sf.Plan(4, [(h, w)] * 4)]
+    p[0].bind(ptr[:4])
+    sf.pipeline(None, p[0])
+    p[1].bind(ptr[4:])
+    sf.capture_pipeline(p[0], p[1])
+    sf.replay_pipeline(p[0], p[1])
+    sf.pipeline(p[1], None)
+    for c in range(2):
+        p[c].wait()
+        counts = p[c].counts()
+        for k in range(4):
+            got = p[c].output(k, int(counts[k]))
+            assert got.tobytes() == want[c][k].tobytes(), (c, k)
+        assert len(want[c][0]) > 20000
+    for q in p:
+        q.close()
+    buf.close()
 
 
 def test_replayed_steps_and_the_labeller(sf):
@@ -161,7 +239,7 @@ def test_refusals(sf):
         sf.pipeline(a, a)
     c32.bind([buf.ptr])
     with pytest.raises(ValueError):
-        sf.pipeline(None, c32)
+        sf.pipeline(b, c32)  # pixels of two sizes
     sf.pipeline(None, a)
     with pytest.raises(ValueError):
         sf.replay_pipeline(a, None)  # nothing captured
