@@ -357,6 +357,133 @@ def pipelined_step_timing(sf, plans, chunks, steps, labeller=None,
     return per_step, a, b, o
 
 
+SWEEP_DENSITIES = (1000, 6000, 30000, 130000)  # signal pixels a frame
+SWEEP_SPANS = (1, 3, 10)  # frames a reflection spans
+SWEEP_FRAMES = 40
+
+
+def sweep_lists(signal_dtype, density, span):
+    """The labeller sweep's frames (tests/bench_dials_spots.cc:62-111):
+    blobs of the 21 pixels within sqrt(5) of a centre, each spanning `span`
+    consecutive frames, scattered over a 4362 x 4148 detector and valued
+    50 + int(1500 x rocking x radial); each frame ascending by index without
+    duplicates. Generated here with numpy, seeded."""
+    rng = np.random.default_rng(7)
+    blobs = density * SWEEP_FRAMES // (19 * span)
+    y = rng.integers(8, HEIGHT - 8, blobs)
+    x = rng.integers(8, WIDTH - 8, blobs)
+    first = np.arange(blobs) % (SWEEP_FRAMES - span)
+    offs = [(dy, dx) for dy in range(-2, 3) for dx in range(-2, 3)
+            if dy * dy + dx * dx <= 5]
+    per = [[] for _ in range(SWEEP_FRAMES)]
+    for step in range(span):
+        middle = step - (span - 1) / 2.0
+        rocking = np.exp(-middle * middle / 4.0)
+        for dy, dx in offs:
+            radial = np.exp(-(dy * dy + dx * dx) / 4.0)
+            value = 50 + int(1500.0 * rocking * radial)
+            index = (y + dy) * WIDTH + x + dx
+            frame = first + step
+            for f in range(SWEEP_FRAMES):
+                sel = frame == f
+                if sel.any():
+                    per[f].append(index[sel])
+                    per[f].append(np.full(int(sel.sum()), value, np.int64))
+    lists = []
+    for f in range(SWEEP_FRAMES):
+        idx = np.concatenate(per[f][0::2]) if per[f] else np.zeros(0, np.int64)
+        val = np.concatenate(per[f][1::2]) if per[f] else np.zeros(0, np.int64)
+        order = np.argsort(idx, kind="stable")
+        idx, val = idx[order], val[order]
+        keep = np.ones(len(idx), bool)
+        keep[1:] = idx[1:] != idx[:-1]
+        sig = np.zeros(int(keep.sum()), signal_dtype)
+        sig["index"] = idx[keep]
+        sig["value"] = val[keep]
+        lists.append(sig)
+    return lists
+
+
+def labeller_sweep(sf, repeats):
+    """Labelling cost against density and span, as the reference's grouping
+    benchmark sweeps it (tests/bench_dials_spots.cc:150-160): 40 frames of
+    1k/6k/30k/130k signal pixels with reflections spanning 1/3/10 frames.
+    The device labeller takes the lists from device memory (add_lists, one
+    chunk) and is timed from the call to finish()'s return, best of
+    `repeats`; the reference's Labeller (compiled from its sources) takes
+    the host lists on one core, best of `repeats`; their tables are
+    compared byte for byte."""
+    from oracle import cpu
+
+    rows = []
+    for density in SWEEP_DENSITIES:
+        for span in SWEEP_SPANS:
+            lists = sweep_lists(sf.SIGNAL, density, span)
+            counts = [len(x) for x in lists]
+            total = sum(counts)
+            buf = sf.DeviceBuffer(16 * max(total, 1))
+            cbuf = sf.DeviceBuffer(4 * SWEEP_FRAMES)
+            offsets = np.concatenate([[0], np.cumsum(counts)[:-1]])
+            if total:
+                buf.upload(np.concatenate(lists))
+            cbuf.upload(np.array(counts, np.uint32))
+            ptrs = [buf.ptr + 16 * int(o) for o in offsets]
+            cptrs = [cbuf.ptr + 4 * k for k in range(SWEEP_FRAMES)]
+            best, got = None, None
+            for _ in range(repeats):
+                lab = sf.Labeller(HEIGHT, WIDTH)
+                sf.lib().sf_hip_synchronize()
+                t0 = time.perf_counter()
+                lab.add_lists(list(range(SWEEP_FRAMES)), ptrs, cptrs, counts,
+                              counts)
+                lab.finish()
+                spent = time.perf_counter() - t0
+                best = spent if best is None else min(best, spent)
+                got = lab.results()
+                lab.close()
+            ref_best, want = None, None
+            if cpu.have_ref():
+                for _ in range(repeats):
+                    ref = cpu.RefLabeller(HEIGHT, WIDTH)
+                    t0 = time.perf_counter()
+                    for f, x in enumerate(lists):
+                        ref.add(f, x)
+                    ref.finish()
+                    spent = time.perf_counter() - t0
+                    ref_best = spent if ref_best is None else min(ref_best,
+                                                                  spent)
+                    want = ref.results()
+            buf.close()
+            cbuf.close()
+            per_frame = total / SWEEP_FRAMES
+            row = {
+                "pixels_per_frame": round(per_frame),
+                "span": span,
+                "device_ms_per_frame": round(1e3 * best / SWEEP_FRAMES, 4),
+                "device_mpixel_per_s": round(total / best / 1e6, 1),
+                "spots": len(got[0]),
+            }
+            if want is not None:
+                row["reference_ms_per_frame"] = round(
+                    1e3 * ref_best / SWEEP_FRAMES, 4)
+                row["reference_mpixel_per_s"] = round(total / ref_best / 1e6,
+                                                      1)
+                row["table_diffs"] = (
+                    int(got[0].tobytes() != want[0].tobytes()) +
+                    int(got[1].tobytes() != want[1].tobytes()) +
+                    int(got[2] != want[2]))
+            rows.append(row)
+    return {
+        "workload": f"{SWEEP_FRAMES} frames of 4362x4148, 21-pixel blobs "
+                    "spanning 1/3/10 frames at 1k/6k/30k/130k signal "
+                    "pixels a frame (the reference's grouping benchmark)",
+        "device": "lists in device memory, one chunk, add_lists to finish",
+        "reference": "the reference's Labeller on one core, host lists",
+        "repeats": repeats,
+        "rows": rows,
+    }
+
+
 def batched_workload(sf, fr, steps):
     """Chunks of uneven frame sizes in one launch: 32 frames of 2162 x 2068,
     16 of 3269 x 3110 and 8 of 4362 x 4148 (H x W) u16, the 1800-frame
@@ -672,6 +799,9 @@ def main() -> int:
     ap.add_argument("--decode-steps", type=int, default=5,
                     help="timed decodes of a chunk of bitshuffle+LZ4 chunks "
                          "(0: skip)")
+    ap.add_argument("--sweep-repeats", type=int, default=3,
+                    help="the labeller density x span sweep, best of this "
+                         "many (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy, three times (the "
                          "known-size kernel that scales FETCH_SIZE and "
@@ -941,6 +1071,11 @@ def main() -> int:
                                    decode_chunks, refl, args.cpu_repeats)
     del host
 
+    sweep = None
+    if args.sweep_repeats > 0:
+        sweep = labeller_sweep(sf, args.sweep_repeats)
+        log(f"labeller sweep: {sweep['rows']}")
+
     others = {}
     if args.other_steps > 0:
         plan.close()
@@ -1043,6 +1178,7 @@ def main() -> int:
         "p99_latency_ms": latency,
         "serving": serving,
         "other_workloads": others,
+        "labeller_sweep": sweep,
     }
     print(json.dumps(record))
     plan.close()
@@ -1051,7 +1187,8 @@ def main() -> int:
         baseline["decode_port_diffs"] == 0 and \
         baseline["refl_identical"] is not False
     tables_ok = baseline["table_diffs"] in (None, 0) and \
-        others.get("u32", {}).get("list_diffs", 0) == 0
+        others.get("u32", {}).get("list_diffs", 0) == 0 and \
+        all(r.get("table_diffs", 0) == 0 for r in (sweep or {}).get("rows", []))
     return 0 if diffs == 0 and decoded_ok and tables_ok else 3
 
 

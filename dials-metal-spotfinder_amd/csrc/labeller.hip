@@ -2584,6 +2584,15 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
   return SF_OK;
 }
 
+int sf_hip_labeller_add_lists(sf_labeller *l, int n, const int64_t *frames,
+                              const sf_signal_pixel *const *lists,
+                              const uint32_t *const *count_ptrs,
+                              const uint32_t *capacities,
+                              const uint32_t *counts, void *stream) {
+  return sf::labeller_add_lists(l, n, frames, lists, count_ptrs, capacities,
+                                counts, static_cast<hipStream_t>(stream));
+}
+
 int sf_hip_labeller_flush(sf_labeller *l) {
   if (!l)
     return SF_BAD_ARGUMENT;

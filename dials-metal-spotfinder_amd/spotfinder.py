@@ -150,6 +150,9 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_labeller_destroy", None, _p)
     fn("sf_hip_labeller_add", _i, _p, ctypes.c_int64, _p, _sz)
     fn("sf_hip_labeller_add_plan", _i, _p, _p, ctypes.c_int64)
+    fn("sf_hip_labeller_add_lists", _i, _p, _i,
+       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_p),
+       ctypes.POINTER(_p), ctypes.POINTER(_u32), ctypes.POINTER(_u32), _p)
     fn("sf_hip_labeller_flush", _i, _p)
     fn("sf_hip_labeller_finish", _i, _p)
     fn("sf_hip_labeller_wait", _i, _p)
@@ -558,6 +561,21 @@ class Labeller:
         thread; a failure there is raised by the next call."""
         self._status(lib().sf_hip_labeller_add_plan(
             self.handle, plan.handle, int(first_frame)), "add_plan")
+
+    def add_lists(self, frames, lists, count_ptrs, capacities, counts,
+                  stream=None) -> None:
+        """Add frames whose ordered lists are in device memory: frame
+        numbers, device pointers of the lists and of their counts, the
+        lists' capacities and the counts as the host knows them."""
+        n = len(frames)
+        if not (len(lists) == len(count_ptrs) == len(capacities) ==
+                len(counts) == n):
+            raise ValueError("add_lists: one entry a frame in every argument")
+        self._status(lib().sf_hip_labeller_add_lists(
+            self.handle, n, (ctypes.c_int64 * n)(*frames),
+            (_p * n)(*lists), (_p * n)(*count_ptrs),
+            (_u32 * n)(*capacities), (_u32 * n)(*counts), stream),
+            "add_lists")
 
     def flush(self) -> None:
         self._status(lib().sf_hip_labeller_flush(self.handle), "flush")

@@ -272,6 +272,21 @@ int sf_hip_labeller_add(sf_labeller *labeller, int64_t frame,
  * plan's list capacity in device memory for the copies. */
 int sf_hip_labeller_add_plan(sf_labeller *labeller, sf_plan *plan,
                              int64_t first_frame);
+/* Add n frames whose ordered lists are already in device memory, as frame
+ * numbers frames[0] < frames[1] < ... (above the last added; gaps close
+ * components as src/dials_spots.cc:68-69 does). For frame k: lists[k] and
+ * count_ptrs[k] are device pointers, capacities[k] its list's entries,
+ * counts[k] the count as the host knows it (a count above the capacity adds
+ * the frame empty). The lists are read on `stream` (NULL: the labeller's
+ * own), so the caller may reuse them once later work on that stream runs.
+ * No counterpart in the reference, whose Labeller takes host vectors: for
+ * callers whose lists are on the device already (the serving loop). */
+int sf_hip_labeller_add_lists(sf_labeller *labeller, int n,
+                              const int64_t *frames,
+                              const sf_signal_pixel *const *lists,
+                              const uint32_t *const *count_ptrs,
+                              const uint32_t *capacities,
+                              const uint32_t *counts, void *stream);
 /* Label what has been added so far (queries do this themselves). */
 int sf_hip_labeller_flush(sf_labeller *labeller);
 /* Labeller::finish (src/dials_spots.cc:147-151). */
