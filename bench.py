@@ -615,22 +615,31 @@ SERVE_SERIES = (
 )
 
 
-def serve_latency(sf, fr, seconds, ring, budget_gb, max_batch):
+def serve_latency(sf, fr, seconds, ring, budget_gb, max_batch, host=False,
+                  which=None):
     """The serving loop on four concurrent series arriving in real time for
     `seconds` (3600 frames of the 750 Hz series at 4.8 s). Each series cycles
-    through `ring` distinct device-resident frames; latencies run from each
-    frame's scheduled arrival."""
+    through `ring` distinct frames, device-resident or (host) in pinned
+    host memory, which the loop copies in on its batch streams; latencies
+    run from each frame's scheduled arrival, so they include that copy."""
     server = sf.Server(int(budget_gb * (1 << 30)), max_batch=max_batch)
     buffers, rings, ids, rates, counts = [], [], [], [], []
-    for k, (pb, h, w, rate, bg, peak) in enumerate(SERVE_SERIES):
+    chosen = [SERVE_SERIES[k] for k in (which or range(len(SERVE_SERIES)))]
+    for k, (pb, h, w, rate, bg, peak) in enumerate(chosen):
         buf = sf.DeviceBuffer(pb * h * w * ring)
         spots = fr.diffraction_spots(100 + k, h, w, ring,
                                      1500.0 * h * w / (HEIGHT * WIDTH),
                                      peak=peak)
         sf.synth_frames(buf, pb, h, w, ring, 100 + k, bg, True, 0.0005, spots)
+        if host:
+            pinned = sf.HostBuffer(pb * h * w * ring)
+            buf.download(pinned.array(np.uint16 if pb == 2 else np.uint32,
+                                      (ring, h, w)))
+            buf.close()
+            buf = pinned
         buffers.append(buf)
         rings.append([buf.ptr + pb * h * w * f for f in range(ring)])
-        ids.append(server.open(pb, h, w))
+        ids.append(server.open(pb, h, w, host=host))
         rates.append(rate)
         counts.append(int(round(rate * seconds)))
     sf.lib().sf_hip_synchronize()
@@ -639,7 +648,7 @@ def serve_latency(sf, fr, seconds, ring, budget_gb, max_batch):
     wall = time.perf_counter() - t0
     series = []
     worst = 0.0
-    for sid, (pb, h, w, rate, bg, peak), n in zip(ids, SERVE_SERIES, counts):
+    for sid, (pb, h, w, rate, bg, peak), n in zip(ids, chosen, counts):
         server.close_series(sid)
         fm = server.frames(sid)
         assert len(fm["frame"]) == n
@@ -671,6 +680,10 @@ def serve_latency(sf, fr, seconds, ring, budget_gb, max_batch):
         "admission_waits": stats["admission_waits"],
         "peak_slot_gb": round(stats["peak_bytes"] / 2 ** 30, 3),
         "budget_gb": budget_gb,
+        "frames_from": "pinned host memory, copied in on the batch streams"
+                       if host else "device memory",
+        "ingest_gb_per_s": round(stats["ingested_bytes"] / wall / 1e9, 2)
+                           if host else None,
     }, worst
 
 
@@ -1089,16 +1102,29 @@ def main() -> int:
             u32_host, baseline["host"]["nproc"], args.cpu_repeats)
         del u32_host
 
-    latency, serving = None, None
+    latency, serving, serving_device = None, None, None
     if args.serve_seconds > 0:
         plan.close()
         labeller = None
         series.close()
+        # frames from pinned host memory (the headline latency: it includes
+        # the copy), then the same from device memory
         serving, worst = serve_latency(sf, fr, args.serve_seconds,
-                                       args.serve_ring, args.serve_budget_gb,
-                                       args.serve_batch)
+                                       min(args.serve_ring, 12),
+                                       args.serve_budget_gb, args.serve_batch,
+                                       host=True)
         latency = round(worst, 3)
-        log(f"serving: {serving}")
+        log(f"serving from host memory: {serving}")
+        serving["750_hz_series_alone"], _ = serve_latency(
+            sf, fr, args.serve_seconds, min(args.serve_ring, 12),
+            args.serve_budget_gb, args.serve_batch, host=True, which=[0])
+        log(f"the 750 Hz series alone from host memory: "
+            f"{serving['750_hz_series_alone']}")
+        serving_device, _ = serve_latency(sf, fr, args.serve_seconds,
+                                          args.serve_ring,
+                                          args.serve_budget_gb,
+                                          args.serve_batch)
+        log(f"serving from device memory: {serving_device}")
 
     record = {
         "metric": METRIC,
@@ -1177,6 +1203,7 @@ def main() -> int:
         "cpu_baseline": baseline,
         "p99_latency_ms": latency,
         "serving": serving,
+        "serving_device": serving_device,
         "other_workloads": others,
         "labeller_sweep": sweep,
     }

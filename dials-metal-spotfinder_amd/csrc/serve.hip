@@ -76,6 +76,7 @@ struct Series;
 struct Slot {
   Series *series = nullptr;
   void *base = nullptr;
+  void *frame = nullptr; // a host-fed series' frame, copied in per batch
   sf::FrameDesc desc{}; // buffers filled; pixels, counters, items per batch
 };
 
@@ -99,6 +100,7 @@ struct Series {
   int id = 0;
   int pixel_bytes = 2;
   uint32_t height = 0, width = 0;
+  bool host = false; // frames arrive in (pinned) host memory
   sf_labeller *labeller = nullptr;
   size_t slot_bytes = 0;
   std::vector<Slot *> slots, free_slots;
@@ -122,12 +124,17 @@ struct Batch {
   int capacity = 0;
   std::vector<Request> requests;
   std::vector<Slot *> slots;
-  sf::FrameDesc *h_desc = nullptr, *d_desc = nullptr;
-  uint32_t *d_counters = nullptr, *d_found = nullptr, *h_found = nullptr;
+  // descriptors and found counts pass through host-mapped memory, moved by
+  // one-block kernels: no copy of a batch waits in the copy engines behind
+  // another batch's frames
+  sf::FrameDesc *h_desc = nullptr, *d_desc = nullptr, *hd_desc = nullptr;
+  uint32_t *d_counters = nullptr, *d_found = nullptr, *h_found = nullptr,
+           *hd_found = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t masked = nullptr, released = nullptr;
   Clock::time_point masked_at; // when the watcher saw `masked` complete
   hipError_t mask_error = hipSuccess;
+  uint64_t ingested = 0;        // bytes copied in from host memory
   int jobs_left = 0;            // labelling jobs still reading the lists
   std::vector<std::vector<sf_signal_pixel>> lists; // keep_lists copies
 };
@@ -154,6 +161,7 @@ struct sf_server {
   // statistics
   uint64_t n_batches = 0, batched_frames = 0, admission_waits = 0;
   uint64_t overflowed = 0; // frames whose list overflowed (labelled empty)
+  uint64_t ingested = 0;   // bytes of host-fed frames copied to the device
   int max_inflight = 0;
 };
 
@@ -208,6 +216,10 @@ hipError_t make_slot(Series *se, Slot **out) {
   at = (at + 255) & ~(size_t)255;
   const size_t sparse_at = at;
   at += sparse_part(se->pixel_bytes, d);
+  at = (at + 255) & ~(size_t)255;
+  const size_t frame_at = at;
+  if (se->host)
+    at += (size_t)se->pixel_bytes * se->height * se->width;
   const hipError_t e = hipMalloc(&slot->base, at);
   if (e != hipSuccess) {
     delete slot;
@@ -221,18 +233,22 @@ hipError_t make_slot(Series *se, Slot **out) {
   d.cell_count = reinterpret_cast<uint8_t *>(b + count_at);
   if (se->pixel_bytes == 2)
     sf::sparse_carve(reinterpret_cast<uint8_t *>(b + sparse_at), d);
+  if (se->host)
+    slot->frame = b + frame_at;
   *out = slot;
   return hipSuccess;
 }
 
-size_t slot_bytes(int pixel_bytes, uint32_t height, uint32_t width) {
+size_t slot_bytes(int pixel_bytes, uint32_t height, uint32_t width,
+                  bool host) {
   sf::FrameDesc d{};
   sf::fused_layout(pixel_bytes, height, width, d);
   const size_t cap = std::max<size_t>(d.capacity, 1);
   const size_t cells = sf::cell_entries(height, width);
   return 2 * cap * sizeof(sf_signal_pixel) + cells * sizeof(uint32_t) +
          (((size_t)d.n_bands + 1) * sizeof(uint32_t)) +
-         cells + 255 + sparse_part(pixel_bytes, d);
+         cells + 255 + sparse_part(pixel_bytes, d) + 255 +
+         (host ? (size_t)pixel_bytes * height * width : 0);
 }
 
 hipError_t make_batch(int capacity, Batch **out) {
@@ -242,13 +258,21 @@ hipError_t make_batch(int capacity, Batch **out) {
   if (e == hipSuccess)
     e = hipMalloc(&b->d_desc, sizeof(sf::FrameDesc) * capacity);
   if (e == hipSuccess)
-    e = hipHostMalloc(&b->h_desc, sizeof(sf::FrameDesc) * capacity);
+    e = hipHostMalloc(&b->h_desc, sizeof(sf::FrameDesc) * capacity,
+                      hipHostMallocMapped);
+  if (e == hipSuccess)
+    e = hipHostGetDevicePointer(reinterpret_cast<void **>(&b->hd_desc),
+                                b->h_desc, 0);
   if (e == hipSuccess)
     e = hipMalloc(&b->d_counters, sizeof(uint32_t) * 2 * capacity);
   if (e == hipSuccess) {
     b->d_found = b->d_counters + capacity;
-    e = hipHostMalloc(&b->h_found, sizeof(uint32_t) * capacity);
+    e = hipHostMalloc(&b->h_found, sizeof(uint32_t) * capacity,
+                      hipHostMallocMapped);
   }
+  if (e == hipSuccess)
+    e = hipHostGetDevicePointer(reinterpret_cast<void **>(&b->hd_found),
+                                b->h_found, 0);
   if (e == hipSuccess)
     e = hipEventCreateWithFlags(&b->masked, hipEventDisableTiming);
   if (e == hipSuccess)
@@ -275,15 +299,40 @@ void free_batch(Batch *b) {
   delete b;
 }
 
+// n words from `from` to `to` by one block (either side host-mapped).
+__global__ __launch_bounds__(256) void move_words(const uint32_t *from,
+                                                  uint32_t *to, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    to[i] = from[i];
+  if (threadIdx.x == 0)
+    __threadfence_system();
+}
+
 // Enqueue a formed batch on its stream (no lock held).
 hipError_t launch(Batch *b) {
   const int n = (int)b->requests.size();
   uint32_t items = 0, max_bands = 0, max_cell_rows = 0;
   bool any_vec = false, any_scalar = false;
   const uint32_t k = b->pixel_bytes == 2 ? 4 : 2;
+  // host-fed frames first come in, on the batch's own stream
+  b->ingested = 0;
+  for (int f = 0; f < n; f++) {
+    const Slot *slot = b->slots[f];
+    if (!slot->series->host)
+      continue;
+    const size_t bytes = (size_t)slot->series->pixel_bytes *
+                         slot->series->height * slot->series->width;
+    const hipError_t e =
+        hipMemcpyAsync(slot->frame, b->requests[f].pixels, bytes,
+                       hipMemcpyHostToDevice, b->stream);
+    if (e != hipSuccess)
+      return e;
+    b->ingested += bytes;
+  }
   for (int f = 0; f < n; f++) {
     sf::FrameDesc d = b->slots[f]->desc;
-    d.pixels = b->requests[f].pixels;
+    d.pixels = b->slots[f]->series->host ? b->slots[f]->frame
+                                         : b->requests[f].pixels;
     d.counter = b->d_counters + f;
     d.out_count = b->d_found + f;
     d.item_begin = items;
@@ -301,15 +350,20 @@ hipError_t launch(Batch *b) {
     any_scalar |= d.vec == 0;
     b->h_desc[f] = d;
   }
-  hipError_t e =
-      hipMemcpyAsync(b->d_desc, b->h_desc, sizeof(sf::FrameDesc) * n,
-                     hipMemcpyHostToDevice, b->stream);
+  static_assert(sizeof(sf::FrameDesc) % 4 == 0, "descriptors move as words");
+  move_words<<<1, 256, 0, b->stream>>>(
+      reinterpret_cast<const uint32_t *>(b->hd_desc),
+      reinterpret_cast<uint32_t *>(b->d_desc),
+      (uint32_t)(sizeof(sf::FrameDesc) / 4 * n));
+  hipError_t e = hipGetLastError();
   if (e == hipSuccess)
     e = sf::launch_batch(b->pixel_bytes, any_vec, any_scalar, b->d_desc, n,
                          items, max_bands, max_cell_rows, b->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(b->h_found, b->d_found, sizeof(uint32_t) * n,
-                       hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) {
+    move_words<<<1, 256, 0, b->stream>>>(b->d_found, b->hd_found,
+                                         (uint32_t)n);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess)
     e = hipEventRecord(b->masked, b->stream);
   return e;
@@ -438,6 +492,7 @@ void scheduler_loop(sf_server *s) {
           set_failure(s, SF_DEVICE_ERROR, hip_message("launching a batch", e));
         s->n_batches++;
         s->batched_frames += b->requests.size();
+        s->ingested += b->ingested;
         s->inflight.push_back(b);
         s->max_inflight = std::max(
             s->max_inflight, (int)(s->batches.size() - s->free_batches.size()));
@@ -709,13 +764,17 @@ void sf_hip_server_destroy(sf_server *s) {
   delete s;
 }
 
-int sf_hip_server_open(sf_server *s, int pixel_bytes, size_t height,
-                       size_t width, const sf_options *options) {
+} // extern "C"
+
+namespace {
+
+int open_series(sf_server *s, int pixel_bytes, size_t height, size_t width,
+                const sf_options *options, bool host) {
   if (!s || (pixel_bytes != 2 && pixel_bytes != 4) || height == 0 ||
       width == 0 || height > 0x7fffffffu / width)
     return fail_arg("sf_hip_server_open: bad argument");
   const size_t bytes =
-      slot_bytes(pixel_bytes, (uint32_t)height, (uint32_t)width);
+      slot_bytes(pixel_bytes, (uint32_t)height, (uint32_t)width, host);
   if (bytes > s->budget)
     return fail_arg("sf_hip_server_open: one frame's buffers exceed the "
                     "HBM budget");
@@ -729,6 +788,7 @@ int sf_hip_server_open(sf_server *s, int pixel_bytes, size_t height,
   se->pixel_bytes = pixel_bytes;
   se->height = (uint32_t)height;
   se->width = (uint32_t)width;
+  se->host = host;
   se->labeller = lab;
   se->slot_bytes = bytes;
   std::lock_guard<std::mutex> lk(s->m);
@@ -736,6 +796,20 @@ int sf_hip_server_open(sf_server *s, int pixel_bytes, size_t height,
   s->series.push_back(se);
   se->worker = std::thread(series_loop, s, se);
   return se->id;
+}
+
+} // namespace
+
+extern "C" {
+
+int sf_hip_server_open(sf_server *s, int pixel_bytes, size_t height,
+                       size_t width, const sf_options *options) {
+  return open_series(s, pixel_bytes, height, width, options, false);
+}
+
+int sf_hip_server_open_host(sf_server *s, int pixel_bytes, size_t height,
+                            size_t width, const sf_options *options) {
+  return open_series(s, pixel_bytes, height, width, options, true);
 }
 
 int sf_hip_server_submit_at(sf_server *s, int series, int64_t frame,
@@ -870,6 +944,7 @@ int sf_hip_server_stats(sf_server *s, sf_server_stats *stats) {
   stats->peak_bytes = s->peak;
   stats->budget = s->budget;
   stats->overflowed = s->overflowed;
+  stats->ingested_bytes = s->ingested;
   return SF_OK;
 }
 
@@ -878,7 +953,7 @@ size_t sf_hip_server_slot_bytes(int pixel_bytes, size_t height,
   if ((pixel_bytes != 2 && pixel_bytes != 4) || !height || !width ||
       height > 0x7fffffffu / width)
     return 0;
-  return slot_bytes(pixel_bytes, (uint32_t)height, (uint32_t)width);
+  return slot_bytes(pixel_bytes, (uint32_t)height, (uint32_t)width, false);
 }
 
 int sf_hip_server_play(sf_server *s, int n_series, const int *series,

@@ -86,7 +86,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "batches", "batched_frames", "admission_waits",
         "max_batches_in_flight", "bytes_in_use", "peak_bytes", "budget",
-        "overflowed")]
+        "overflowed", "ingested_bytes")]
 
 
 def _declare(lib: ctypes.CDLL) -> None:
@@ -167,6 +167,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_server_create", _p, _sz, _i, _i)
     fn("sf_hip_server_destroy", None, _p)
     fn("sf_hip_server_open", _i, _p, _i, _sz, _sz, ctypes.POINTER(Options))
+    fn("sf_hip_server_open_host", _i, _p, _i, _sz, _sz,
+       ctypes.POINTER(Options))
     fn("sf_hip_server_submit", _i, _p, _i, ctypes.c_int64, _p)
     fn("sf_hip_server_submit_at", _i, _p, _i, ctypes.c_int64, _p,
        ctypes.c_double)
@@ -348,6 +350,37 @@ class DeviceBuffer:
     def close(self) -> None:
         if self.ptr:
             lib().sf_hip_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HostBuffer:
+    """Pinned host memory (hipHostMalloc), freed on close()/garbage
+    collection; `array(dtype, shape)` views it."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().sf_hip_host_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise DeviceError(f"pinned allocation of {nbytes} bytes: "
+                              f"{last_error()}")
+
+    def array(self, dtype, shape, offset: int = 0) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        count = int(np.prod(shape))
+        assert offset + count * dtype.itemsize <= self.nbytes
+        raw = (ctypes.c_uint8 * (count * dtype.itemsize)).from_address(
+            self.ptr + offset)
+        return np.frombuffer(raw, dtype=dtype, count=count).reshape(shape)
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().sf_hip_host_free(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -654,11 +687,16 @@ class Server:
 
     def open(self, pixel_bytes: int, height: int, width: int,
              min_spot_size: int = 3, max_spot_size: int = 1000,
-             max_separation: float = 2.0, two_d: bool = False) -> int:
+             max_separation: float = 2.0, two_d: bool = False,
+             host: bool = False) -> int:
+        """Open a series. host: its frames are submitted as pointers to
+        (pinned) host memory and copied in by the serving loop."""
         options = Options(min_spot_size, max_spot_size, float(max_separation),
                           1 if two_d else 0)
-        sid = lib().sf_hip_server_open(self.handle, pixel_bytes, height,
-                                       width, ctypes.byref(options))
+        opener = (lib().sf_hip_server_open_host if host
+                  else lib().sf_hip_server_open)
+        sid = opener(self.handle, pixel_bytes, height, width,
+                     ctypes.byref(options))
         if sid < 0:
             self._status(sid, "open")
         return sid

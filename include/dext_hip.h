@@ -326,6 +326,8 @@ typedef struct sf_server_stats {
    * slot's capacity) and the labeller takes them as empty frames, as the
    * driver does not group a frame that failed */
   uint64_t overflowed;
+  /* bytes of host-fed frames (sf_hip_server_open_host) copied to the device */
+  uint64_t ingested_bytes;
 } sf_server_stats;
 /* hbm_budget: bytes of slots the server may hold; max_batch_frames: frames
  * per launch; keep_lists: keep every frame's ordered list on the host (for
@@ -338,6 +340,17 @@ void sf_hip_server_destroy(sf_server *server);
  * options are as sf_hip_labeller_create's. */
 int sf_hip_server_open(sf_server *server, int pixel_bytes, size_t height,
                        size_t width, const sf_options *options);
+/* Opens a series whose frames arrive in host memory: sf_hip_server_submit
+ * then takes a host pointer (pinned -- sf_hip_host_alloc -- for the copy to
+ * overlap; it must stay unchanged until the frame completes). Each frame is
+ * copied into its slot (which holds a frame more) on the serving loop's
+ * batch stream just before its batch's kernels, so its latencies include
+ * the copy. No counterpart in the reference, whose driver reads frames
+ * into host memory and copies each in on its worker's stream
+ * (src/find_spots.cc:205-221, src/dext_cuda.cu:633). */
+int sf_hip_server_open_host(sf_server *server, int pixel_bytes,
+                            size_t height, size_t width,
+                            const sf_options *options);
 /* Hands over one frame (device memory, valid until the frame completes);
  * frame numbers ascend per series. Returns at once. _at stamps the arrival
  * at a time of sf_hip_server_clock() (seconds) instead of now. */

@@ -257,3 +257,57 @@ def test_plan_rerun_and_closed_right_after_add_plan(sf):
     assert expected[2]["accepted"] > 3
     lab.close()
     buf.close()
+
+
+def test_add_lists_from_device_memory(sf):
+    """Lists already in device memory (sf_hip_labeller_add_lists) over two
+    calls with a gap in the frame numbers, against the oracle fed the same
+    lists: blobs of 21 pixels spanning three frames."""
+    h, w, n = 400, 500, 12
+    rng = np.random.default_rng(21)
+    frames = [[] for _ in range(n)]
+    for _ in range(120):
+        y, x = int(rng.integers(3, h - 3)), int(rng.integers(3, w - 3))
+        first = int(rng.integers(0, n - 3))
+        for step in range(3):
+            for dy in range(-2, 3):
+                for dx in range(-2, 3):
+                    if dy * dy + dx * dx <= 5:
+                        v = 50 + int(1500 * np.exp(-(dy * dy + dx * dx) / 4)
+                                     * np.exp(-(step - 1) ** 2 / 4))
+                        frames[first + step].append(((y + dy) * w + x + dx, v))
+    lists = []
+    for f in range(n):
+        pts = dict(sorted(frames[f]))  # one value an index
+        sig = np.zeros(len(pts), sf.SIGNAL)
+        sig["index"] = list(pts.keys())
+        sig["value"] = list(pts.values())
+        lists.append(sig)
+    numbers = [10 + f for f in range(6)] + [20 + f for f in range(6)]
+    oracle = cpu.Labeller(h, w)
+    for z, sig in zip(numbers, lists):
+        assert oracle.add(z, sig) == 0
+    oracle.finish()
+    want = oracle.results()
+    counts = [len(x) for x in lists]
+    buf = sf.DeviceBuffer(16 * sum(counts))
+    buf.upload(np.concatenate(lists))
+    cbuf = sf.DeviceBuffer(4 * n)
+    cbuf.upload(np.array(counts, np.uint32))
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    ptrs = [buf.ptr + 16 * int(s) for s in starts]
+    cptrs = [cbuf.ptr + 4 * k for k in range(n)]
+    lab = sf.Labeller(h, w)
+    for a, b in ((0, 6), (6, 12)):
+        lab.add_lists(numbers[a:b], ptrs[a:b], cptrs[a:b], counts[a:b],
+                      counts[a:b])
+    with pytest.raises(ValueError):  # not above the last frame
+        lab.add_lists([5], ptrs[:1], cptrs[:1], counts[:1], counts[:1])
+    lab.finish()
+    got = lab.results()
+    assert got[2] == want[2]
+    assert same_tables(got, want)
+    assert want[2]["accepted"] > 20
+    lab.close()
+    buf.close()
+    cbuf.close()

@@ -41,19 +41,22 @@ def make_series(sf):
     return out
 
 
-def run(sf, data, which, budget, max_batch, rates=None):
+def run(sf, data, which, budget, max_batch, rates=None, host=None):
+    """host: per series, pinned host pointers of its frames (host-fed)."""
     server = sf.Server(budget, max_batch=max_batch, keep_lists=True)
     ids = []
     for i in which:
         pb, h, w = SERIES[i][:3]
-        ids.append(server.open(pb, h, w, **data[i][3]))
+        ids.append(server.open(pb, h, w, host=host is not None,
+                               **data[i][3]))
+    ptrs = {i: (host[i] if host is not None else data[i][2]) for i in which}
     if rates is None:
         for sid, i in zip(ids, which):
-            for k, p in enumerate(data[i][2]):
+            for k, p in enumerate(ptrs[i]):
                 server.submit(sid, 1000 + k, p)
     else:
-        server.play(ids, rates, [len(data[i][2]) for i in which],
-                    [data[i][2] for i in which], first_frame=1000)
+        server.play(ids, rates, [len(ptrs[i]) for i in which],
+                    [ptrs[i] for i in which], first_frame=1000)
     out = {}
     for sid, i in zip(ids, which):
         server.close_series(sid)
@@ -109,6 +112,36 @@ def test_alone_equals_mix_equals_oracle(sf):
         # latencies are recorded for every frame and ordered sensibly
         assert (m_frames["mask_ms"] > 0).all()
         assert (m_frames["table_ms"] >= m_frames["mask_ms"]).all()
+
+
+def test_host_fed_series_equal_device_fed(sf):
+    """Series whose frames arrive in pinned host memory (copied in on the
+    batch streams) give the same lists and tables as device-fed ones, in a
+    mix under a tight budget, and the copied bytes are counted."""
+    data = make_series(sf)
+    pinned, host_ptrs = [], {}
+    for i, (pb, h, w, n, *_rest) in enumerate(SERIES):
+        hb = sf.HostBuffer(pb * h * w * n)
+        hb.array(data[i][1].dtype, data[i][1].shape)[:] = data[i][1]
+        pinned.append(hb)
+        host_ptrs[i] = [hb.ptr + pb * h * w * k for k in range(n)]
+    tight = sum(sf.server_slot_bytes(*s[:3]) for s in SERIES) * 3
+    which = [0, 1, 2, 3]
+    fed, stats = run(sf, data, which, tight, max_batch=5,
+                     rates=[3000.0, 2000.0, 1500.0, 2500.0], host=host_ptrs)
+    dev, _ = run(sf, data, which, 1 << 30, max_batch=8)
+    assert stats["ingested_bytes"] == sum(pb * h * w * n
+                                          for pb, h, w, n, *_ in SERIES)
+    for i in which:
+        f_frames, f_lists, f_tables = fed[i]
+        d_frames, d_lists, d_tables = dev[i]
+        assert (f_frames["count"] == d_frames["count"]).all(), i
+        for k in range(len(d_lists)):
+            assert f_lists[k].tobytes() == d_lists[k].tobytes(), (i, k)
+        assert same_tables(f_tables, d_tables), i
+        assert (f_frames["mask_ms"] > 0).all()
+    for hb in pinned:
+        hb.close()
 
 
 def test_submit_rules(sf):
