@@ -1102,29 +1102,30 @@ def main() -> int:
             u32_host, baseline["host"]["nproc"], args.cpu_repeats)
         del u32_host
 
-    latency, serving, serving_device = None, None, None
+    latency, latency_host, serving, serving_host = None, None, None, None
     if args.serve_seconds > 0:
         plan.close()
         labeller = None
         series.close()
-        # frames from pinned host memory (the headline latency: it includes
-        # the copy), then the same from device memory
+        # device-resident frames (the serving workload as specified), then
+        # the same arriving in pinned host memory, copied in by the loop
+        # (latency includes the copy; a host-fed slot holds a frame more,
+        # so twice the budget), and the 750 Hz series alone that way
         serving, worst = serve_latency(sf, fr, args.serve_seconds,
-                                       min(args.serve_ring, 12),
-                                       args.serve_budget_gb, args.serve_batch,
-                                       host=True)
+                                       args.serve_ring, args.serve_budget_gb,
+                                       args.serve_batch)
         latency = round(worst, 3)
-        log(f"serving from host memory: {serving}")
-        serving["750_hz_series_alone"], _ = serve_latency(
+        log(f"serving from device memory: {serving}")
+        serving_host, worst_host = serve_latency(
             sf, fr, args.serve_seconds, min(args.serve_ring, 12),
-            args.serve_budget_gb, args.serve_batch, host=True, which=[0])
+            2 * args.serve_budget_gb, args.serve_batch, host=True)
+        latency_host = round(worst_host, 3)
+        log(f"serving from host memory: {serving_host}")
+        serving_host["750_hz_series_alone"], _ = serve_latency(
+            sf, fr, args.serve_seconds, min(args.serve_ring, 12),
+            2 * args.serve_budget_gb, args.serve_batch, host=True, which=[0])
         log(f"the 750 Hz series alone from host memory: "
-            f"{serving['750_hz_series_alone']}")
-        serving_device, _ = serve_latency(sf, fr, args.serve_seconds,
-                                          args.serve_ring,
-                                          args.serve_budget_gb,
-                                          args.serve_batch)
-        log(f"serving from device memory: {serving_device}")
+            f"{serving_host['750_hz_series_alone']}")
 
     record = {
         "metric": METRIC,
@@ -1202,8 +1203,9 @@ def main() -> int:
         "table_diffs": baseline["table_diffs"],
         "cpu_baseline": baseline,
         "p99_latency_ms": latency,
+        "p99_latency_host_ms": latency_host,
         "serving": serving,
-        "serving_device": serving_device,
+        "serving_host": serving_host,
         "other_workloads": others,
         "labeller_sweep": sweep,
     }
