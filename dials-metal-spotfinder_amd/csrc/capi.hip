@@ -44,11 +44,6 @@ hipError_t launch_sparse32_pipeline(const FrameDesc *, int, uint32_t,
 hipError_t launch_sparse_pipeline(const FrameDesc *, int, uint32_t, uint32_t,
                                   const FrameDesc *, int, uint32_t,
                                   hipStream_t, hipEvent_t *);
-hipError_t launch_probe(int, const FrameDesc *, int, uint32_t, int,
-                        hipStream_t);
-int probe_shapes();
-int probe_tile_width(int shape);
-int probe_tile_height(int shape);
 } // namespace sf
 
 namespace {
@@ -135,10 +130,9 @@ struct sf_plan {
   std::vector<uint32_t> heights, widths;
   std::vector<uint32_t> capacity;
   int band_rows = 236;
-  int form = 0;   // 0 fused (sparse passes at 16 bits, sweep at 32),
-                  // 1 plain, 2 sweep, 3 probe tiles (at 16 bits)
+  int form = 0;   // 0 fused (sparse passes; the sweep for 32-bit frames
+                  // of 2^29 pixels or more), 1 plain, 2 sweep
   int blocks = 0; // 0: one block per item
-  int tile = 0;   // probe kernel tile shape (sf::probe_tile_width/height)
   hipStream_t stream = nullptr;
   bool timing = false;
 
@@ -178,7 +172,7 @@ struct sf_plan {
 namespace {
 
 // The kernel a form runs for this plan's pixel type.
-enum Kernel { kSweep, kPlain, kProbe, kSparse };
+enum Kernel { kSweep, kPlain, kSparse };
 
 Kernel kernel_for(const sf_plan *p, int form) {
   if (form == 1)
@@ -187,8 +181,6 @@ Kernel kernel_for(const sf_plan *p, int form) {
   // offsets: frames below 2^29 pixels (the sweep takes larger ones)
   if (form == 0 && (p->pixel_bytes == 2 || p->max_pixels < (1u << 29)))
     return kSparse;
-  if (form == 3 && p->pixel_bytes == 2)
-    return kProbe;
   return kSweep;
 }
 
@@ -196,9 +188,6 @@ uint32_t strips_for(const sf_plan *p, uint32_t width, int form) {
   const Kernel k = kernel_for(p, form);
   if (k == kPlain)
     return (width + 63) / 64;
-  if (k == kProbe)
-    return (width + sf::probe_tile_width(p->tile) - 1) /
-           sf::probe_tile_width(p->tile);
   const uint32_t ow = p->pixel_bytes == 2 ? sf::strip_width<uint16_t>()
                                           : sf::strip_width<uint32_t>();
   return (width + ow - 1) / ow;
@@ -267,9 +256,7 @@ void layout(sf_plan *p) {
       p->max_cell_rows = std::max(p->max_cell_rows, d.n_cell_rows);
     } else {
       d.n_strips = strips_for(p, d.width, p->form);
-      d.band_rows = kernel == kProbe
-                        ? (uint32_t)sf::probe_tile_height(p->tile)
-                        : (uint32_t)p->band_rows;
+      d.band_rows = (uint32_t)p->band_rows;
       d.n_bands = (d.height + d.band_rows - 1) / d.band_rows;
     }
     d.item_begin = item;
@@ -354,15 +341,7 @@ int enqueue(sf_plan *p, bool timing) {
   SF_CHECK(sf::launch_zero(p->d_desc, p->n, p->stream), "zeroing counters");
   if (timing)
     SF_CHECK(hipEventRecord(p->ev[0], p->stream), "hipEventRecord");
-  if (kernel == kProbe) {
-    SF_CHECK(sf::launch_probe(p->tile, p->d_desc, p->n, p->items, p->blocks,
-                              p->stream),
-             "launching the probe threshold");
-    if (timing) {
-      SF_CHECK(hipEventRecord(p->ev[1], p->stream), "hipEventRecord");
-      SF_CHECK(hipEventRecord(p->ev[2], p->stream), "hipEventRecord");
-    }
-  } else if (kernel == kSweep) {
+  if (kernel == kSweep) {
     SF_CHECK(sf::launch_fused(p->pixel_bytes, p->any_vec, p->any_scalar,
                               p->d_desc, p->n, p->items, p->blocks,
                               p->stream),
@@ -425,13 +404,9 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
     staging += p->capacity.back();
     sparse += sf::sparse_bytes(heights[f], widths[f], p->capacity.back());
     cells += (size_t)heights[f] * ((widths[f] + 63) / 64);
-    // enough band totals for the shortest band any form or tile uses
-    int shortest_tile = sf::probe_tile_height(0);
-    for (int t = 1; t < sf::probe_shapes(); t++)
-      shortest_tile = std::min(shortest_tile, sf::probe_tile_height(t));
-    const uint32_t shortest =
-        (uint32_t)std::min(p->band_rows, shortest_tile);
-    bands += (heights[f] + shortest - 1) / shortest;
+    // band totals for the plan's band (the autotuner grows them for
+    // shorter candidates)
+    bands += (heights[f] + (uint32_t)p->band_rows - 1) / (uint32_t)p->band_rows;
     p->max_pixels = px > p->max_pixels ? px : p->max_pixels;
   }
   p->desc.resize(n_frames);
@@ -488,7 +463,7 @@ void sf_hip_plan_destroy(sf_plan *plan) {
 }
 
 int sf_hip_plan_set_form(sf_plan *p, int form) {
-  if (!p || form < 0 || form > 3)
+  if (!p || form < 0 || form > 2)
     return bad("sf_hip_plan_set_form: bad argument");
   if (form == p->form)
     return SF_OK;
@@ -537,15 +512,8 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
         hipGetDeviceProperties(&prop, device) == hipSuccess)
       cus = prop.multiProcessorCount;
   }
-  // (band rows, tile shape) candidates: band heights for the sweep kernel,
-  // tile shapes for the probe kernel
-  std::vector<std::pair<int, int>> shapes;
-  if (kernel == kSweep)
-    for (const int band : {44, 108, 236, 492})
-      shapes.emplace_back(band, p->tile);
-  else
-    for (int tile = 0; tile < sf::probe_shapes(); tile++)
-      shapes.emplace_back(p->band_rows, tile);
+  // candidate band heights (the rows a sweep tile covers)
+  const int bands[] = {44, 108, 236, 492};
   const int grids[] = {0, 4 * cus, 8 * cus, 16 * cus};
   // band totals for the shortest candidate band
   size_t need_bands = 0;
@@ -563,7 +531,7 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
   p->graph_form = -1;
   p->scouted = false;
   drop_pipe_graph(p);
-  if (kernel == kSweep && need_bands > p->band_capacity) {
+  if (need_bands > p->band_capacity) {
     uint32_t *grown = nullptr;
     SF_CHECK(hipMalloc(&grown, sizeof(uint32_t) * need_bands),
              "allocating band totals");
@@ -573,11 +541,9 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
   }
   const bool was_timing = p->timing;
   double best = 1e30;
-  int best_band = p->band_rows, best_grid = p->blocks, best_tile = p->tile;
-  for (const auto &shape : shapes) {
-    const int band = shape.first;
+  int best_band = p->band_rows, best_grid = p->blocks;
+  for (const int band : bands) {
     p->band_rows = band;
-    p->tile = shape.second;
     layout(p);
     p->bound.clear(); // force a descriptor upload
     for (const int grid : grids) {
@@ -601,48 +567,23 @@ int sf_hip_plan_autotune(sf_plan *p, const void *const *frames, int trials,
         best = median;
         best_band = band;
         best_grid = grid;
-        best_tile = shape.second;
       }
     }
   }
   p->timing = was_timing;
   p->band_rows = best_band;
   p->blocks = best_grid;
-  p->tile = best_tile;
   layout(p);
   p->bound.clear();
   SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
   p->upload_pending = false;
-  if (band_rows) // the rows a tile covers: the band, or the probe tile
-    *band_rows = kernel == kProbe ? sf::probe_tile_height(best_tile)
-                                  : best_band;
+  if (band_rows)
+    *band_rows = best_band;
   if (blocks)
     *blocks = best_grid;
   return SF_OK;
 }
 
-int sf_hip_plan_set_tile(sf_plan *p, int shape) {
-  if (!p || shape < 0 || shape >= sf::probe_shapes())
-    return bad("sf_hip_plan_set_tile: bad argument");
-  SF_CHECK(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
-  if (p->exec) {
-    (void)hipGraphExecDestroy(p->exec);
-    p->exec = nullptr;
-  }
-  if (p->graph) {
-    (void)hipGraphDestroy(p->graph);
-    p->graph = nullptr;
-  }
-  p->graph_form = -1;
-  p->scouted = false;
-  drop_pipe_graph(p);
-  p->tile = shape;
-  layout(p);
-  p->bound.clear();
-  return SF_OK;
-}
-
-int sf_hip_plan_tile(sf_plan *p) { return p ? p->tile : -1; }
 
 int sf_hip_plan_set_timing(sf_plan *p, int on) {
   if (!p)

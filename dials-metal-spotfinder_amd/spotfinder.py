@@ -124,8 +124,6 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_plan_set_grid", _i, _p, _i)
     fn("sf_hip_plan_autotune", _i, _p, ctypes.POINTER(_p), _i,
        ctypes.POINTER(_i), ctypes.POINTER(_i))
-    fn("sf_hip_plan_set_tile", _i, _p, _i)
-    fn("sf_hip_plan_tile", _i, _p)
     fn("sf_hip_plan_run", _i, _p, ctypes.POINTER(_p))
     fn("sf_hip_plan_bind", _i, _p, ctypes.POINTER(_p))
     fn("sf_hip_plan_capture", _i, _p, ctypes.POINTER(_p))
@@ -418,10 +416,10 @@ def synth_frames(buffer: DeviceBuffer, pixel_bytes: int, height: int,
 class Plan:
     """Fixed frame shapes thresholded together on the device.
 
-    form 0 is the fused default (the sparse passes at 16 bits, the register
-    sweep at 32), form 1 the plain three-kernel per-frame form, form 2 the
-    register sweep, form 3 the probe-lattice tile kernel (16 bits). Frames
-    are device pointers (ints).
+    form 0 is the fused default (the sparse passes; the register sweep for
+    32-bit frames of 2^29 pixels or more), form 1 the plain three-kernel
+    per-frame form, form 2 the register sweep. Frames are device pointers
+    (ints).
     """
 
     def __init__(self, pixel_bytes: int, shapes, band_rows: int = 0,
@@ -456,14 +454,6 @@ class Plan:
                                           trials, ctypes.byref(band),
                                           ctypes.byref(blocks)), "autotune")
         return band.value, blocks.value
-
-    def set_tile(self, shape: int) -> None:
-        """Probe kernel (form 3) tile: 0 = 64x104, 1 = 128x104, 2 = 64x232,
-        3 = 32x104."""
-        _check(lib().sf_hip_plan_set_tile(self.handle, shape), "set_tile")
-
-    def tile(self) -> int:
-        return int(lib().sf_hip_plan_tile(self.handle))
 
     def set_timing(self, on: bool) -> None:
         _check(lib().sf_hip_plan_set_timing(self.handle, 1 if on else 0),

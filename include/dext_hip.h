@@ -161,11 +161,12 @@ int sf_hip_find_static(int pixel_bytes, const void *frame, size_t height,
  * owns the staging, cell tables and ordered output lists for them. run()
  * thresholds every frame of the plan in one sequence of launches (threshold,
  * then ordering), asynchronously on the plan's stream.
- * form 0 = fused (default: the sparse passes for 16-bit frames -- a probe
- * per 5x5 cell, then erosion, Poisson test and ordering only around active
- * cells -- the register-sweep kernel for 32-bit ones), 1 = plain
- * three-kernel form, one frame per launch sequence, 2 = the register-sweep
- * kernel, 3 = the probe-lattice tile kernel (16-bit; the sweep at 32). */
+ * form 0 = fused (default: the sparse passes -- a probe per 5x5 cell, then
+ * erosion, Poisson test and ordering only around active cells -- for 16-bit
+ * frames and for 32-bit frames below 2^29 pixels; the register-sweep kernel
+ * for larger 32-bit ones), 1 = plain three-kernel form, one frame per launch
+ * sequence, 2 = the register-sweep kernel. Any other form is refused with
+ * SF_BAD_ARGUMENT. */
 typedef struct sf_plan sf_plan;
 sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
                             const uint32_t *heights, const uint32_t *widths,
@@ -179,20 +180,12 @@ int sf_hip_plan_set_grid(sf_plan *plan, int blocks);
  * these frames (each `trials` times, after a warm-up run) and keeps the
  * fastest. For the sweep kernel the candidates are band heights (the tile's
  * rows: 44, 108, 236, 492) crossed with grids (0 and 4 / 8 / 16 resident
- * blocks a CU); for the probe kernel, its four tile shapes crossed with the
- * same grids; the sparse passes' launch shape follows the frames, so they
+ * blocks a CU); the sparse passes' launch shape follows the frames, so they
  * have nothing to time (band_rows reports their 80-row scout band). Reports
- * the chosen tile rows and grid through band_rows / blocks (either may be
+ * the chosen band rows and grid through band_rows / blocks (either may be
  * NULL). Drops a captured graph: capture again. */
 int sf_hip_plan_autotune(sf_plan *plan, const void *const *device_frames,
                          int trials, int *band_rows, int *blocks);
-/* The probe kernel's (form 3) tile shape: 0 = 64 x 104 (256 threads, the
- * default),
- * 1 = 128 x 104 and 2 = 64 x 232 (512 threads each), 3 = 32 x 104 (128
- * threads). The autotuner tries all of them; sf_hip_plan_tile reports the
- * current one. Drops a captured graph. */
-int sf_hip_plan_set_tile(sf_plan *plan, int shape);
-int sf_hip_plan_tile(sf_plan *plan);
 int sf_hip_plan_run(sf_plan *plan, const void *const *device_frames);
 /* Point the plan at these device frames without launching (the descriptor
  * upload that run() does first). A captured graph reads the descriptors at

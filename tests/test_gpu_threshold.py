@@ -105,14 +105,12 @@ def test_random_frames_edges_and_both_paths(sf):
             assert same(got, expected), (shape, dtype, describe(got, expected))
 
 
-@pytest.mark.parametrize("form,tile", [(0, 0), (3, 0), (3, 1), (3, 2), (3, 3),
-                                       (2, 0)])
-def test_random_frames_device_resident(sf, form, tile):
+@pytest.mark.parametrize("form", [0, 2])
+def test_random_frames_device_resident(sf, form):
     """The same edge cases through device-resident plans: the sparse passes
-    (form 0 at 16 bits), the probe tile kernel (form 3) in each of its four
-    tile shapes, and the register-sweep kernel (form 2), including widths
-    that are not a multiple of 4 and an unaligned frame start."""
-    rng = np.random.default_rng(40 + form + 7 * tile)
+    (form 0) and the register-sweep kernel (form 2), including widths that
+    are not a multiple of 4 and an unaligned frame start."""
+    rng = np.random.default_rng(40 + form)
     shapes = [(1, 1), (2, 3), (9, 1), (5, 5), (7, 300), (300, 7), (33, 65),
               (64, 64), (129, 257), (250, 232), (250, 233), (251, 464),
               (97, 105), (128, 104), (65, 209)]
@@ -126,7 +124,6 @@ def test_random_frames_device_resident(sf, form, tile):
                 buf = sf.DeviceBuffer(frame.nbytes + 16)
                 buf.upload(frame, offset=offset)
                 plan = sf.Plan(pb, [shape], form=form)
-                plan.set_tile(tile)
                 plan.run([buf.ptr + offset])
                 plan.wait()
                 n = int(plan.counts()[0])
@@ -201,15 +198,14 @@ def test_full_size_model_frames_fused_and_plain(sf):
     buf, host = model_frames(sf, 2, h, w, seed=101)
     expected = [cpu.dext(host[f], capacity=h * w // 8)[1] for f in range(2)]
     assert all(len(e) > 1000 for e in expected)
-    for form, tile in ((0, 0), (3, 0), (3, 1), (3, 2), (3, 3), (1, 0), (2, 0)):
+    for form in (0, 1, 2):
         plan = sf.Plan(2, [(h, w)] * 2, form=form)
-        plan.set_tile(tile)
         plan.run([buf.ptr, buf.ptr + 2 * h * w])
         plan.wait()
         counts = plan.counts()
         for f in range(2):
             got = plan.output(f, int(counts[f]))
-            assert same(got, expected[f]), (form, tile, f,
+            assert same(got, expected[f]), (form, f,
                                              describe(got, expected[f]))
         plan.close()
 
@@ -218,7 +214,7 @@ def test_batched_uneven_sizes_graph_and_persistent(sf):
     """The uneven-chunk shape mix (2162x2068, 3269x3110, 4362x4148, scaled
     to keep the oracle quick) in one launch, replayed from a captured graph
     with eager work between replays, through a persistent grid, and after
-    the autotuner has picked its launch shape (sparse, sweep and probe
+    the autotuner has picked its launch shape (sparse and sweep
     kernels)."""
     shapes = [(216, 206), (327, 311), (436, 414), (216, 206), (101, 3)]
     bufs, hosts = [], []
@@ -259,7 +255,10 @@ def test_batched_uneven_sizes_graph_and_persistent(sf):
     counts = plan.counts()
     for f in range(len(shapes)):
         assert same(plan.output(f, int(counts[f])), expected[f]), ("grid", f)
-    for form in (0, 2, 3):
+    for form in (-1, 3):  # no such forms
+        with pytest.raises(ValueError, match="set_form"):
+            plan.set_form(form)
+    for form in (0, 2):
         plan.set_form(form)
         band, blocks = plan.autotune(ptrs, 1)
         assert blocks >= 0 and band > 0
