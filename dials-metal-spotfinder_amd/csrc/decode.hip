@@ -48,7 +48,8 @@ struct Block {
   uint32_t comp;  // LZ4 bytes (0 for a raw copy)
   uint32_t bytes; // decoded bytes
   uint32_t chunk;
-  uint32_t kind;
+  uint32_t room;  // plain LZ4: the output room the block is decoded with
+                  // (its bytes may be fewer); otherwise equal to bytes
 };
 static_assert(sizeof(Block) == 32, "block table entry");
 
@@ -58,6 +59,51 @@ uint32_t be32(const uint8_t *p) {
 }
 
 size_t lz4_bound(size_t n) { return n + n / 255 + 16; }
+
+// The bytes one LZ4 block decodes to with `room` bytes of output, under the
+// rules wave_lz4 decodes by (the published block format, with
+// LZ4_decompress_safe's end-of-block conditions): its sequences' lengths
+// walked on the host, nothing copied; -1 when the block is malformed.
+long long lz4_length(const uint8_t *src, size_t n, size_t room) {
+  size_t i = 0, o = 0;
+  for (;;) {
+    if (i >= n)
+      return -1;
+    const unsigned token = src[i++];
+    size_t lit = token >> 4;
+    if (lit == 15) {
+      unsigned b;
+      do {
+        if (i >= n)
+          return -1;
+        b = src[i++];
+        lit += b;
+      } while (b == 255);
+    }
+    if (o + lit + kMfLimit > room || i + lit + 8 > n) // the last sequence
+      return i + lit == n && o + lit <= room ? (long long)(o + lit) : -1;
+    i += lit;
+    o += lit;
+    const size_t off = (size_t)src[i] | ((size_t)src[i + 1] << 8);
+    i += 2;
+    if (off == 0 || off > o)
+      return -1;
+    size_t len = token & 15;
+    if (len == 15) {
+      unsigned b;
+      do {
+        if (i >= n)
+          return -1;
+        b = src[i++];
+        len += b;
+      } while (b == 255);
+    }
+    len += kMinMatch;
+    if (o + len + kLastLiterals > room)
+      return -1;
+    o += len;
+  }
+}
 
 // bitshuffle's default block: 8192 bytes of elements, a multiple of 8, at
 // least 128 elements.
@@ -594,7 +640,7 @@ __global__ __launch_bounds__(64) void lz4_kernel(const Block *blocks,
   const Block b = blocks[blockIdx.x];
   const long long got =
       wave_lz4<long long>(chunks[b.chunk] + b.src, (long long)b.comp,
-                          frames[b.chunk] + b.dst, (long long)b.bytes,
+                          frames[b.chunk] + b.dst, (long long)b.room,
                           threadIdx.x);
   if (got != (long long)b.bytes && threadIdx.x == 0)
     status[b.chunk] = -1;
@@ -798,7 +844,7 @@ int walk_chunk(sf_decoder *d, const uint8_t *data, size_t size, int f,
         return fail(SF_BAD_DATA, bad + "-91");
       d->kinds[kShuffled].push_back(Block{pos, done * es, comp,
                                           (uint32_t)(count * es), (uint32_t)f,
-                                          kShuffled});
+                                          (uint32_t)(count * es)});
       d->max_comp = std::max(d->max_comp, comp);
       d->max_bytes = std::max(d->max_bytes, (uint32_t)(count * es));
       pos += comp;
@@ -809,10 +855,13 @@ int walk_chunk(sf_decoder *d, const uint8_t *data, size_t size, int f,
       return fail(SF_BAD_DATA, bad + "-91");
     if (leftover)
       d->kinds[kRaw].push_back(Block{pos, done * es, 0, (uint32_t)leftover,
-                                     (uint32_t)f, kRaw});
+                                     (uint32_t)f, (uint32_t)leftover});
     return SF_OK;
   }
-  // "lz4": the HDF5 LZ4 filter's framing (src/decompress.cc:42-70)
+  // "lz4": the HDF5 LZ4 filter's framing (src/decompress.cc:42-70). Each
+  // block is decoded with room for min(block size, what is left) and the
+  // next one goes where its output ended, so a block's length is walked
+  // here (its sequences' lengths, no copies) to place the next one.
   const size_t block_size = block > 0 ? block : expected;
   size_t written = 0;
   while (written < expected) {
@@ -825,10 +874,14 @@ int walk_chunk(sf_decoder *d, const uint8_t *data, size_t size, int f,
     const size_t want = std::min(expected - written, block_size);
     if (want > UINT32_MAX)
       return fail(SF_BAD_ARGUMENT, at + "an lz4 block over 4 GiB");
-    d->kinds[kLz4].push_back(
-        Block{pos, written, comp, (uint32_t)want, (uint32_t)f, kLz4});
+    const long long got = lz4_length(data + pos, comp, want);
+    if (got < 0)
+      return fail(SF_BAD_DATA, at + "lz4 decompression failed with code -1");
+    if (got > 0)
+      d->kinds[kLz4].push_back(Block{pos, written, comp, (uint32_t)got,
+                                     (uint32_t)f, (uint32_t)want});
     pos += comp;
-    written += want;
+    written += (size_t)got;
   }
   return SF_OK;
 }

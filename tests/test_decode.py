@@ -234,6 +234,43 @@ def test_lz4_chunks_round_trip():
     assert cpu.decompress_image(chunk, "lz4", 16, 30, 50)[0] == x.tobytes()
 
 
+def lz4_pieces_chunk(data: bytes, header_block: int, pieces) -> bytes:
+    """An LZ4 filter chunk whose blocks hold the given numbers of bytes of
+    `data` in turn (each at most header_block, some fewer)."""
+    out = bytearray(len(data).to_bytes(8, "big"))
+    out += header_block.to_bytes(4, "big")
+    at = 0
+    for n in pieces:
+        comp = sf.lz4_compress_block(data[at:at + n])
+        out += len(comp).to_bytes(4, "big") + comp
+        at += n
+    assert at == len(data)
+    return bytes(out)
+
+
+SHORT_PIECES = [(1000, [600, 1000, 0, 1000, 400]),  # an empty block too
+                (0, [700, 2300]),                   # 0: the whole frame
+                (3000, [1, 2999])]
+
+
+def test_lz4_chunks_with_short_blocks():
+    """A block may decode to fewer bytes than the header's block size: the
+    next block starts where it ended (src/decompress.cc:44-70 advances by
+    what LZ4_decompress_safe produced), so the frame is the blocks' bytes in
+    turn; a chunk whose blocks run out first is truncated."""
+    rng = np.random.default_rng(5)
+    x = rng.poisson(2.0, (30, 50)).astype(np.uint16)
+    data = x.tobytes()
+    for header_block, pieces in SHORT_PIECES:
+        chunk = lz4_pieces_chunk(data, header_block, pieces)
+        out, msg = cpu.decompress_image(chunk, "lz4", 16, 30, 50)
+        assert msg == "" and out == data, (header_block, pieces)
+    short = lz4_pieces_chunk(data[:2000], 1000, [600, 1000, 400])
+    short = len(data).to_bytes(8, "big") + short[8:]
+    assert cpu.decompress_image(short, "lz4", 16, 30, 50)[1] == \
+        "lz4 data truncated"
+
+
 def test_messages_follow_the_reference():
     """The messages of decompress::image (src/decompress.cc:83-135)."""
     x = np.arange(8, dtype=np.uint16)

@@ -9,7 +9,8 @@ import pytest
 
 from oracle import cpu
 import frames as fr
-from test_decode import LZ4, lz4_chunk, lz4_compress
+from test_decode import (LZ4, SHORT_PIECES, lz4_chunk, lz4_compress,
+                         lz4_pieces_chunk)
 
 pytestmark = pytest.mark.gpu
 
@@ -186,6 +187,23 @@ def test_lz4_and_stored_chunks(sf):
     check(sf, chunks, "lz4", 16, h, w)
     chunks = [x.tobytes() for x in xs]
     check(sf, chunks, "", 16, h, w)
+
+
+def test_lz4_chunks_with_short_blocks(sf):
+    """Blocks that decode to fewer bytes than the header's block size (an
+    empty one among them): each next block goes where the last one ended,
+    as the reference's loop places them; equal to the oracle."""
+    rng = np.random.default_rng(8)
+    xs = [rng.poisson(2.0, (30, 50)).astype(np.uint16) for _ in range(3)]
+    chunks = [lz4_pieces_chunk(x.tobytes(), hb, pieces)
+              for x, (hb, pieces) in zip(xs, SHORT_PIECES)]
+    check(sf, chunks, "lz4", 16, 30, 50)
+    dec = sf.Decoder(16, 30, 50, max_frames=1)
+    short = lz4_pieces_chunk(xs[0].tobytes()[:2000], 1000, [600, 1000, 400])
+    short = (3000).to_bytes(8, "big") + short[8:]
+    with pytest.raises(sf.DecodeError, match="^lz4 data truncated$"):
+        dec.index([short], "lz4")
+    dec.close()
 
 
 def test_errors_follow_the_reference(sf):
