@@ -1660,7 +1660,8 @@ struct sf_labeller {
   // add_plan's chunks: the plan's lists are copied on the plan's stream to
   // one of two snapshot sides (snapshot_lists), and a worker thread labels
   // the chunk once the copy is done, so add_plan returns without waiting
-  // for the plan or the labelling. One chunk is handed over at a time.
+  // for the plan or the labelling. A chunk is handed over while the worker
+  // may still hold the one before it (jobs_given - jobs_done <= 2).
   struct Job {
     std::vector<sf::ChunkDesc> frames;
     int side = 0;
@@ -1669,6 +1670,7 @@ struct sf_labeller {
   std::mutex mu;
   std::condition_variable cv;
   bool job_ready = false, busy = false, stopping = false;
+  uint64_t jobs_given = 0, jobs_done = 0;
   Job job;
   int deferred_status = 0; // the first failure of a handed-over chunk
   std::string deferred_error;
@@ -1696,6 +1698,7 @@ struct sf_labeller {
   uint32_t *h_scalars = nullptr, *hd_scalars = nullptr;
   unsigned long long *h_totals = nullptr, *hd_totals = nullptr;
   hipEvent_t runs_ready = nullptr;
+  hipEvent_t synced = nullptr; // recorded and polled to wait for the stream
   sf::ChunkDesc *h_chunk = nullptr; // pinned staging of the chunk table
   uint32_t h_chunk_cap = 0;
   // per node
@@ -1791,6 +1794,30 @@ hipError_t sort64(void *temp, size_t &bytes, const uint64_t *keys_in,
                                           values_in, values_out, n, 0, 64, s);
 }
 
+// Waits for an event by polling it: a short spin, then a yield between
+// queries. A blocking wait sleeps until the device's interrupt, whose
+// wake-up has taken from tens of microseconds to milliseconds, and the
+// labeller's waits sit between launches that keep the device busy.
+hipError_t poll_event(hipEvent_t e) {
+  for (uint32_t k = 0;; k++) {
+    const hipError_t q = hipEventQuery(e);
+    if (q != hipErrorNotReady)
+      return q;
+    if (k < 64) {
+      for (int i = 0; i < 32; i++)
+        __builtin_ia32_pause();
+    } else {
+      std::this_thread::yield();
+    }
+  }
+}
+
+// Everything enqueued on the labeller's stream so far, done.
+hipError_t poll_stream(sf_labeller *l) {
+  const hipError_t e = hipEventRecord(l->synced, l->stream);
+  return e == hipSuccess ? poll_event(l->synced) : e;
+}
+
 // Label one chunk whose frame lists are device-resident. `frames` holds
 // list/count/capacity/number for each frame; offsets are filled here from
 // the host counts.
@@ -1799,7 +1826,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
               hipStream_t producer) {
   hipStream_t s = l->stream;
   // the frontier and the scratch of the previous chunk must be done with
-  LAB_CHECK(hipStreamSynchronize(s), "waiting for the previous chunk");
+  LAB_CHECK(poll_stream(l), "waiting for the previous chunk");
   l->n_carry = *l->h_scalars;
   // Every pixel of the last labelled frame is in an open component, so the
   // frontier ends with exactly that frame's pixels.
@@ -1981,7 +2008,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
                                             l->last_slot.ptr);
   LAB_CHECK(hipGetLastError(), "label kernels");
   // the keys, the sort and the scans take their sizes from the host
-  LAB_CHECK(hipEventSynchronize(l->runs_ready), "waiting for the run count");
+  LAB_CHECK(poll_event(l->runs_ready), "waiting for the run count");
   const uint32_t runs = l->h_scalars[1];
   // Sort keys: 32 bits when (last slot, root run) fit, else 64.
   int root_bits = 1, slot_bits = 1;
@@ -2140,11 +2167,14 @@ int plan_lists(sf_plan *plan, std::vector<const sf_signal_pixel *> &lists,
 
 namespace {
 
-// Waits until the worker has labelled (enqueued) every handed-over chunk;
-// reports the first failure among them, once, as this thread's error.
-int drain(sf_labeller *l) {
+// Waits until the worker has labelled (enqueued) every handed-over chunk
+// but `ahead`, and the hand-over slot is free; reports the first failure
+// among them, once, as this thread's error.
+int drain(sf_labeller *l, uint64_t ahead = 0) {
   std::unique_lock<std::mutex> lock(l->mu);
-  l->cv.wait(lock, [&] { return !l->job_ready && !l->busy; });
+  l->cv.wait(lock, [&] {
+    return !l->job_ready && l->jobs_done + ahead >= l->jobs_given;
+  });
   const int status = l->deferred_status;
   if (status != SF_OK) {
     g_label_error = l->deferred_error;
@@ -2158,8 +2188,7 @@ int drain(sf_labeller *l) {
 // free for the snapshot after next once the labelling has read it.
 int label_job(sf_labeller *l, sf_labeller::Job &job) {
   const int side = job.side;
-  LAB_CHECK(hipEventSynchronize(l->snap_ready[side]),
-            "waiting for the plan's lists");
+  LAB_CHECK(poll_event(l->snap_ready[side]), "waiting for the plan's lists");
   std::vector<uint32_t> counts(job.frames.size());
   for (size_t k = 0; k < counts.size(); k++)
     counts[k] = l->h_snap_counts[side][k];
@@ -2186,6 +2215,7 @@ void worker_loop(sf_labeller *l) {
       l->deferred_error = g_label_error;
     }
     l->busy = false;
+    l->jobs_done++;
     l->cv.notify_all();
   }
 }
@@ -2209,7 +2239,7 @@ int flush_pending(sf_labeller *l) {
     return drained;
   if (l->pending_frames.empty())
     return SF_OK;
-  LAB_CHECK(hipStreamSynchronize(l->stream), "waiting for the labeller");
+  LAB_CHECK(poll_stream(l), "waiting for the labeller");
   const size_t total = l->pending.size();
   LAB_CHECK(l->pending_dev.ensure(std::max<size_t>(total, 1)), "alloc");
   LAB_CHECK(l->pending_count_dev.ensure(l->pending_counts.size()), "alloc");
@@ -2330,6 +2360,8 @@ sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
                  hipSuccess;
   ok = ok && hipEventCreateWithFlags(&l->runs_ready, hipEventDisableTiming) ==
                  hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&l->synced, hipEventDisableTiming) ==
+                 hipSuccess;
   ok = ok && hipGetDevice(&l->device) == hipSuccess;
   for (int i = 0; i < 2; i++) {
     ok = ok && hipEventCreateWithFlags(&l->snap_ready[i],
@@ -2434,6 +2466,8 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   l->pending_count_dev.release();
   if (l->runs_ready)
     (void)hipEventDestroy(l->runs_ready);
+  if (l->synced)
+    (void)hipEventDestroy(l->synced);
   if (l->h_totals)
     (void)hipHostFree(l->h_totals);
   if (l->h_chunk)
@@ -2491,7 +2525,10 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
     g_label_error = "sf_hip_labeller_add_plan: bad argument";
     return SF_BAD_ARGUMENT;
   }
-  int status = flush_pending(l); // also waits for the last handed-over chunk
+  // Frames added from the host go first, after every handed-over chunk.
+  // Otherwise the worker may keep the last chunk while this one is handed
+  // over: its snapshot side's last reader, the chunk before that, is done.
+  int status = l->pending_frames.empty() ? drain(l, 1) : flush_pending(l);
   if (status != SF_OK)
     return status;
   if (l->started && first_frame <= l->last_frame) {
@@ -2575,6 +2612,7 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
     l->job.frames = std::move(frames);
     l->job.side = side;
     l->job_ready = true;
+    l->jobs_given++;
     if (!l->worker.joinable())
       l->worker = std::thread(worker_loop, l);
   }
@@ -2612,7 +2650,7 @@ int sf_hip_labeller_finish(sf_labeller *l) {
   status = run_chunk(l, none, no_counts, true, nullptr);
   if (status != SF_OK)
     return status;
-  LAB_CHECK(hipStreamSynchronize(l->stream), "hipStreamSynchronize");
+  LAB_CHECK(poll_stream(l), "waiting for the labeller");
   l->n_carry = *l->h_scalars;
   return SF_OK;
 }
@@ -2623,7 +2661,7 @@ int sf_hip_labeller_wait(sf_labeller *l) {
   const int drained = drain(l);
   if (drained != SF_OK)
     return drained;
-  LAB_CHECK(hipStreamSynchronize(l->stream), "hipStreamSynchronize");
+  LAB_CHECK(poll_stream(l), "waiting for the labeller");
   l->n_carry = *l->h_scalars;
   return SF_OK;
 }
