@@ -344,7 +344,8 @@ int sf_hip_server_open(sf_server *server, int pixel_bytes, size_t height,
 int sf_hip_server_open_host(sf_server *server, int pixel_bytes,
                             size_t height, size_t width,
                             const sf_options *options);
-/* Hands over one frame (device memory, valid until the frame completes);
+/* Hands over one frame (device memory, or pinned host memory for a series
+ * opened with sf_hip_server_open_host; valid until the frame completes);
  * frame numbers ascend per series. Returns at once. _at stamps the arrival
  * at a time of sf_hip_server_clock() (seconds) instead of now. */
 int sf_hip_server_submit(sf_server *server, int series, int64_t frame,
