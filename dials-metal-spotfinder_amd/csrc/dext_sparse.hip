@@ -564,18 +564,22 @@ __device__ __forceinline__ void poisson_load(const FrameDesc &fd,
   load_column(fd, rsrc, valid, cell, c, in.v);
 }
 
-// The Poisson test at a cell's candidates (src/dext.cc:146-215): 11x11 sums
-// of pixels neither masked nor eroded. Returns the group's survivors; for
-// the lane's pixel on each of the cell's rows: the mean, and count | value
-// << 16.
-__device__ __forceinline__ uint32_t poisson_test(const FrameDesc &fd,
-                                                 bool valid, uint64_t entry,
-                                                 const PoissonIn &in,
-                                                 float (&mean)[5],
-                                                 uint32_t (&pop)[5]) {
-  const uint32_t lane = lane_id();
-  const int c = (int)(lane & 15u);
-  const uint32_t cell = (uint32_t)entry, c25 = (uint32_t)(entry >> 32);
+struct Compact;
+__device__ __forceinline__ void put_candidate(Compact &cp, uint32_t mine,
+                                              uint32_t first, int t,
+                                              uint32_t ns, uint32_t p);
+
+// The 11x11 sums of a cell's pixels neither masked nor eroded
+// (src/dext.cc:146-189), for the lane's column c of the cell (lanes c = 0..4
+// of the group), row by row of the cell: each candidate's packed count and
+// sum (count << 23 | sum) and its value go to the step's compacted list as
+// they are formed. `mine`: the cell's candidates; `first`: the list entry of
+// its first.
+__device__ __forceinline__ void poisson_sums(const FrameDesc &fd, bool valid,
+                                             uint32_t cell, const PoissonIn &in,
+                                             uint32_t mine, uint32_t first,
+                                             Compact &cp) {
+  const int c = (int)(lane_id() & 15u);
   const int i = (int)(cell >> 16), j = (int)(cell & 0xFFFFu);
   const int x = 5 * j - 5 + c, ncr = (int)fd.n_cell_rows;
   const bool col = valid && c < 15 && x >= 0 && x < (int)fd.width;
@@ -600,24 +604,29 @@ __device__ __forceinline__ uint32_t poisson_test(const FrameDesc &fd,
         (in.v[r] < 0xFFFEu && !out) ? (in.v[r] | (1u << 23)) : 0u;
     pn[r + 1] = pn[r] + xr;
   }
-  // tested on lane c (0..4) for the cell's column c, i.e. column c+5
-  const uint32_t at = 16u * (lane >> 4);
-  uint32_t s25 = 0;
 #pragma unroll
-  for (int t = 0; t < 5; t++) {
-    const uint32_t ns = window11(pn[t + 11] - pn[t]);
-    const uint32_t n = ns >> 23, sum = ns & 0x7FFFFFu;
-    const uint32_t p = right<5>(in.v[t + 5]);
-    // src/dext.cc:203-206, in the reference's order and rounding
-    const float m = n >= 2 ? (float)sum / (float)n : 0.0f;
-    const bool ok = (float)p >= m + 3.0f * sqrtf(m);
-    const bool cand = c <= 4 && ((c25 >> (5 * t + c)) & 1u);
-    const uint64_t b = ballot(valid && cand && ok);
-    s25 |= (uint32_t)((b >> at) & 31u) << (5 * t);
-    mean[t] = m;
-    pop[t] = n | (p << 16);
-  }
-  return s25;
+  for (int t = 0; t < 5; t++)
+    put_candidate(cp, mine, first, t, window11(pn[t + 11] - pn[t]),
+                  right<5>(in.v[t + 5]));
+}
+
+// The Poisson test (src/dext.cc:201-206) in the reference's order and
+// rounding, from the 11x11 window's count and sum and the candidate's value
+// p; the mean goes out through `mean`.
+__device__ __forceinline__ bool poisson_signal(uint32_t ns, uint32_t p,
+                                               float &mean) {
+  const uint32_t n = ns >> 23, sum = ns & 0x7FFFFFu;
+  const float m = n >= 2 ? (float)sum / (float)n : 0.0f;
+  mean = m;
+  return (float)p >= m + 3.0f * sqrtf(m);
+}
+
+// Orders one wave's LDS stores before its later LDS loads (lanes read what
+// other lanes wrote).
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // A wave's survivors wait here until a whole run of them takes its place in
@@ -632,6 +641,30 @@ struct Stash {
   uint32_t s25[kStashCells];
   uint32_t at[kStashCells]; // the cell's run in the stash
 };
+
+// A wave-step's candidates, compacted in (cell, row, column) order: at most
+// four cells of 25 pixels.
+constexpr int kCompact = 100;
+struct Compact {
+  uint32_t ns[kCompact];  // window count << 23 | sum
+  uint32_t pix[kCompact]; // value | (5 row + column) << 16 | group << 24
+  uint32_t s25[4];        // each group's survivors
+};
+
+// Lane c (0..4) of a group: its pixel on the cell's row t, if a candidate,
+// to its place in the step's compacted list.
+__device__ __forceinline__ void put_candidate(Compact &cp, uint32_t mine,
+                                              uint32_t first, int t,
+                                              uint32_t ns, uint32_t p) {
+  const uint32_t lane = lane_id(), c = lane & 15u;
+  const uint32_t bit = 5u * (uint32_t)t + c;
+  if (c <= 4 && ((mine >> bit) & 1u)) {
+    const uint32_t at =
+        first + (uint32_t)__builtin_popcount(mine & ((1u << bit) - 1u));
+    cp.ns[at] = ns;
+    cp.pix[at] = p | (bit << 16) | ((lane >> 4) << 24);
+  }
+}
 
 __device__ __forceinline__ void flush(const FrameDesc &fd, Stash &st,
                                       uint32_t &n, uint32_t &cells) {
@@ -672,18 +705,21 @@ __device__ __forceinline__ void flush(const FrameDesc &fd, Stash &st,
 }
 
 // One item's candidate cells: each 64 of its active cells are compacted to
-// their candidate cells (in LDS) and tested four at a time, the next four
-// cells' inputs fetched while the current four are tested; survivors gather
-// in the stash.
+// their candidate cells (in LDS) and summed four at a time, the next four
+// cells' inputs fetched while the current four are summed. The step's
+// candidate pixels are then compacted in (cell, row, column) order, a lane
+// each, and tested there, so that the division and square root run once a
+// candidate; survivors gather in the stash in that order, which is each
+// cell's pixel order.
 __device__ __forceinline__ void poisson_item(const FrameDesc &fd,
                                              uint32_t local, uint64_t *slots,
-                                             Stash &st) {
+                                             Stash &st, Compact &cp) {
   const uint32_t n = fd.icount[local];
   const uint32_t lane = lane_id(), grp = lane >> 4;
-  const int c = (int)(lane & 15u);
   const uint32_t *cells = fd.alist + (size_t)local * kSegment;
   const uint32_t *cand = fd.ccand + (size_t)local * kSegment;
   const __amdgpu_buffer_rsrc_t rsrc = frame_buffer(fd);
+  const uint32_t width = fd.width;
   uint32_t stashed = 0, stashed_cells = 0;
   for (uint32_t c0 = 0; c0 < n; c0 += 64) {
     const bool here = c0 + lane < n;
@@ -696,6 +732,7 @@ __device__ __forceinline__ void poisson_item(const FrameDesc &fd,
     const uint32_t steps = (nc + 3) / 4;
     if (!steps)
       continue;
+    wave_sync_lds();
     bool valid = grp < nc;
     uint64_t entry = valid ? slots[grp] : 0ull;
     PoissonIn in;
@@ -707,49 +744,71 @@ __device__ __forceinline__ void poisson_item(const FrameDesc &fd,
       PoissonIn ahead;
       if (g + 1 < steps)
         poisson_load(fd, rsrc, next_valid, (uint32_t)next_entry, ahead);
-      float mean[5];
-      uint32_t pop[5];
-      const uint32_t s25 = poisson_test(fd, valid, entry, in, mean, pop);
-      // this step's survivors: counts of the four cells, in group order
-      const uint32_t k = (uint32_t)__builtin_popcount(s25);
-      const uint32_t k0 = (uint32_t)__shfl((int)k, 0, 64);
-      const uint32_t k1 = (uint32_t)__shfl((int)k, 16, 64);
-      const uint32_t k2 = (uint32_t)__shfl((int)k, 32, 64);
-      const uint32_t k3 = (uint32_t)__shfl((int)k, 48, 64);
+      // the step's candidates: counts of the four cells, in group order
+      const uint32_t mine = (uint32_t)(entry >> 32); // 0 on invalid groups
+      const uint32_t k = (uint32_t)__builtin_popcount(mine);
+      const uint32_t k0 = (uint32_t)__builtin_amdgcn_readlane((int)k, 0);
+      const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)k, 16);
+      const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)k, 32);
+      const uint32_t k3 = (uint32_t)__builtin_amdgcn_readlane((int)k, 48);
       const uint32_t total = k0 + k1 + k2 + k3;
-      if (total) {
-        if (stashed + total > (uint32_t)kStash ||
-            stashed_cells + 4 > (uint32_t)kStashCells)
-          flush(fd, st, stashed, stashed_cells);
-        const uint32_t before =
-            (grp > 0 ? k0 : 0u) + (grp > 1 ? k1 : 0u) + (grp > 2 ? k2 : 0u);
-        const uint32_t run = stashed + before;
-        // the cell's record, by its group leader
-        const uint64_t lead = ballot(c == 0 && k != 0);
-        if (c == 0 && k) {
-          const uint32_t slot = stashed_cells + count_below(lead);
-          st.cell[slot] = (uint32_t)entry;
-          st.s25[slot] = s25;
-          st.at[slot] = run;
+      if (stashed + total > (uint32_t)kStash ||
+          stashed_cells + 4 > (uint32_t)kStashCells)
+        flush(fd, st, stashed, stashed_cells);
+      const uint32_t first =
+          (grp > 0 ? k0 : 0u) + (grp > 1 ? k1 : 0u) + (grp > 2 ? k2 : 0u);
+      if (lane < 4)
+        cp.s25[lane] = 0;
+      poisson_sums(fd, valid, (uint32_t)entry, in, mine, first, cp);
+      wave_sync_lds();
+      // a lane a candidate; survivors to the stash in candidate order
+      uint32_t kept = 0;
+      for (uint32_t b = 0; b < total; b += 64) {
+        const uint32_t at = b + lane;
+        bool ok = false;
+        float mean = 0.0f;
+        uint32_t ns_at = 0, pix = 0;
+        if (at < total) {
+          ns_at = cp.ns[at];
+          pix = cp.pix[at];
+          ok = poisson_signal(ns_at, pix & 0xFFFFu, mean);
         }
-        if (c <= 4) {
-          const uint32_t cell_i = (uint32_t)entry >> 16;
-          const uint32_t cell_j = (uint32_t)entry & 0xFFFFu;
+        const uint64_t okb = ballot(ok);
+        if (ok) {
+          const uint32_t e = stashed + kept + count_below(okb);
+          const uint32_t gi = pix >> 24, bit = (pix >> 16) & 0xFFu;
+          const uint32_t ce = (uint32_t)slots[4 * g + gi];
+          const uint32_t t = bit / 5u, cc = bit - 5u * t;
+          st.mean[e] = mean;
+          st.pop[e] = (ns_at >> 23) | ((pix & 0xFFFFu) << 16);
+          st.index[e] =
+              (5u * (ce >> 16) + t) * width + 5u * (ce & 0xFFFFu) + cc;
+          atomicOr(&cp.s25[gi], 1u << bit);
+        }
+        kept += (uint32_t)__builtin_popcountll(okb);
+      }
+      if (kept) {
+        wave_sync_lds();
+        // each cell with survivors: its record, by the lane of its group
+        const uint32_t s25 = lane < 4 ? cp.s25[lane] : 0u;
+        const uint32_t ks = (uint32_t)__builtin_popcount(s25);
+        const uint64_t lead = ballot(s25 != 0);
+        uint32_t run = 0; // survivors of the groups before this lane's
 #pragma unroll
-          for (int t = 0; t < 5; t++) {
-            const int bit = 5 * t + c;
-            if (!((s25 >> bit) & 1u))
-              continue;
-            const uint32_t e =
-                run + (uint32_t)__builtin_popcount(s25 & ((1u << bit) - 1u));
-            st.mean[e] = mean[t];
-            st.pop[e] = pop[t];
-            st.index[e] = (5 * cell_i + t) * fd.width + 5 * cell_j + (uint32_t)c;
-          }
+        for (int q = 0; q < 3; q++) {
+          const uint32_t kq = (uint32_t)__builtin_amdgcn_readlane((int)ks, q);
+          run += lane > (uint32_t)q ? kq : 0u;
         }
-        stashed += total;
+        if (s25) {
+          const uint32_t slot = stashed_cells + count_below(lead);
+          st.cell[slot] = (uint32_t)slots[4 * g + lane];
+          st.s25[slot] = s25;
+          st.at[slot] = stashed + run;
+        }
+        stashed += kept;
         stashed_cells += (uint32_t)__builtin_popcountll(lead);
       }
+      wave_sync_lds();
       in = ahead;
       entry = next_entry;
       valid = next_valid;
@@ -763,11 +822,12 @@ __global__ __launch_bounds__(256) void poisson_kernel(const FrameDesc *frames,
                                                       uint32_t items) {
   __shared__ uint64_t slots[4][64];
   __shared__ Stash stash[4];
+  __shared__ Compact compact[4];
   FrameDesc fd;
   uint32_t local;
   const uint32_t wave = threadIdx.x >> 6;
   if (block_item(frames, n_frames, items, fd, local))
-    poisson_item(fd, local, slots[wave], stash[wave]);
+    poisson_item(fd, local, slots[wave], stash[wave], compact[wave]);
 }
 
 // ---------------------------------------------------------------------------
@@ -808,6 +868,7 @@ __global__ __launch_bounds__(256, 4) void scout_poisson_kernel(Side scout,
   __shared__ uint64_t masks[4][2 * kBandCells];
   __shared__ uint64_t slots[4][64];
   __shared__ Stash stash[4];
+  __shared__ Compact compact[4];
   fill_lut(lut);
   __syncthreads();
   const uint32_t wave = threadIdx.x >> 6;
@@ -823,7 +884,7 @@ __global__ __launch_bounds__(256, 4) void scout_poisson_kernel(Side scout,
       scout_wave(fd, local, lut, masks[wave]);
   } else if (wave_item(sparse.frames, sparse.n_frames, sparse.first,
                        sparse.end, index, fd, local)) {
-    poisson_item(fd, local, slots[wave], stash[wave]);
+    poisson_item(fd, local, slots[wave], stash[wave], compact[wave]);
   }
 }
 

@@ -571,19 +571,22 @@ __device__ __forceinline__ void poisson_load(const FrameDesc &fd,
   load_column(fd, rsrc, valid, cell, c, in.v);
 }
 
-// The Poisson test at a cell's candidates (src/dext.cc:146-215): 11x11 sums
-// of the pixels neither masked nor eroded, the test in double. Returns the
-// group's survivors; for the lane's pixel on each of the cell's rows: the
-// mean (as the list stores it), the count and the value.
-__device__ __forceinline__ uint32_t poisson_test(const FrameDesc &fd,
-                                                 bool valid, uint64_t entry,
-                                                 const PoissonIn &in,
-                                                 float (&mean)[5],
-                                                 uint32_t (&count)[5],
-                                                 uint32_t (&value)[5]) {
-  const uint32_t lane = lane_id();
-  const int c = (int)(lane & 15u);
-  const uint32_t cell = (uint32_t)entry, c25 = (uint32_t)(entry >> 32);
+struct Compact;
+__device__ __forceinline__ void put_candidate(Compact &cp, uint32_t mine,
+                                              uint32_t first, int t,
+                                              uint32_t n, uint64_t sum,
+                                              uint32_t p);
+
+// The 11x11 sums of a cell's pixels neither masked nor eroded
+// (src/dext.cc:146-189), for the lane's column c of the cell (lanes c = 0..4
+// of the group), row by row of the cell: each candidate's count, sum and
+// value go to the step's compacted list as they are formed. `mine`: the
+// cell's candidates; `first`: the list entry of its first.
+__device__ __forceinline__ void poisson_sums(const FrameDesc &fd, bool valid,
+                                             uint32_t cell, const PoissonIn &in,
+                                             uint32_t mine, uint32_t first,
+                                             Compact &cp) {
+  const int c = (int)(lane_id() & 15u);
   const int i = (int)(cell >> 16), j = (int)(cell & 0xFFFFu);
   const int x = 5 * j - 5 + c, ncr = (int)fd.n_cell_rows;
   const bool col = valid && c < 15 && x >= 0 && x < (int)fd.width;
@@ -614,9 +617,6 @@ __device__ __forceinline__ uint32_t poisson_test(const FrameDesc &fd,
     cn += ok ? 1u : 0u;
     cs += ok ? in.v[r] : 0u;
   }
-  // tested on lane c (0..4) for the cell's column c, i.e. column c+5
-  const uint32_t at = 16u * (lane >> 4);
-  uint32_t s25 = 0;
 #pragma unroll
   for (int t = 0; t < 5; t++) {
     {
@@ -632,22 +632,31 @@ __device__ __forceinline__ uint32_t poisson_test(const FrameDesc &fd,
     const uint32_t n = window11(cn);
     const uint64_t sum = window11(cs);
     const uint32_t p = right<5>(in.v[t + 5]);
-    // src/dext.cc:203-206, in double, in the reference's order
-    const double m = n >= 2 ? (double)sum / (double)n : 0.0;
-    const bool ok = (double)p >= m + 3.0 * sqrt(m);
-    const bool cand = c <= 4 && ((c25 >> (5 * t + c)) & 1u);
-    const uint64_t b = ballot(valid && cand && ok);
-    s25 |= (uint32_t)((b >> at) & 31u) << (5 * t);
-    mean[t] = (float)m;
-    count[t] = n;
-    value[t] = p;
+    put_candidate(cp, mine, first, t, n, sum, p);
   }
-  return s25;
+}
+
+// The Poisson test (src/dext.cc:201-206) in double, in the reference's order:
+// the 11x11 window's count n and sum, the candidate's value p; the mean goes
+// out through `mean`.
+__device__ __forceinline__ bool poisson_signal(uint32_t n, uint64_t sum,
+                                               uint32_t p, double &mean) {
+  const double m = n >= 2 ? (double)sum / (double)n : 0.0;
+  mean = m;
+  return (double)p >= m + 3.0 * sqrt(m);
+}
+
+// Orders one wave's LDS stores before its later LDS loads (lanes read what
+// other lanes wrote).
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // A wave's survivors wait here until a whole run of them takes its place in
 // staging under one atomic: each cell's run in its pixel order.
-constexpr int kStash = 512;
+constexpr int kStash = 384;
 constexpr int kStashCells = 64;
 struct Stash {
   float mean[kStash];
@@ -658,6 +667,33 @@ struct Stash {
   uint32_t s25[kStashCells];
   uint32_t at[kStashCells]; // the cell's run in the stash
 };
+
+// A wave-step's candidates, compacted in (cell, row, column) order: at most
+// four cells of 25 pixels.
+constexpr int kCompact = 100;
+struct Compact {
+  uint64_t sum[kCompact];  // window sum
+  uint32_t value[kCompact];
+  uint32_t info[kCompact]; // count | (5 row + column) << 8 | group << 16
+  uint32_t s25[4];         // each group's survivors
+};
+
+// Lane c (0..4) of a group: its pixel on the cell's row t, if a candidate,
+// to its place in the step's compacted list.
+__device__ __forceinline__ void put_candidate(Compact &cp, uint32_t mine,
+                                              uint32_t first, int t,
+                                              uint32_t n, uint64_t sum,
+                                              uint32_t p) {
+  const uint32_t lane = lane_id(), c = lane & 15u;
+  const uint32_t bit = 5u * (uint32_t)t + c;
+  if (c <= 4 && ((mine >> bit) & 1u)) {
+    const uint32_t at =
+        first + (uint32_t)__builtin_popcount(mine & ((1u << bit) - 1u));
+    cp.sum[at] = sum;
+    cp.value[at] = p;
+    cp.info[at] = n | (bit << 8) | ((lane >> 4) << 16);
+  }
+}
 
 __device__ __forceinline__ void flush(const FrameDesc &fd, Stash &st,
                                       uint32_t &n, uint32_t &cells) {
@@ -698,18 +734,21 @@ __device__ __forceinline__ void flush(const FrameDesc &fd, Stash &st,
 }
 
 // One item's candidate cells: each 64 of its active cells are compacted to
-// their candidate cells (in LDS) and tested four at a time, the next four
-// cells' inputs fetched while the current four are tested; survivors gather
-// in the stash.
+// their candidate cells (in LDS) and summed four at a time, the next four
+// cells' inputs fetched while the current four are summed. The step's
+// candidate pixels are then compacted in (cell, row, column) order, a lane
+// each, and tested there, so that the double division and square root run
+// once a candidate; survivors gather in the stash in that order, which is
+// each cell's pixel order.
 __device__ __forceinline__ void poisson_item(const FrameDesc &fd,
                                              uint32_t local, uint64_t *slots,
-                                             Stash &st) {
+                                             Stash &st, Compact &cp) {
   const uint32_t n = fd.icount[local];
   const uint32_t lane = lane_id(), grp = lane >> 4;
-  const int c = (int)(lane & 15u);
   const uint32_t *cells = fd.alist + (size_t)local * kSegment;
   const uint32_t *cand = fd.ccand + (size_t)local * kSegment;
   const __amdgpu_buffer_rsrc_t rsrc = frame_buffer(fd);
+  const uint32_t width = fd.width;
   uint32_t stashed = 0, stashed_cells = 0;
   for (uint32_t c0 = 0; c0 < n; c0 += 64) {
     const bool here = c0 + lane < n;
@@ -718,9 +757,7 @@ __device__ __forceinline__ void poisson_item(const FrameDesc &fd,
     const uint64_t has = ballot(c25 != 0);
     if (c25)
       slots[count_below(has)] = (uint64_t)cell | ((uint64_t)c25 << 32);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync_lds();
     const uint32_t nc = (uint32_t)__builtin_popcountll(has);
     const uint32_t steps = (nc + 3) / 4;
     if (!steps)
@@ -736,55 +773,72 @@ __device__ __forceinline__ void poisson_item(const FrameDesc &fd,
       PoissonIn ahead;
       if (g + 1 < steps)
         poisson_load(fd, rsrc, next_valid, (uint32_t)next_entry, ahead);
-      float mean[5];
-      uint32_t count[5], value[5];
-      const uint32_t s25 =
-          poisson_test(fd, valid, entry, in, mean, count, value);
-      // this step's survivors: counts of the four cells, in group order
-      const uint32_t k = (uint32_t)__builtin_popcount(s25);
-      const uint32_t k0 = (uint32_t)__shfl((int)k, 0, 64);
-      const uint32_t k1 = (uint32_t)__shfl((int)k, 16, 64);
-      const uint32_t k2 = (uint32_t)__shfl((int)k, 32, 64);
-      const uint32_t k3 = (uint32_t)__shfl((int)k, 48, 64);
+      // the step's candidates: counts of the four cells, in group order
+      const uint32_t mine = (uint32_t)(entry >> 32); // 0 on invalid groups
+      const uint32_t k = (uint32_t)__builtin_popcount(mine);
+      const uint32_t k0 = (uint32_t)__builtin_amdgcn_readlane((int)k, 0);
+      const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)k, 16);
+      const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)k, 32);
+      const uint32_t k3 = (uint32_t)__builtin_amdgcn_readlane((int)k, 48);
       const uint32_t total = k0 + k1 + k2 + k3;
-      if (total) {
-        if (stashed + total > (uint32_t)kStash ||
-            stashed_cells + 4 > (uint32_t)kStashCells)
-          flush(fd, st, stashed, stashed_cells);
-        const uint32_t before =
-            (grp > 0 ? k0 : 0u) + (grp > 1 ? k1 : 0u) + (grp > 2 ? k2 : 0u);
-        const uint32_t run = stashed + before;
-        // the cell's record, by its group leader
-        const uint64_t lead = ballot(c == 0 && k != 0);
-        if (c == 0 && k) {
-          const uint32_t slot = stashed_cells + count_below(lead);
-          st.cell[slot] = (uint32_t)entry;
-          st.s25[slot] = s25;
-          st.at[slot] = run;
+      if (stashed + total > (uint32_t)kStash ||
+          stashed_cells + 4 > (uint32_t)kStashCells)
+        flush(fd, st, stashed, stashed_cells);
+      const uint32_t first =
+          (grp > 0 ? k0 : 0u) + (grp > 1 ? k1 : 0u) + (grp > 2 ? k2 : 0u);
+      if (lane < 4)
+        cp.s25[lane] = 0;
+      poisson_sums(fd, valid, (uint32_t)entry, in, mine, first, cp);
+      wave_sync_lds();
+      // a lane a candidate; survivors to the stash in candidate order
+      uint32_t kept = 0;
+      for (uint32_t b = 0; b < total; b += 64) {
+        const uint32_t at = b + lane;
+        bool ok = false;
+        double mean = 0.0;
+        uint32_t info = 0, value = 0;
+        if (at < total) {
+          info = cp.info[at];
+          value = cp.value[at];
+          ok = poisson_signal(info & 0xFFu, cp.sum[at], value, mean);
         }
-        if (c <= 4) {
-          const uint32_t cell_i = (uint32_t)entry >> 16;
-          const uint32_t cell_j = (uint32_t)entry & 0xFFFFu;
-#pragma unroll
-          for (int t = 0; t < 5; t++) {
-            const int bit = 5 * t + c;
-            if (!((s25 >> bit) & 1u))
-              continue;
-            const uint32_t e =
-                run + (uint32_t)__builtin_popcount(s25 & ((1u << bit) - 1u));
-            st.mean[e] = mean[t];
-            st.count[e] = count[t];
-            st.value[e] = value[t];
-            st.index[e] =
-                (5 * cell_i + t) * fd.width + 5 * cell_j + (uint32_t)c;
-          }
+        const uint64_t okb = ballot(ok);
+        if (ok) {
+          const uint32_t e = stashed + kept + count_below(okb);
+          const uint32_t gi = info >> 16, bit = (info >> 8) & 0xFFu;
+          const uint32_t ce = (uint32_t)slots[4 * g + gi];
+          const uint32_t t = bit / 5u, cc = bit - 5u * t;
+          st.mean[e] = (float)mean;
+          st.count[e] = info & 0xFFu;
+          st.value[e] = value;
+          st.index[e] =
+              (5u * (ce >> 16) + t) * width + 5u * (ce & 0xFFFFu) + cc;
+          atomicOr(&cp.s25[gi], 1u << bit);
         }
-        stashed += total;
-        stashed_cells += (uint32_t)__builtin_popcountll(lead);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        kept += (uint32_t)__builtin_popcountll(okb);
       }
+      if (kept) {
+        wave_sync_lds();
+        // each cell with survivors: its record, by the lane of its group
+        const uint32_t s25 = lane < 4 ? cp.s25[lane] : 0u;
+        const uint32_t ks = (uint32_t)__builtin_popcount(s25);
+        const uint64_t lead = ballot(s25 != 0);
+        uint32_t run = 0; // survivors of the groups before this lane's
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const uint32_t kq = (uint32_t)__builtin_amdgcn_readlane((int)ks, q);
+          run += lane > (uint32_t)q ? kq : 0u;
+        }
+        if (s25) {
+          const uint32_t slot = stashed_cells + count_below(lead);
+          st.cell[slot] = (uint32_t)slots[4 * g + lane];
+          st.s25[slot] = s25;
+          st.at[slot] = stashed + run;
+        }
+        stashed += kept;
+        stashed_cells += (uint32_t)__builtin_popcountll(lead);
+      }
+      wave_sync_lds();
       in = ahead;
       entry = next_entry;
       valid = next_valid;
@@ -798,11 +852,12 @@ __global__ __launch_bounds__(256) void poisson32_kernel(const FrameDesc *frames,
                                                         uint32_t items) {
   __shared__ uint64_t slots[4][64];
   __shared__ Stash stash[4];
+  __shared__ Compact compact[4];
   FrameDesc fd;
   uint32_t local;
   const uint32_t wave = threadIdx.x >> 6;
   if (block_item(frames, n_frames, items, fd, local))
-    poisson_item(fd, local, slots[wave], stash[wave]);
+    poisson_item(fd, local, slots[wave], stash[wave], compact[wave]);
 }
 
 // Pipelined launches, as the u16 form's (dext_sparse.hip): a chunk's scout
@@ -837,6 +892,7 @@ __global__ __launch_bounds__(256, 4) void scout_poisson32_kernel(Side scout,
   __shared__ uint64_t masks[4][2 * kBandCells];
   __shared__ uint64_t slots[4][64];
   __shared__ Stash stash[4];
+  __shared__ Compact compact[4];
   fill_lut(lut);
   __syncthreads();
   const uint32_t wave = threadIdx.x >> 6;
@@ -852,7 +908,7 @@ __global__ __launch_bounds__(256, 4) void scout_poisson32_kernel(Side scout,
       scout_wave(fd, local, lut, masks[wave]);
   } else if (wave_item(sparse.frames, sparse.n_frames, sparse.first,
                        sparse.end, index, fd, local)) {
-    poisson_item(fd, local, slots[wave], stash[wave]);
+    poisson_item(fd, local, slots[wave], stash[wave], compact[wave]);
   }
 }
 
