@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/decode_hip.h"
@@ -757,6 +758,15 @@ void put_be(uint8_t *p, uint64_t v, int bytes) {
 
 // ------------------------------------------------------------- decoder
 
+// One chunk's walk: its blocks by kind, its largest block, and its status
+// and message when it does not walk.
+struct ChunkWalk {
+  std::vector<Block> kinds[3];
+  uint32_t max_comp = 0, max_bytes = 0;
+  int status = SF_OK;
+  std::string message;
+};
+
 struct sf_decoder {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -767,6 +777,7 @@ struct sf_decoder {
   bool stored = false;
   std::string algorithm;
   std::vector<Block> kinds[3];
+  std::vector<ChunkWalk> walks; // per chunk of the batch, reused
   uint32_t first[3] = {0, 0, 0}, count[3] = {0, 0, 0};
   uint32_t max_comp = 0, max_bytes = 0;
   Block *h_blocks = nullptr, *d_blocks = nullptr;
@@ -794,9 +805,10 @@ std::string prefix(int n, int f) {
 }
 
 // The checks of decompress::image (src/decompress.cc:83-135) on one chunk,
-// and its blocks appended to the tables.
-int walk_chunk(sf_decoder *d, const uint8_t *data, size_t size, int f,
-               int n) {
+// and its blocks into `w`. Reads the decoder's settings only, so chunks can
+// be walked on several threads at once.
+int walk_chunk(const sf_decoder *d, const uint8_t *data, size_t size, int f,
+               int n, ChunkWalk &w) {
   const std::string at = prefix(n, f);
   const size_t expected = d->expected, es = d->element;
   if (d->stored) {
@@ -842,11 +854,11 @@ int walk_chunk(sf_decoder *d, const uint8_t *data, size_t size, int f,
       pos += 4;
       if (comp > size - pos || comp > lz4_bound(count * es))
         return fail(SF_BAD_DATA, bad + "-91");
-      d->kinds[kShuffled].push_back(Block{pos, done * es, comp,
+      w.kinds[kShuffled].push_back(Block{pos, done * es, comp,
                                           (uint32_t)(count * es), (uint32_t)f,
                                           (uint32_t)(count * es)});
-      d->max_comp = std::max(d->max_comp, comp);
-      d->max_bytes = std::max(d->max_bytes, (uint32_t)(count * es));
+      w.max_comp = std::max(w.max_comp, comp);
+      w.max_bytes = std::max(w.max_bytes, (uint32_t)(count * es));
       pos += comp;
       done += count;
     }
@@ -854,7 +866,7 @@ int walk_chunk(sf_decoder *d, const uint8_t *data, size_t size, int f,
     if (leftover > size - pos)
       return fail(SF_BAD_DATA, bad + "-91");
     if (leftover)
-      d->kinds[kRaw].push_back(Block{pos, done * es, 0, (uint32_t)leftover,
+      w.kinds[kRaw].push_back(Block{pos, done * es, 0, (uint32_t)leftover,
                                      (uint32_t)f, (uint32_t)leftover});
     return SF_OK;
   }
@@ -878,10 +890,64 @@ int walk_chunk(sf_decoder *d, const uint8_t *data, size_t size, int f,
     if (got < 0)
       return fail(SF_BAD_DATA, at + "lz4 decompression failed with code -1");
     if (got > 0)
-      d->kinds[kLz4].push_back(Block{pos, written, comp, (uint32_t)got,
+      w.kinds[kLz4].push_back(Block{pos, written, comp, (uint32_t)got,
                                      (uint32_t)f, (uint32_t)want});
     pos += comp;
     written += (size_t)got;
+  }
+  return SF_OK;
+}
+
+// Walks the chunks of a batch, on up to 16 host threads when there are
+// several: the block lengths form a dependent chain through each chunk, a
+// cache and TLB miss a block, and chunks are independent. Their blocks join
+// the tables in chunk order; the first chunk in order that does not walk
+// gives the status and message.
+int walk_chunks(sf_decoder *d, const uint8_t *const *chunks,
+                const size_t *sizes, int n) {
+  if ((int)d->walks.size() < n)
+    d->walks.resize(n);
+  for (int f = 0; f < n; f++) {
+    ChunkWalk &w = d->walks[f];
+    for (auto &k : w.kinds)
+      k.clear();
+    w.max_comp = w.max_bytes = 0;
+    w.status = SF_OK;
+    w.message.clear();
+  }
+  const auto walk = [&](int f) {
+    ChunkWalk &w = d->walks[f];
+    w.status = walk_chunk(d, chunks[f], sizes[f], f, n, w);
+    if (w.status != SF_OK)
+      w.message = g_decode_error;
+  };
+  const int threads = std::min<int>(
+      {n, 16, std::max<int>(1, (int)std::thread::hardware_concurrency())});
+  if (threads <= 1 || d->stored) {
+    for (int f = 0; f < n; f++)
+      walk(f);
+  } else {
+    std::vector<std::thread> pool;
+    pool.reserve(threads - 1);
+    for (int t = 1; t < threads; t++)
+      pool.emplace_back([&, t] {
+        for (int f = t; f < n; f += threads)
+          walk(f);
+      });
+    for (int f = 0; f < n; f += threads)
+      walk(f);
+    for (auto &th : pool)
+      th.join();
+  }
+  for (int f = 0; f < n; f++) {
+    const ChunkWalk &w = d->walks[f];
+    if (w.status != SF_OK)
+      return fail(w.status, w.message);
+    for (int k = 0; k < 3; k++)
+      d->kinds[k].insert(d->kinds[k].end(), w.kinds[k].begin(),
+                         w.kinds[k].end());
+    d->max_comp = std::max(d->max_comp, w.max_comp);
+    d->max_bytes = std::max(d->max_bytes, w.max_bytes);
   }
   return SF_OK;
 }
@@ -952,11 +1018,9 @@ extern "C" int sf_hip_decoder_index(sf_decoder *d,
     k.clear();
   d->max_comp = 0;
   d->max_bytes = 0;
-  for (int f = 0; f < n; f++) {
-    const int status = walk_chunk(d, chunks[f], sizes[f], f, n);
-    if (status != SF_OK)
-      return status;
-  }
+  const int walked = walk_chunks(d, chunks, sizes, n);
+  if (walked != SF_OK)
+    return walked;
   size_t total = 0;
   for (int k = 0; k < 3; k++) {
     if (d->kinds[k].size() > (size_t)INT32_MAX)

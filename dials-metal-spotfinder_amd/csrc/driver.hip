@@ -14,7 +14,10 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/driver_hip.h"
@@ -30,11 +33,14 @@ int fail(int status, std::string what) {
 
 constexpr int kDefaultChunk = 64;
 
+// A submitted frame. Its chunk's bytes are in the driver's pinned staging
+// at [at, at + size), and on their way to the same place in the device
+// staging.
 struct Pending {
   int64_t number;
   bool written;
   std::string algorithm;
-  std::vector<uint8_t> data;
+  size_t at = 0, size = 0;
 };
 
 std::string format(const char *fmt, ...) {
@@ -75,8 +81,14 @@ struct sf_driver {
   sf_labeller *lab = nullptr;
   std::map<int, sf_plan *> plans; // by frame count
   uint8_t *d_frames = nullptr;    // `chunk` frame slots
-  uint8_t *d_stage = nullptr;     // the chunks of a batch
-  size_t stage_cap = 0;
+  // The submitted chunks, 256-byte aligned, in pinned host memory and at the
+  // same offsets in device memory: a submit copies its chunk in and starts
+  // its copy to the device on the decoder's stream, so that the copies run
+  // while later frames are submitted. Both are reused from the start once
+  // no frame is waiting.
+  uint8_t *h_stage = nullptr;
+  uint8_t *d_stage = nullptr;
+  size_t stage_cap = 0, stage_used = 0;
   std::vector<Pending> waiting;
   const uint32_t *mask_bits = nullptr; // the detector mask, device bits
   std::string log;
@@ -90,6 +102,129 @@ struct sf_driver {
 namespace {
 
 void note(sf_driver *d, const std::string &line) { d->log += line + "\n"; }
+
+// Pinned host memory takes long to allocate (its pages are locked), so the
+// drivers of a process share the staging buffers they have finished with:
+// a driver takes the smallest free one that is large enough, and gives it
+// back when it grows or is destroyed. The pool keeps the two largest.
+std::mutex g_pinned_mu;
+std::vector<std::pair<uint8_t *, size_t>> g_pinned;
+
+uint8_t *pinned_take(size_t need, size_t &cap) {
+  {
+    std::lock_guard<std::mutex> lock(g_pinned_mu);
+    size_t best = g_pinned.size();
+    for (size_t k = 0; k < g_pinned.size(); k++)
+      if (g_pinned[k].second >= need &&
+          (best == g_pinned.size() || g_pinned[k].second < g_pinned[best].second))
+        best = k;
+    if (best < g_pinned.size()) {
+      uint8_t *p = g_pinned[best].first;
+      cap = g_pinned[best].second;
+      g_pinned.erase(g_pinned.begin() + best);
+      return p;
+    }
+  }
+  uint8_t *p = nullptr;
+  if (hipHostMalloc(reinterpret_cast<void **>(&p), need) != hipSuccess)
+    return nullptr;
+  cap = need;
+  return p;
+}
+
+void pinned_give(uint8_t *p, size_t cap) {
+  if (!p)
+    return;
+  std::lock_guard<std::mutex> lock(g_pinned_mu);
+  g_pinned.push_back({p, cap});
+  if (g_pinned.size() > 2) {
+    size_t small = 0;
+    for (size_t k = 1; k < g_pinned.size(); k++)
+      if (g_pinned[k].second < g_pinned[small].second)
+        small = k;
+    (void)hipHostFree(g_pinned[small].first);
+    g_pinned.erase(g_pinned.begin() + small);
+  }
+}
+
+hipStream_t decoder_stream(sf_driver *d) {
+  return static_cast<hipStream_t>(sf_hip_decoder_stream(d->dec));
+}
+
+const uint8_t *chunk_bytes(const sf_driver *d, const Pending &p) {
+  return p.size ? d->h_stage + p.at : reinterpret_cast<const uint8_t *>("");
+}
+
+// Room for `end` bytes of staging, `hint` bytes at least when it grows (a
+// batch's worth of chunks like the one being staged). Copies to the device in
+// flight read the old buffers, so the decoder's stream is drained first; the
+// chunks staged so far move to the new buffers.
+int stage_room(sf_driver *d, size_t end, size_t hint) {
+  if (end <= d->stage_cap)
+    return SF_OK;
+  size_t cap = std::max({end, hint, 2 * d->stage_cap, (size_t)64 << 20});
+  int status = sf_hip_decoder_wait(d->dec);
+  if (status != SF_OK)
+    return fail(status, sf_hip_decode_last_error());
+  uint8_t *dev = nullptr;
+  uint8_t *host = pinned_take(cap, cap);
+  hipError_t e = host ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void **>(&dev), cap);
+  if (e == hipSuccess && d->stage_used) {
+    std::memcpy(host, d->h_stage, d->stage_used);
+    e = hipMemcpy(dev, d->d_stage, d->stage_used, hipMemcpyDeviceToDevice);
+  }
+  if (e != hipSuccess) {
+    pinned_give(host, cap);
+    if (dev)
+      (void)hipFree(dev);
+    return fail(SF_DEVICE_ERROR,
+                std::string("staging chunks: ") + hipGetErrorString(e));
+  }
+  pinned_give(d->h_stage, d->stage_cap);
+  if (d->d_stage)
+    (void)hipFree(d->d_stage);
+  d->h_stage = host;
+  d->d_stage = dev;
+  d->stage_cap = cap;
+  return SF_OK;
+}
+
+// A submitted chunk into the staging, and its copy to the device started.
+int stage_chunk(sf_driver *d, const uint8_t *data, size_t size, Pending &p) {
+  p.at = (d->stage_used + 255) / 256 * 256;
+  p.size = size;
+  if (size == 0)
+    return SF_OK;
+  const size_t hint = ((size + 255) / 256 * 256) * ((size_t)d->chunk + 1);
+  const int status = stage_room(d, p.at + size, hint);
+  if (status != SF_OK)
+    return status;
+  // a large chunk is copied in by four threads
+  uint8_t *to = d->h_stage + p.at;
+  if (size >= ((size_t)2 << 20)) {
+    const size_t part = (size / 4 + 4095) / 4096 * 4096;
+    std::thread helpers[3];
+    for (int t = 1; t < 4; t++) {
+      const size_t a = std::min(size, t * part), b = std::min(size, a + part);
+      helpers[t - 1] = std::thread([=] { std::memcpy(to + a, data + a, b - a); });
+    }
+    std::memcpy(to, data, std::min(size, part));
+    for (auto &h : helpers)
+      h.join();
+  } else {
+    std::memcpy(to, data, size);
+  }
+  const hipError_t e =
+      hipMemcpyAsync(d->d_stage + p.at, d->h_stage + p.at, size,
+                     hipMemcpyHostToDevice, decoder_stream(d));
+  if (e != hipSuccess)
+    return fail(SF_DEVICE_ERROR,
+                std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+  d->stage_used = p.at + size;
+  return SF_OK;
+}
 
 void frame_failed(sf_driver *d, int64_t number, const std::string &message) {
   // src/find_spots.cc:528-533: "key: what()"
@@ -188,9 +323,8 @@ int decode_group(sf_driver *d, std::vector<const Pending *> &frames,
     std::vector<const uint8_t *> host(frames.size());
     std::vector<size_t> sizes(frames.size());
     for (size_t f = 0; f < frames.size(); f++) {
-      host[f] = frames[f]->data.empty() ? reinterpret_cast<const uint8_t *>("")
-                                        : frames[f]->data.data();
-      sizes[f] = frames[f]->data.size();
+      host[f] = chunk_bytes(d, *frames[f]);
+      sizes[f] = frames[f]->size;
     }
     const int status = sf_hip_decoder_index(d->dec, host.data(), sizes.data(),
                                             (int)frames.size(),
@@ -210,37 +344,12 @@ int decode_group(sf_driver *d, std::vector<const Pending *> &frames,
     frames.erase(frames.begin() + bad);
     slots.erase(slots.begin() + bad);
   }
-  // the chunks into the device staging, 256-byte aligned
-  std::vector<size_t> at(frames.size());
-  size_t total = 0;
-  for (size_t f = 0; f < frames.size(); f++) {
-    at[f] = total;
-    total += (frames[f]->data.size() + 255) / 256 * 256;
-  }
-  if (total > d->stage_cap) {
-    if (d->d_stage)
-      (void)hipFree(d->d_stage);
-    d->d_stage = nullptr;
-    d->stage_cap = 0;
-    const hipError_t e =
-        hipMalloc(reinterpret_cast<void **>(&d->d_stage), total);
-    if (e != hipSuccess)
-      return fail(SF_DEVICE_ERROR,
-                  std::string("hipMalloc: ") + hipGetErrorString(e));
-    d->stage_cap = total;
-  }
+  // the chunks are in the device staging (copied on the decoder's stream
+  // when submitted, so the decode follows their copies)
   std::vector<const uint8_t *> chunks(frames.size());
   std::vector<void *> out(frames.size());
   for (size_t f = 0; f < frames.size(); f++) {
-    if (!frames[f]->data.empty()) {
-      const hipError_t e =
-          hipMemcpy(d->d_stage + at[f], frames[f]->data.data(),
-                    frames[f]->data.size(), hipMemcpyHostToDevice);
-      if (e != hipSuccess)
-        return fail(SF_DEVICE_ERROR,
-                    std::string("hipMemcpy: ") + hipGetErrorString(e));
-    }
-    chunks[f] = d->d_stage + at[f];
+    chunks[f] = d->d_stage ? d->d_stage + frames[f]->at : nullptr;
     out[f] = d->d_frames + (size_t)slots[f] * d->frame_bytes;
   }
   int status = sf_hip_decoder_decode(d->dec, chunks.data(), out.data());
@@ -251,11 +360,9 @@ int decode_group(sf_driver *d, std::vector<const Pending *> &frames,
     std::vector<const Pending *> kept;
     std::vector<int> kept_slots;
     for (size_t f = 0; f < frames.size(); f++) {
-      const int one = sf_hip_decoder_image(
-          d->dec,
-          frames[f]->data.empty() ? reinterpret_cast<const uint8_t *>("")
-                                  : frames[f]->data.data(),
-          frames[f]->data.size(), algorithm.c_str(), out[f]);
+      const int one =
+          sf_hip_decoder_image(d->dec, chunk_bytes(d, *frames[f]),
+                               frames[f]->size, algorithm.c_str(), out[f]);
       if (one == SF_OK) {
         kept.push_back(frames[f]);
         kept_slots.push_back(slots[f]);
@@ -304,6 +411,10 @@ int run_batch(sf_driver *d, size_t n) {
     for (size_t f = 0; f < g.second.first.size(); f++)
       decoded.push_back({g.second.first[f]->number, g.second.second[f]});
   }
+  // the batch's chunks are decoded (the decoder waited for its stream):
+  // with no frame waiting, the staging starts over
+  if (d->waiting.empty())
+    d->stage_used = 0;
   std::sort(decoded.begin(), decoded.end());
   if (d->mask_bits && !decoded.empty()) {
     // the detector mask into the decoded frames, before the threshold
@@ -422,7 +533,9 @@ extern "C" int sf_hip_driver_submit(sf_driver *d, int64_t number,
   p.written = data != nullptr;
   if (data) {
     p.algorithm = algorithm;
-    p.data.assign(data, data + size);
+    const int status = stage_chunk(d, data, size, p);
+    if (status != SF_OK)
+      return status;
   }
   d->waiting.push_back(std::move(p));
   d->stats.dispatched++;
@@ -568,14 +681,17 @@ extern "C" void sf_hip_driver_destroy(sf_driver *d) {
     return;
   for (auto &p : d->plans)
     sf_hip_plan_destroy(p.second);
-  if (d->dec)
+  if (d->dec) {
+    (void)sf_hip_decoder_wait(d->dec); // copies from the staging in flight
     sf_hip_decoder_destroy(d->dec);
+  }
   if (d->lab)
     sf_hip_labeller_destroy(d->lab);
   if (d->d_frames)
     (void)hipFree(d->d_frames);
   if (d->d_stage)
     (void)hipFree(d->d_stage);
+  pinned_give(d->h_stage, d->stage_cap);
   delete d;
 }
 
