@@ -1307,9 +1307,15 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
                              const uint32_t *node_index,
                              const uint32_t *node_value,
                              const unsigned long long *totals,
-                             sf_spot *spots, sf_pixel *pixels) {
+                             sf_spot *spots, unsigned long long spots_from,
+                             sf_pixel *pixels,
+                             unsigned long long pixels_from) {
   const uint32_t closed = *d_closed_runs;
+  // rows are numbered over the whole table; `spots` and `pixels` hold it
+  // from row spots_from and pixels_from on
   const unsigned long long spot_base = totals[0], pixel_base = totals[1];
+  sf_spot *spot_rows = spots + (spot_base - spots_from);
+  sf_pixel *pixel_rows = pixels + (pixel_base - pixels_from);
   // the kept spots' rows, after what earlier chunks wrote
   const uint32_t comps = *n_comps;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < comps;
@@ -1318,7 +1324,7 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
       continue;
     sf_spot sp = res[c].spot;
     sp.first = pixel_base + (uint32_t)spot_rank[c];
-    spots[spot_base + (spot_rank[c] >> 32)] = sp;
+    spot_rows[spot_rank[c] >> 32] = sp;
   }
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
        g += gridDim.x * blockDim.x) {
@@ -1335,7 +1341,7 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
     px.frame = node_frame[g];
     px.index = node_index[g];
     px.value = node_value[g];
-    pixels[pixel_base + (uint32_t)spot_rank[c] + (at - comp_mem[c])] = px;
+    pixel_rows[(uint32_t)spot_rank[c] + (at - comp_mem[c])] = px;
   }
 }
 
@@ -1623,30 +1629,66 @@ template <typename T> struct DevArray {
     cap = want;
     return hipSuccess;
   }
-  // grow keeping the first `keep` elements (stream must be idle)
-  hipError_t grow_keep(size_t n, size_t keep) {
-    if (n <= cap)
-      return hipSuccess;
-    size_t want = cap ? cap : 1024;
-    while (want < n)
-      want *= 2;
-    T *fresh = nullptr;
-    hipError_t e = hipMalloc(&fresh, want * sizeof(T));
-    if (e != hipSuccess)
-      return e;
-    if (ptr && keep)
-      e = hipMemcpy(fresh, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice);
-    if (ptr)
-      (void)hipFree(ptr);
-    ptr = fresh;
-    cap = want;
-    return e;
-  }
   void release() {
     if (ptr)
       (void)hipFree(ptr);
     ptr = nullptr;
     cap = 0;
+  }
+};
+
+// A result table that only grows: a list of device segments, each holding
+// the rows from its `base` (the table's row count when it was started) on.
+// When a chunk's rows may not fit in the last segment, a new one (twice as
+// large, or larger) starts; nothing is copied or freed until release(), so
+// growing never waits for the device.
+template <typename T> struct Segments {
+  struct Seg {
+    T *ptr;
+    size_t base, cap;
+  };
+  std::vector<Seg> segs;
+  // room for `need` rows after the `total` already written
+  hipError_t room(size_t total, size_t need, const Seg *&out) {
+    if (!segs.empty()) {
+      const Seg &last = segs.back();
+      if (total >= last.base && total - last.base + need <= last.cap) {
+        out = &last;
+        return hipSuccess;
+      }
+    }
+    size_t cap = segs.empty() ? 4 * need : 2 * segs.back().cap;
+    cap = std::max<size_t>(cap, 1024);
+    while (cap < need)
+      cap *= 2;
+    T *fresh = nullptr;
+    const hipError_t e = hipMalloc(&fresh, cap * sizeof(T));
+    if (e != hipSuccess)
+      return e;
+    segs.push_back(Seg{fresh, total, cap});
+    out = &segs.back();
+    return hipSuccess;
+  }
+  // the first `total` rows, in order, into host memory
+  hipError_t read(T *host, size_t total) const {
+    for (size_t k = 0; k < segs.size(); k++) {
+      const size_t lo = segs[k].base;
+      const size_t hi = std::min(total, k + 1 < segs.size() ? segs[k + 1].base
+                                                           : total);
+      if (hi > lo) {
+        const hipError_t e = hipMemcpy(host + lo, segs[k].ptr,
+                                       (hi - lo) * sizeof(T),
+                                       hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+          return e;
+      }
+    }
+    return hipSuccess;
+  }
+  void release() {
+    for (auto &g : segs)
+      (void)hipFree(g.ptr);
+    segs.clear();
   }
 };
 
@@ -1725,8 +1767,8 @@ struct sf_labeller {
   sf::DevArray<sf::ChunkDesc> chunk;
   uint32_t *d_scalars = nullptr; // 8 device counters of a chunk
   // results
-  sf::DevArray<sf_spot> spots;
-  sf::DevArray<sf_pixel> pixels;
+  sf::Segments<sf_spot> spots;
+  sf::Segments<sf_pixel> pixels;
   unsigned long long *d_counts = nullptr; // 6
   unsigned long long *d_totals = nullptr; // spots, pixels written
   uint64_t bound_spots = 0, bound_pixels = 0;
@@ -1860,15 +1902,17 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   // outputs: at worst every node is kept
   l->bound_pixels += n - l->n_carry;
   l->bound_spots += (n - l->n_carry) / std::max<uint64_t>(1, a.min_size) + 1;
+  // the segments the chunk's spots and pixels go to: the totals as the
+  // last chunk's kernels left them in mapped memory (the stream is idle)
+  const sf::Segments<sf_spot>::Seg *spot_seg = nullptr;
+  const sf::Segments<sf_pixel>::Seg *pixel_seg = nullptr;
   {
-    // the totals as the last chunk's kernels left them in mapped memory
-    // (the stream is idle here)
     const unsigned long long totals[2] = {l->h_totals[0], l->h_totals[1]};
-    LAB_CHECK(l->spots.grow_keep(totals[0] + n / std::max<uint64_t>(
-                                                  1, a.min_size) + 1,
-                                 totals[0]),
+    LAB_CHECK(l->spots.room(totals[0],
+                            n / std::max<uint64_t>(1, a.min_size) + 1,
+                            spot_seg),
               "growing the spot table");
-    LAB_CHECK(l->pixels.grow_keep(totals[1] + n + 1, totals[1]),
+    LAB_CHECK(l->pixels.room(totals[1], (size_t)n + 1, pixel_seg),
               "growing the pixel table");
   }
   // the chunk table, through pinned staging (the stream is idle, so the
@@ -2133,8 +2177,8 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
       a, l->results.ptr, d_comps, d_closed_runs, l->run_of.ptr,
       l->pos_of.ptr, l->run_head.ptr, l->run_pack.ptr, l->run_rank.ptr,
       l->comp_mem.ptr, l->spot_rank.ptr, l->node_frame.ptr,
-      l->node_index.ptr, l->node_value.ptr, l->d_totals, l->spots.ptr,
-      l->pixels.ptr);
+      l->node_index.ptr, l->node_value.ptr, l->d_totals, spot_seg->ptr,
+      spot_seg->base, pixel_seg->ptr, pixel_seg->base);
   sf::close_chunk<<<1, 256, 0, s>>>(l->tallies.ptr, finish_grid,
                                     d_comps, l->spot_rank.ptr, l->results.ptr,
                                     l->d_counts, l->d_totals, l->hd_totals);
@@ -2708,13 +2752,9 @@ int sf_hip_labeller_results(sf_labeller *l, sf_spot *spots, sf_pixel *pixels,
   if (status != SF_OK)
     return status;
   if (spots && n_spots)
-    LAB_CHECK(hipMemcpy(spots, l->spots.ptr, n_spots * sizeof(sf_spot),
-                        hipMemcpyDeviceToHost),
-              "reading spots");
+    LAB_CHECK(l->spots.read(spots, n_spots), "reading spots");
   if (pixels && n_pixels)
-    LAB_CHECK(hipMemcpy(pixels, l->pixels.ptr, n_pixels * sizeof(sf_pixel),
-                        hipMemcpyDeviceToHost),
-              "reading pixels");
+    LAB_CHECK(l->pixels.read(pixels, n_pixels), "reading pixels");
   if (counts) {
     unsigned long long c[6];
     LAB_CHECK(hipMemcpy(c, l->d_counts, sizeof(c), hipMemcpyDeviceToHost),
