@@ -167,6 +167,10 @@ struct sf_plan {
   hipGraphExec_t pipe_exec = nullptr;
   sf_plan *pipe_cur = nullptr, *pipe_next = nullptr;
   hipEvent_t handoff = nullptr;
+  // a reader of the lists on another stream (the labeller's snapshot): the
+  // plan's next write of its lists waits for it
+  hipEvent_t read_done = nullptr;
+  bool reader_pending = false;
 };
 
 namespace {
@@ -206,9 +210,13 @@ void drop_pipe_graph(sf_plan *p) {
 void plan_free(sf_plan *p) {
   if (p->stream)
     (void)hipStreamSynchronize(p->stream);
+  if (p->reader_pending)
+    (void)hipEventSynchronize(p->read_done);
   drop_pipe_graph(p);
   if (p->handoff)
     (void)hipEventDestroy(p->handoff);
+  if (p->read_done)
+    (void)hipEventDestroy(p->read_done);
   if (p->exec)
     (void)hipGraphExecDestroy(p->exec);
   if (p->graph)
@@ -274,6 +282,16 @@ void layout(sf_plan *p) {
     d.out_count = p->d_found + f;
   }
   p->items = item;
+}
+
+// Work enqueued on `s` that writes the plan's lists waits for their last
+// reader on another stream.
+int wait_reader(sf_plan *p, hipStream_t s) {
+  if (p && p->reader_pending) {
+    SF_CHECK(hipStreamWaitEvent(s, p->read_done, 0), "hipStreamWaitEvent");
+    p->reader_pending = false;
+  }
+  return SF_OK;
 }
 
 int upload_desc(sf_plan *p, const void *const *frames) {
@@ -445,6 +463,8 @@ sf_plan *sf_hip_plan_create(int pixel_bytes, int n_frames,
                  hipSuccess;
   ok = ok && hipEventCreateWithFlags(&p->handoff, hipEventDisableTiming) ==
                  hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&p->read_done, hipEventDisableTiming) ==
+                 hipSuccess;
   if (!ok) {
     g_last_error = "sf_hip_plan_create: device allocation failed";
     plan_free(p);
@@ -602,6 +622,9 @@ int sf_hip_plan_run(sf_plan *p, const void *const *frames) {
   if (up != SF_OK)
     return up;
   p->scouted = false;
+  const int waited = wait_reader(p, p->stream);
+  if (waited != SF_OK)
+    return waited;
   return enqueue(p, p->timing);
 }
 
@@ -648,6 +671,9 @@ int sf_hip_plan_capture(sf_plan *p, const void *const *frames) {
 int sf_hip_plan_replay(sf_plan *p) {
   if (!p || !p->exec)
     return bad("sf_hip_plan_replay: nothing captured");
+  const int waited = wait_reader(p, p->stream);
+  if (waited != SF_OK)
+    return waited;
   SF_CHECK(hipGraphLaunch(p->exec, p->stream), "hipGraphLaunch");
   return SF_OK;
 }
@@ -797,6 +823,8 @@ int sf_hip_plan_pipeline(sf_plan *cur, sf_plan *next) {
     return status;
   if ((status = pipeline_wait_other(next, s)) != SF_OK)
     return status;
+  if ((status = wait_reader(cur, s)) != SF_OK)
+    return status;
   sf_plan *owner = cur ? cur : next;
   if ((status = pipeline_enqueue(cur, next, s, owner->timing)) != SF_OK)
     return status;
@@ -838,6 +866,8 @@ int sf_hip_plan_replay_pipeline(sf_plan *cur, sf_plan *next) {
     return bad("sf_hip_plan_replay_pipeline: no step captured for these "
                "plans");
   if ((status = pipeline_wait_other(next, s)) != SF_OK)
+    return status;
+  if ((status = wait_reader(cur, s)) != SF_OK)
     return status;
   SF_CHECK(hipGraphLaunch(owner->pipe_exec, s), "hipGraphLaunch");
   return pipeline_after(cur, next, s);
@@ -1117,6 +1147,16 @@ int plan_lists(sf_plan *p, std::vector<const sf_signal_pixel *> &lists,
     capacity[f] = p->capacity[f];
   }
   *stream = p->stream;
+  return SF_OK;
+}
+
+// A reader of the plan's lists enqueued on `reader` (another stream): the
+// plan's next write of its lists waits for what is enqueued there now.
+int plan_add_reader(sf_plan *p, hipStream_t reader) {
+  if (!p)
+    return SF_BAD_ARGUMENT;
+  SF_CHECK(hipEventRecord(p->read_done, reader), "hipEventRecord");
+  p->reader_pending = true;
   return SF_OK;
 }
 } // namespace sf

@@ -1724,6 +1724,11 @@ struct sf_labeller {
   uint32_t *hd_snap_counts[2] = {nullptr, nullptr};
   size_t snap_frames_cap[2] = {0, 0};
   hipEvent_t snap_ready[2] = {nullptr, nullptr}, snap_free[2] = {nullptr, nullptr};
+  // add_plan's snapshots run on a stream of their own, after the plan's run
+  // that wrote the lists (snap_after), so that the plan's later work does not
+  // wait for them
+  hipStream_t snap_stream = nullptr;
+  hipEvent_t snap_after[2] = {nullptr, nullptr};
   int snap_side = 0;
   hipEvent_t done = nullptr;
   bool started = false;
@@ -2207,6 +2212,7 @@ namespace sf {
 int plan_lists(sf_plan *plan, std::vector<const sf_signal_pixel *> &lists,
                std::vector<const uint32_t *> &counts,
                std::vector<uint32_t> &capacity, hipStream_t *stream);
+int plan_add_reader(sf_plan *plan, hipStream_t reader);
 } // namespace sf
 
 namespace {
@@ -2400,6 +2406,8 @@ sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
   l->options = o;
   bool ok =
       hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&l->snap_stream, hipStreamNonBlocking) ==
+                 hipSuccess;
   ok = ok && hipEventCreateWithFlags(&l->done, hipEventDisableTiming) ==
                  hipSuccess;
   ok = ok && hipEventCreateWithFlags(&l->runs_ready, hipEventDisableTiming) ==
@@ -2411,6 +2419,8 @@ sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
     ok = ok && hipEventCreateWithFlags(&l->snap_ready[i],
                                        hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&l->snap_free[i],
+                                       hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&l->snap_after[i],
                                        hipEventDisableTiming) == hipSuccess;
   }
   ok = ok && hipMalloc(&l->d_scalars, 8 * sizeof(uint32_t)) == hipSuccess;
@@ -2448,14 +2458,17 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   stop_worker(l); // labels what was handed over first
   if (l->stream)
     (void)hipStreamSynchronize(l->stream);
+  if (l->snap_stream) // a snapshot may still be copying into a side
+    (void)hipStreamSynchronize(l->snap_stream);
   for (int i = 0; i < 2; i++) {
-    // the plans' streams may still be copying into a side
     if (l->snap_ready[i]) {
       (void)hipEventSynchronize(l->snap_ready[i]);
       (void)hipEventDestroy(l->snap_ready[i]);
     }
     if (l->snap_free[i])
       (void)hipEventDestroy(l->snap_free[i]);
+    if (l->snap_after[i])
+      (void)hipEventDestroy(l->snap_after[i]);
     l->snap_list[i].release();
     l->snap_table[i].release();
     l->snap_counts[i].release();
@@ -2528,6 +2541,8 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
     (void)hipEventDestroy(l->done);
   if (l->stream)
     (void)hipStreamDestroy(l->stream);
+  if (l->snap_stream)
+    (void)hipStreamDestroy(l->snap_stream);
   delete l;
 }
 
@@ -2636,21 +2651,29 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
         k > 0 || (l->started && first_frame == l->last_frame + 1);
     at += caps[k];
   }
-  // on the plan's stream: after the run that wrote the lists, and before
-  // anything enqueued there later
-  LAB_CHECK(hipStreamWaitEvent(producer, l->snap_free[side], 0),
+  // on the snapshot stream: after the plan's run that wrote the lists (what
+  // is enqueued on the plan's stream now) and after the side's last reader;
+  // the plan's next write of its lists waits for the snapshot, and nothing
+  // else the plan's stream does later waits for it
+  LAB_CHECK(hipEventRecord(l->snap_after[side], producer), "event");
+  LAB_CHECK(hipStreamWaitEvent(l->snap_stream, l->snap_after[side], 0),
+            "event wait");
+  LAB_CHECK(hipStreamWaitEvent(l->snap_stream, l->snap_free[side], 0),
             "event wait");
   LAB_CHECK(hipMemcpyAsync(l->snap_table[side].ptr, l->h_snap_table[side],
                            n_frames * sizeof(sf::SnapFrame),
-                           hipMemcpyHostToDevice, producer),
+                           hipMemcpyHostToDevice, l->snap_stream),
             "snapshot table");
   sf::snapshot_lists<<<dim3(32, std::min<uint32_t>(n_frames, 65535)), 256, 0,
-                       producer>>>(l->snap_table[side].ptr, n_frames,
-                                   l->snap_list[side].ptr,
-                                   l->snap_counts[side].ptr,
-                                   l->hd_snap_counts[side]);
+                       l->snap_stream>>>(l->snap_table[side].ptr, n_frames,
+                                         l->snap_list[side].ptr,
+                                         l->snap_counts[side].ptr,
+                                         l->hd_snap_counts[side]);
   LAB_CHECK(hipGetLastError(), "snapshot");
-  LAB_CHECK(hipEventRecord(l->snap_ready[side], producer), "event");
+  LAB_CHECK(hipEventRecord(l->snap_ready[side], l->snap_stream), "event");
+  status = sf::plan_add_reader(plan, l->snap_stream);
+  if (status != SF_OK)
+    return status;
   {
     std::lock_guard<std::mutex> lock(l->mu);
     l->job.frames = std::move(frames);
