@@ -40,10 +40,10 @@ hipError_t launch_sparse(const FrameDesc *, int, uint32_t, uint32_t,
                          hipStream_t);
 hipError_t launch_sparse32_pipeline(const FrameDesc *, int, uint32_t,
                                     uint32_t, const FrameDesc *, int, uint32_t,
-                                    hipStream_t, hipEvent_t *);
+                                    hipStream_t, hipEvent_t *, bool);
 hipError_t launch_sparse_pipeline(const FrameDesc *, int, uint32_t, uint32_t,
                                   const FrameDesc *, int, uint32_t,
-                                  hipStream_t, hipEvent_t *);
+                                  hipStream_t, hipEvent_t *, bool);
 } // namespace sf
 
 namespace {
@@ -167,6 +167,10 @@ struct sf_plan {
   hipGraphExec_t pipe_exec = nullptr;
   sf_plan *pipe_cur = nullptr, *pipe_next = nullptr;
   hipEvent_t handoff = nullptr;
+  // deferred ordering: a pipelined step that finishes this plan leaves its
+  // ordering to sf_hip_plan_order (order_pending until then); whether the
+  // captured step was captured so
+  bool defer_order = false, order_pending = false, pipe_deferred = false;
   // a reader of the lists on another stream (the labeller's snapshot): the
   // plan's next write of its lists waits for it
   hipEvent_t read_done = nullptr;
@@ -622,6 +626,7 @@ int sf_hip_plan_run(sf_plan *p, const void *const *frames) {
   if (up != SF_OK)
     return up;
   p->scouted = false;
+  p->order_pending = false; // a run orders its own lists
   const int waited = wait_reader(p, p->stream);
   if (waited != SF_OK)
     return waited;
@@ -671,6 +676,7 @@ int sf_hip_plan_capture(sf_plan *p, const void *const *frames) {
 int sf_hip_plan_replay(sf_plan *p) {
   if (!p || !p->exec)
     return bad("sf_hip_plan_replay: nothing captured");
+  p->order_pending = false; // a replayed run orders its own lists
   const int waited = wait_reader(p, p->stream);
   if (waited != SF_OK)
     return waited;
@@ -764,6 +770,10 @@ int pipeline_check(sf_plan *cur, sf_plan *next, const char *what,
   if (running && cur && !cur->scouted)
     return bad(std::string(what) +
                ": the current plan's frames were not scouted by a step");
+  // scouting clears the tables that a pending ordering reads
+  if (running && next && next->order_pending)
+    return bad(std::string(what) +
+               ": the next plan's ordering is pending (sf_hip_plan_order)");
   if (running && next && next->bound.size() != (size_t)next->n)
     return bad(std::string(what) + ": the next plan has no frames bound");
   *stream = cur ? cur->stream : next->stream;
@@ -788,15 +798,17 @@ int pipeline_after(sf_plan *cur, sf_plan *next, hipStream_t s) {
     SF_CHECK(hipStreamWaitEvent(next->stream, owner->handoff, 0),
              "hipStreamWaitEvent");
   }
-  if (cur)
+  if (cur) {
     cur->scouted = false;
+    cur->order_pending = cur->defer_order;
+  }
   if (next)
     next->scouted = true;
   return SF_OK;
 }
 
 int pipeline_enqueue(sf_plan *cur, sf_plan *next, hipStream_t s,
-                     bool timing) {
+                     bool timing, bool order) {
   sf_plan *owner = cur ? cur : next;
   // events as the plan's pass times read them: first mixed launch, second,
   // ordering
@@ -807,7 +819,8 @@ int pipeline_enqueue(sf_plan *cur, sf_plan *next, hipStream_t s,
   SF_CHECK(launch(cur ? cur->d_desc : nullptr, cur ? cur->n : 0,
                   cur ? cur->items : 0, cur ? cur->max_cell_rows : 0,
                   next ? next->d_desc : nullptr, next ? next->n : 0,
-                  next ? next->items : 0, s, timing ? marks : nullptr),
+                  next ? next->items : 0, s, timing ? marks : nullptr,
+                  order),
            "launching a pipelined step");
   return SF_OK;
 }
@@ -823,10 +836,12 @@ int sf_hip_plan_pipeline(sf_plan *cur, sf_plan *next) {
     return status;
   if ((status = pipeline_wait_other(next, s)) != SF_OK)
     return status;
-  if ((status = wait_reader(cur, s)) != SF_OK)
+  const bool order = !(cur && cur->defer_order);
+  if (order && (status = wait_reader(cur, s)) != SF_OK)
     return status;
   sf_plan *owner = cur ? cur : next;
-  if ((status = pipeline_enqueue(cur, next, s, owner->timing)) != SF_OK)
+  if ((status = pipeline_enqueue(cur, next, s, owner->timing, order)) !=
+      SF_OK)
     return status;
   return pipeline_after(cur, next, s);
 }
@@ -842,7 +857,8 @@ int sf_hip_plan_capture_pipeline(sf_plan *cur, sf_plan *next) {
   drop_pipe_graph(owner);
   SF_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal),
            "hipStreamBeginCapture");
-  const int enqueued = pipeline_enqueue(cur, next, s, false);
+  const bool order = !(cur && cur->defer_order);
+  const int enqueued = pipeline_enqueue(cur, next, s, false, order);
   hipGraph_t graph = nullptr;
   const hipError_t end = hipStreamEndCapture(s, &graph);
   if (enqueued != SF_OK)
@@ -853,6 +869,7 @@ int sf_hip_plan_capture_pipeline(sf_plan *cur, sf_plan *next) {
            "hipGraphInstantiate");
   owner->pipe_cur = cur;
   owner->pipe_next = next;
+  owner->pipe_deferred = !order;
   return SF_OK;
 }
 
@@ -865,12 +882,42 @@ int sf_hip_plan_replay_pipeline(sf_plan *cur, sf_plan *next) {
   if (!owner->pipe_exec || owner->pipe_cur != cur || owner->pipe_next != next)
     return bad("sf_hip_plan_replay_pipeline: no step captured for these "
                "plans");
+  if (cur && owner->pipe_deferred != cur->defer_order)
+    return bad("sf_hip_plan_replay_pipeline: the step was captured with "
+               "the other ordering setting; capture it again");
   if ((status = pipeline_wait_other(next, s)) != SF_OK)
     return status;
-  if ((status = wait_reader(cur, s)) != SF_OK)
+  if (!owner->pipe_deferred && (status = wait_reader(cur, s)) != SF_OK)
     return status;
   SF_CHECK(hipGraphLaunch(owner->pipe_exec, s), "hipGraphLaunch");
   return pipeline_after(cur, next, s);
+}
+
+int sf_hip_plan_set_deferred_order(sf_plan *p, int deferred) {
+  if (!p)
+    return bad("sf_hip_plan_set_deferred_order: no plan");
+  if (p->order_pending)
+    return bad("sf_hip_plan_set_deferred_order: an ordering is pending");
+  p->defer_order = deferred != 0;
+  return SF_OK;
+}
+
+int sf_hip_plan_order(sf_plan *p) {
+  if (!p)
+    return bad("sf_hip_plan_order: no plan");
+  if (!p->order_pending)
+    return bad("sf_hip_plan_order: no pipelined step left this plan's "
+               "ordering pending");
+  // after the step (enqueued on this plan's stream), and after the last
+  // reader of the lists it rewrites
+  const int waited = wait_reader(p, p->stream);
+  if (waited != SF_OK)
+    return waited;
+  SF_CHECK(sf::launch_sparse_order(p->d_desc, p->n, p->max_cell_rows,
+                                   p->stream),
+           "launching the ordering");
+  p->order_pending = false;
+  return SF_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -1163,16 +1163,17 @@ uint32_t mixed_grid(uint32_t scout_blocks, uint32_t sparse_blocks) {
 }
 
 // One pipelined step on `stream`: the threshold of `cur` (n_cur frames,
-// scouted by the previous step) finished -- erosion, Poisson, ordering --
-// while `next` (n_next frames) is cleared and scouted. Either side may be
-// empty. `marks` (optional, four events): before the first mixed launch,
+// scouted by the previous step) finished -- erosion, Poisson, and the
+// ordering unless `order` is false (then launch_sparse_order runs it later)
+// -- while `next` (n_next frames) is cleared and scouted. Either side may
+// be empty. `marks` (optional, four events): before the first mixed launch,
 // after it, after the second, after the ordering.
 hipError_t launch_sparse_pipeline(const FrameDesc *cur, int n_cur,
                                   uint32_t cur_items,
                                   uint32_t cur_max_cell_rows,
                                   const FrameDesc *next, int n_next,
                                   uint32_t next_items, hipStream_t stream,
-                                  hipEvent_t *marks) {
+                                  hipEvent_t *marks, bool order) {
   hipError_t e;
   if (n_next > 0)
     zero_kernel<<<n_next, 256, 0, stream>>>(next);
@@ -1204,7 +1205,7 @@ hipError_t launch_sparse_pipeline(const FrameDesc *cur, int n_cur,
     scout_poisson_kernel<<<grid_b, 256, 0, stream>>>(b_scout, sparse);
   if (marks && (e = hipEventRecord(marks[2], stream)) != hipSuccess)
     return e;
-  if (n_cur > 0) {
+  if (n_cur > 0 && order) {
     e = launch_sparse_order(cur, n_cur, cur_max_cell_rows, stream);
     if (e != hipSuccess)
       return e;

@@ -950,7 +950,7 @@ hipError_t launch_sparse32_pipeline(const FrameDesc *cur, int n_cur,
                                     uint32_t cur_max_cell_rows,
                                     const FrameDesc *next, int n_next,
                                     uint32_t next_items, hipStream_t stream,
-                                    hipEvent_t *marks) {
+                                    hipEvent_t *marks, bool order) {
   hipError_t e;
   if (n_next > 0 && (e = launch_sparse_zero(next, n_next, stream)) !=
                         hipSuccess)
@@ -983,7 +983,7 @@ hipError_t launch_sparse32_pipeline(const FrameDesc *cur, int n_cur,
     scout_poisson32_kernel<<<grid_b, 256, 0, stream>>>(b_scout, sparse);
   if (marks && (e = hipEventRecord(marks[2], stream)) != hipSuccess)
     return e;
-  if (n_cur > 0) {
+  if (n_cur > 0 && order) {
     e = launch_sparse_order(cur, n_cur, cur_max_cell_rows, stream);
     if (e != hipSuccess)
       return e;

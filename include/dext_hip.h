@@ -237,6 +237,18 @@ int sf_hip_plan_pipeline(sf_plan *current, sf_plan *next);
  * frames). Capture again after either plan changes form. */
 int sf_hip_plan_capture_pipeline(sf_plan *current, sf_plan *next);
 int sf_hip_plan_replay_pipeline(sf_plan *current, sf_plan *next);
+/* Deferred ordering (no reference counterpart). With `deferred` set, a
+ * pipelined step that finishes this plan (as `current`, eager or captured
+ * after the setting) stops after its Poisson test and leaves the ordering of
+ * its lists to sf_hip_plan_order, which enqueues it on the plan's stream.
+ * The next step, on the next plan's stream, does not wait for it: with three
+ * plans in turn, a chunk's ordering runs beside the next chunk's step. Until
+ * sf_hip_plan_order the plan's lists and counts are not yet those of the
+ * step, and the plan cannot be the `next` of a step (SF_BAD_ARGUMENT);
+ * sf_hip_plan_order with no ordering pending, or a setting change while one
+ * is, is SF_BAD_ARGUMENT. A run or replay orders its own lists. */
+int sf_hip_plan_set_deferred_order(sf_plan *plan, int deferred);
+int sf_hip_plan_order(sf_plan *plan);
 
 /* ---- 3-D grouping and centroids on the device
  * (dials_spots::Labeller, src/dials_spots.hh:108-158) -----------------------

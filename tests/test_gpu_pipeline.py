@@ -249,3 +249,103 @@ def test_refusals(sf):
     for q in (a, b, c32):
         q.close()
     buf.close()
+
+
+def test_deferred_ordering_three_plans_and_the_labeller(sf):
+    """Three plans in turn with deferred ordering: each captured step
+    (P[c] -> P[c+1]) leaves P[c]'s ordering to order(), which runs while the
+    next step does; every list equals the oracle's, and the labeller fed
+    from the ordered plans gives the oracle Labeller's tables. At 32 bits
+    the same rotation, eager, gives the oracle's lists."""
+    shapes = [(256, 384)] * 3
+    n = 7
+    chunks = [chunk_frames(sf, shapes, 300 + 10 * c, density=60000.0)
+              for c in range(n)]
+    p = [sf.Plan(2, shapes) for _ in range(3)]
+    for q in p:
+        q.set_deferred_order(True)
+    p[0].bind(chunks[0][1])
+    sf.pipeline(None, p[0])
+    lab = sf.Labeller(256, 384)
+    oracle = cpu.Labeller(256, 384)
+    for c in range(n):
+        cur, nxt = p[c % 3], p[(c + 1) % 3]
+        if c + 1 < n:
+            nxt.bind(chunks[c + 1][1])
+            if c < 3:  # each pair's graph, captured on first use
+                sf.capture_pipeline(cur, nxt)
+            sf.replay_pipeline(cur, nxt)
+        else:
+            sf.pipeline(cur, None)
+        cur.order()
+        lab.add_plan(cur, 3 * c)
+        for k, host in enumerate(chunks[c][2]):
+            assert oracle.add(3 * c + k, cpu.dext(host)[1]) == 0
+        if c % 2:
+            check(cur, chunks[c][2], ("chunk", c))
+    lab.finish()
+    oracle.finish()
+    got, want = lab.results(), oracle.results()
+    assert got[2] == want[2]
+    assert got[0].tobytes() == want[0].tobytes()
+    assert got[1].tobytes() == want[1].tobytes()
+    assert want[2]["accepted"] > 10
+    lab.close()
+    for q in p:
+        q.close()
+    for bufs, _, _ in chunks:
+        for b in bufs:
+            b.close()
+    shapes32 = [(216, 206), (101, 3), (300, 517)]
+    chunks32 = [chunk_frames32(sf, shapes32, 700 + 10 * c) for c in range(4)]
+    p32 = [sf.Plan(4, shapes32) for _ in range(3)]
+    for q in p32:
+        q.set_deferred_order(True)
+    p32[0].bind(chunks32[0][1])
+    sf.pipeline(None, p32[0])
+    for c in range(4):
+        cur, nxt = p32[c % 3], p32[(c + 1) % 3]
+        if c + 1 < 4:
+            nxt.bind(chunks32[c + 1][1])
+            sf.pipeline(cur, nxt)
+        else:
+            sf.pipeline(cur, None)
+        cur.order()
+        check(cur, chunks32[c][2], ("u32 chunk", c))
+    for q in p32:
+        q.close()
+    for bufs, _, _ in chunks32:
+        for b in bufs:
+            b.close()
+
+
+def test_deferred_ordering_refusals(sf):
+    a, b = sf.Plan(2, [(64, 64)]), sf.Plan(2, [(64, 64)])
+    buf = sf.DeviceBuffer(2 * 64 * 64)
+    with pytest.raises(ValueError):
+        a.order()  # nothing pending
+    a.set_deferred_order(True)
+    a.bind([buf.ptr])
+    sf.pipeline(None, a)
+    b.bind([buf.ptr])
+    sf.capture_pipeline(a, b)
+    sf.pipeline(a, b)  # a's ordering now pending
+    with pytest.raises(ValueError):
+        a.set_deferred_order(False)  # a setting change while one is pending
+    with pytest.raises(ValueError):
+        sf.pipeline(b, a)  # a cannot be scouted before it is ordered
+    a.order()
+    with pytest.raises(ValueError):
+        a.order()  # done already
+    a.set_deferred_order(False)
+    a.bind([buf.ptr])
+    sf.pipeline(None, a)
+    b.bind([buf.ptr])
+    with pytest.raises(ValueError):
+        sf.replay_pipeline(a, b)  # captured with the other setting
+    sf.pipeline(a, b)
+    a.wait()
+    assert a.counts()[0] == a.counts()[0]  # ordered by the step itself
+    for q in (a, b):
+        q.close()
+    buf.close()
