@@ -337,6 +337,8 @@ def pipelined_step_timing(sf, plans, chunks, steps, labeller=None,
     t0 = time.perf_counter()
     for s in range(steps):
         step(s)
+    if labeller is not None:
+        labeller.wait()
     sf.lib().sf_hip_synchronize()
     per_step = (time.perf_counter() - t0) / steps
     drain(steps)
@@ -912,6 +914,9 @@ def main() -> int:
     t0 = time.perf_counter()
     for s in range(args.steps):
         step(s)
+    # every chunk of the timed steps labelled, not just enqueued
+    if labeller is not None:
+        labeller.wait()
     sf.lib().sf_hip_synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps * chunk / elapsed
@@ -966,6 +971,8 @@ def main() -> int:
         done, first = pending.pop()
         labeller.add_plan(done, first_frame=first)
     pending.clear()
+    if labeller is not None:
+        labeller.wait()
     sf.lib().sf_hip_synchronize()
     replay_fps = args.steps * chunk / (time.perf_counter() - t1)
     plans[1].close()

@@ -1641,19 +1641,30 @@ template <typename T> struct DevArray {
 // the rows from its `base` (the table's row count when it was started) on.
 // When a chunk's rows may not fit in the last segment, a new one (twice as
 // large, or larger) starts; nothing is copied or freed until release(), so
-// growing never waits for the device.
+// growing never waits for the device. A large hipMalloc can itself take a
+// long time (hundreds of milliseconds for a gigabyte has been seen), so
+// once the last segment is half full the next one is allocated ahead, on a
+// helper thread, and taken when it is needed.
 template <typename T> struct Segments {
   struct Seg {
     T *ptr;
     size_t base, cap;
   };
   std::vector<Seg> segs;
+  int device = 0;
+  std::thread ahead;     // allocating `spare`
+  T *spare = nullptr;    // written by `ahead`, read after joining it
+  size_t spare_cap = 0;
   // room for `need` rows after the `total` already written
   hipError_t room(size_t total, size_t need, const Seg *&out) {
     if (!segs.empty()) {
       const Seg &last = segs.back();
       if (total >= last.base && total - last.base + need <= last.cap) {
         out = &last;
+        // the next chunk may not fit either: start the next segment
+        if (2 * (total - last.base + need) > last.cap && !spare &&
+            !ahead.joinable())
+          start_ahead(2 * last.cap);
         return hipSuccess;
       }
     }
@@ -1661,13 +1672,37 @@ template <typename T> struct Segments {
     cap = std::max<size_t>(cap, 1024);
     while (cap < need)
       cap *= 2;
+    if (ahead.joinable())
+      ahead.join();
     T *fresh = nullptr;
-    const hipError_t e = hipMalloc(&fresh, cap * sizeof(T));
-    if (e != hipSuccess)
-      return e;
+    if (spare && spare_cap >= need) {
+      fresh = spare;
+      cap = spare_cap;
+    } else {
+      if (spare)
+        (void)hipFree(spare);
+      const hipError_t e = hipMalloc(&fresh, cap * sizeof(T));
+      if (e != hipSuccess) {
+        spare = nullptr;
+        spare_cap = 0;
+        return e;
+      }
+    }
+    spare = nullptr;
+    spare_cap = 0;
     segs.push_back(Seg{fresh, total, cap});
     out = &segs.back();
     return hipSuccess;
+  }
+  void start_ahead(size_t cap) {
+    ahead = std::thread([this, cap] {
+      T *p = nullptr;
+      if (hipSetDevice(device) == hipSuccess &&
+          hipMalloc(&p, cap * sizeof(T)) == hipSuccess) {
+        spare = p;
+        spare_cap = cap;
+      }
+    });
   }
   // the first `total` rows, in order, into host memory
   hipError_t read(T *host, size_t total) const {
@@ -1685,7 +1720,17 @@ template <typename T> struct Segments {
     }
     return hipSuccess;
   }
+  ~Segments() {
+    if (ahead.joinable())
+      ahead.join();
+  }
   void release() {
+    if (ahead.joinable())
+      ahead.join();
+    if (spare)
+      (void)hipFree(spare);
+    spare = nullptr;
+    spare_cap = 0;
     for (auto &g : segs)
       (void)hipFree(g.ptr);
     segs.clear();
@@ -2415,6 +2460,7 @@ sf_labeller *sf_hip_labeller_create(size_t height, size_t width,
   ok = ok && hipEventCreateWithFlags(&l->synced, hipEventDisableTiming) ==
                  hipSuccess;
   ok = ok && hipGetDevice(&l->device) == hipSuccess;
+  l->spots.device = l->pixels.device = l->device;
   for (int i = 0; i < 2; i++) {
     ok = ok && hipEventCreateWithFlags(&l->snap_ready[i],
                                        hipEventDisableTiming) == hipSuccess;
