@@ -254,6 +254,17 @@ def cpu_u32_threshold(host_frames, threads, repeats=5):
 BATCH_SHAPES = [(2162, 2068)] * 32 + [(3269, 3110)] * 16 + [(4362, 4148)] * 8
 
 
+def reserve_for(labeller, chunks_done, chunks_ahead):
+    """Sizes the labeller's result tables for `chunks_ahead` more chunks
+    like the `chunks_done` it has labelled (with a quarter to spare), so that
+    timed steps allocate nothing: a large device allocation can stall a
+    series for hundreds of milliseconds or more."""
+    spots, pixels = labeller.sizes()
+    per = 4 * max(chunks_done, 1)
+    labeller.reserve(5 * spots * chunks_ahead // per + 1024,
+                     5 * pixels * chunks_ahead // per + 1024)
+
+
 def chunk_step_timing(sf, plan, chunks, steps, labeller=None, frames_per=0):
     """Graph replays of `plan` over `chunks` (lists of device pointers) in
     turn, with the labeller fed from each replay when given; returns
@@ -271,6 +282,8 @@ def chunk_step_timing(sf, plan, chunks, steps, labeller=None, frames_per=0):
 
     step(0)
     plan.wait()
+    if labeller is not None:
+        reserve_for(labeller, 1, steps)
     sf.lib().sf_hip_synchronize()
     t0 = time.perf_counter()
     for s in range(1, steps + 1):
@@ -330,6 +343,8 @@ def pipelined_step_timing(sf, plans, chunks, steps, labeller=None,
     sf.pipeline(None, plans[0])
     step(0)
     drain(1)
+    if labeller is not None:
+        reserve_for(labeller, 2, steps + 1)
     sf.lib().sf_hip_synchronize()
     bind(plans[0], 0)
     sf.pipeline(None, plans[0])
@@ -907,6 +922,10 @@ def main() -> int:
     for s in range(args.warmup):
         step(s)
     drain(args.warmup)
+    if labeller is not None:
+        # the timed steps, their drain, the unpipelined steps and the
+        # labeller alone below
+        reserve_for(labeller, args.warmup + 1, 2 * args.steps + 6)
     sf.lib().sf_hip_synchronize()
     bind(plans[0], 0)
     sf.pipeline(None, plans[0])

@@ -1642,9 +1642,11 @@ template <typename T> struct DevArray {
 // When a chunk's rows may not fit in the last segment, a new one (twice as
 // large, or larger) starts; nothing is copied or freed until release(), so
 // growing never waits for the device. A large hipMalloc can itself take a
-// long time (hundreds of milliseconds for a gigabyte has been seen), so
-// once the last segment is half full the next one is allocated ahead, on a
-// helper thread, and taken when it is needed.
+// long time (from hundreds of milliseconds to seconds has been seen, and
+// other threads' allocations wait meanwhile), so once the last segment has
+// room for fewer than three more chunks like the current one the next is
+// allocated ahead, on a helper thread, and taken when it is needed; and a
+// caller that knows its series' length reserves the rows up front.
 template <typename T> struct Segments {
   struct Seg {
     T *ptr;
@@ -1661,8 +1663,8 @@ template <typename T> struct Segments {
       const Seg &last = segs.back();
       if (total >= last.base && total - last.base + need <= last.cap) {
         out = &last;
-        // the next chunk may not fit either: start the next segment
-        if (2 * (total - last.base + need) > last.cap && !spare &&
+        // the chunks after the next few may not fit: start the next segment
+        if (last.cap - (total - last.base + need) < 3 * need && !spare &&
             !ahead.joinable())
           start_ahead(2 * last.cap);
         return hipSuccess;
@@ -1692,6 +1694,32 @@ template <typename T> struct Segments {
     spare_cap = 0;
     segs.push_back(Seg{fresh, total, cap});
     out = &segs.back();
+    return hipSuccess;
+  }
+  // room for `rows` more rows after `total` without allocating again: a
+  // new last segment (base `total`) unless the last one has it
+  hipError_t reserve(size_t total, size_t rows) {
+    if (!segs.empty()) {
+      const Seg &last = segs.back();
+      if (total >= last.base && total - last.base + rows <= last.cap)
+        return hipSuccess;
+    }
+    if (ahead.joinable())
+      ahead.join();
+    const size_t cap = std::max<size_t>(rows, 1024);
+    T *fresh = nullptr;
+    if (spare && spare_cap >= cap) {
+      fresh = spare;
+    } else {
+      const hipError_t e = hipMalloc(&fresh, cap * sizeof(T));
+      if (e != hipSuccess)
+        return e;
+      if (spare)
+        (void)hipFree(spare);
+    }
+    segs.push_back(Seg{fresh, total, spare == fresh ? spare_cap : cap});
+    spare = nullptr;
+    spare_cap = 0;
     return hipSuccess;
   }
   void start_ahead(size_t cap) {
@@ -2776,6 +2804,22 @@ int sf_hip_labeller_wait(sf_labeller *l) {
     return drained;
   LAB_CHECK(poll_stream(l), "waiting for the labeller");
   l->n_carry = *l->h_scalars;
+  return SF_OK;
+}
+
+int sf_hip_labeller_reserve(sf_labeller *l, size_t spots, size_t pixels) {
+  if (!l) {
+    g_label_error = "sf_hip_labeller_reserve: bad argument";
+    return SF_BAD_ARGUMENT;
+  }
+  int status = flush_pending(l);
+  if (status == SF_OK)
+    status = sf_hip_labeller_wait(l);
+  if (status != SF_OK)
+    return status;
+  const unsigned long long totals[2] = {l->h_totals[0], l->h_totals[1]};
+  LAB_CHECK(l->spots.reserve(totals[0], spots), "reserving spot rows");
+  LAB_CHECK(l->pixels.reserve(totals[1], pixels), "reserving pixel rows");
   return SF_OK;
 }
 

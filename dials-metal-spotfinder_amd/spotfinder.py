@@ -167,6 +167,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     fn("sf_hip_labeller_live", _sz, _p)
     fn("sf_hip_labeller_sizes", _i, _p, ctypes.POINTER(_sz),
        ctypes.POINTER(_sz))
+    fn("sf_hip_labeller_reserve", _i, _p, _sz, _sz)
     fn("sf_hip_labeller_spot_count", _sz, _p)
     fn("sf_hip_labeller_pixel_count", _sz, _p)
     fn("sf_hip_labeller_results", _i, _p, _p, _p, _p)
@@ -633,6 +634,21 @@ class Labeller:
 
     def live(self) -> int:
         return int(lib().sf_hip_labeller_live(self.handle))
+
+    def sizes(self):
+        """(spot rows, pixel rows) so far."""
+        n_spots, n_pixels = _sz(), _sz()
+        self._status(lib().sf_hip_labeller_sizes(
+            self.handle, ctypes.byref(n_spots), ctypes.byref(n_pixels)),
+            "sizes")
+        return int(n_spots.value), int(n_pixels.value)
+
+    def reserve(self, spots: int, pixels: int) -> None:
+        """Room for `spots` more spot rows and `pixels` more pixel rows,
+        allocated now, so that the chunks that follow allocate nothing (an
+        extension: the reference's tables are host vectors)."""
+        self._status(lib().sf_hip_labeller_reserve(
+            self.handle, int(spots), int(pixels)), "reserve")
 
     def results(self):
         """(spots, pixels, counts dict), as the reference's spots(),

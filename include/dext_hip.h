@@ -304,6 +304,14 @@ int sf_hip_labeller_sizes(sf_labeller *labeller, size_t *spots,
                           size_t *pixels);
 size_t sf_hip_labeller_spot_count(sf_labeller *labeller);
 size_t sf_hip_labeller_pixel_count(sf_labeller *labeller);
+/* Room for `spots` more spot rows and `pixels` more pixel rows in the
+ * device result tables, allocated now (after waiting for what was added), so
+ * that the next chunks allocate nothing. No counterpart in the reference,
+ * whose tables are host vectors: for callers that know their series'
+ * length, since a large device allocation can stall for hundreds of
+ * milliseconds or more while a series runs. */
+int sf_hip_labeller_reserve(sf_labeller *labeller, size_t spots,
+                            size_t pixels);
 /* spots(), pixels() and counts() (any pointer may be NULL). */
 int sf_hip_labeller_results(sf_labeller *labeller, sf_spot *spots,
                             sf_pixel *pixels, sf_counts *counts);

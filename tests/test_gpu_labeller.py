@@ -194,6 +194,46 @@ def test_device_lists_across_chunks(sf):
         plan.close()
 
 
+def test_reserved_tables(sf):
+    """Result rows spread over table segments in every way reserve() can
+    leave them: reserved before the first chunk (too small, so the chunks
+    then grow the table by several segments), reserved in the middle of a
+    series (a new segment while the last still has room), reserved twice
+    (the second a no-op); the tables read back equal the oracle's."""
+    h, w, n, per = 300, 412, 16, 4
+    buf = sf.DeviceBuffer(2 * h * w * n)
+    spots = fr.diffraction_spots(7, h, w, n, 1500.0 * 40)
+    sf.synth_frames(buf, 2, h, w, n, 7, 0.3, True, 0.0005, spots)
+    host = np.zeros((n, h, w), dtype=np.uint16)
+    buf.download(host)
+    oracle = cpu.Labeller(h, w)
+    for z in range(n):
+        assert oracle.add(z, cpu.dext(host[z])[1]) == 0
+    oracle.finish()
+    expected = oracle.results()
+    assert len(expected[1]) > 4 * 1024
+    plan = sf.Plan(2, [(h, w)] * per)
+    for pattern in ("before", "middle", "twice"):
+        lab = sf.Labeller(h, w)
+        if pattern == "before":
+            lab.reserve(1, 1)
+        for c in range(n // per):
+            plan.run([buf.ptr + 2 * h * w * (per * c + k) for k in range(per)])
+            lab.add_plan(plan, per * c)
+            if c == 1 and pattern != "before":
+                lab.reserve(10_000, 100_000)
+                assert lab.sizes()[1] > 0
+                if pattern == "twice":
+                    lab.reserve(10, 10)
+        lab.finish()
+        got = lab.results()
+        assert got[2] == expected[2], (pattern, got[2], expected[2])
+        assert same_tables(got, expected), pattern
+        lab.close()
+    plan.close()
+    buf.close()
+
+
 def test_chunks_past_the_merge_sort_limit(sf):
     """Chunks of more than 65,536 runs, where the component sort takes the
     radix (onesweep) path instead of rocPRIM's merge sort: device lists of
