@@ -765,11 +765,14 @@ def measure_decode(sf, series, chunk: int, frame_bytes: int,
     }
 
 
-def measure_driver(sf, chunks: list, repeats: int = 2):
+def measure_driver(sf, chunks: list, repeats: int = 4):
     """The series driver (sf_hip_driver_*) over the compressed chunks from
     host memory: submit by number, decode, threshold and group in order,
     finish with the summary; host clock over the whole run, which includes
-    the chunks' copies to the device (PCIe)."""
+    the chunks' copies to the device (PCIe) and the driver's set-up. The
+    fastest of `repeats` runs: a run's set-up allocates device and pinned
+    memory, and one such allocation has taken hundreds of milliseconds on
+    the MI355X box now and then."""
     import tempfile
 
     times, lines, spots, tables, write_s = [], [], 0, None, None
@@ -798,6 +801,7 @@ def measure_driver(sf, chunks: list, repeats: int = 2):
                     f"{WIDTH} u16 in host memory through the driver (copy "
                     "to the device, decode, threshold, group, summary)",
         "s_per_run": round(t, 4),
+        "s_per_run_median": round(float(np.median(times)), 4),
         "frames_per_s": round(len(chunks) / t, 1),
         "spots": spots,
         "summary": lines[1:3],
