@@ -813,7 +813,7 @@ def measure_driver(sf, chunks: list, repeats: int = 4):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--chunk", type=int, default=60)
     ap.add_argument("--frames", type=int, default=1800)
