@@ -1056,212 +1056,202 @@ __global__ __launch_bounds__(256) void component_sums(
   const uint32_t lane = lane_id();
   double *mine = terms[threadIdx.x >> 6];
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  const uint32_t wave_id = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  // Components in groups of consecutive ones, about one group a wave: a
-  // lane reads one component's bounds, lanes of rejected sizes store n and
-  // kind themselves, and the wave takes the group's kept components in turn
-  // (most components are rejected, and each would cost the wave a round
-  // trip of its own).
-  const uint32_t group = max(1u, min(64u, (comps + waves - 1) / waves));
-  for (uint32_t g0 = wave_id * group; g0 < comps; g0 += waves * group) {
-    const uint32_t cl = g0 + lane;
-    const bool valid = lane < group && cl < comps;
-    uint32_t s0l = 0, nl = 0;
-    if (valid) {
-      s0l = starts[cl];
-      nl = (cl + 1 < comps ? starts[cl + 1] : end_all) - s0l;
+  uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // each component's bounds are read one component ahead
+  uint32_t next_s0 = c < comps ? starts[c] : 0u;
+  uint32_t next_end = c + 1 < comps ? starts[c + 1] : end_all;
+  for (; c < comps; c += waves) {
+    const uint32_t s0 = next_s0;
+    const uint32_t n = next_end - s0;
+    if (c + waves < comps) {
+      next_s0 = starts[c + waves];
+      next_end = c + waves + 1 < comps ? starts[c + waves + 1] : end_all;
     }
-    const bool rejected = valid && (nl < a.min_size || nl > a.max_size);
-    if (rejected) {
-      sums[cl].n = nl;
-      sums[cl].kind = nl < a.min_size ? kTooSmall : kTooLarge;
-    }
-    uint64_t keep = ballot(valid && !rejected);
-    while (keep) {
-      const int kl = __builtin_ctzll(keep);
-      keep &= keep - 1;
-      const uint32_t c = g0 + (uint32_t)kl;
-      const uint32_t s0 = (uint32_t)__builtin_amdgcn_readlane((int)s0l, kl);
-      const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)nl, kl);
-      // fields are stored as soon as they are known (by lane 0, or lane
-      // `axis` for the variance sums)
-      CompSums *out = sums + c;
-      // the first round's members in registers: lane + 64 j, j < 4
-      uint32_t m_idx[4], m_val[4];
-      int32_t m_z[4];
-  #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const uint32_t k = lane + 64u * j;
-        m_idx[j] = k < n ? mem_index[s0 + k] : 0;
-        m_val[j] = k < n ? mem_value[s0 + k] : 0;
-        m_z[j] = k < n ? mem_frame[s0 + k] : 0;
+    // fields are stored as soon as they are known (by lane 0, or lane
+    // `axis` for the variance sums); a rejected size needs only n and kind
+    CompSums *out = sums + c;
+    if (n < a.min_size || n > a.max_size) {
+      if (lane == 0) {
+        out->n = n;
+        out->kind = n < a.min_size ? kTooSmall : kTooLarge;
       }
-      // bbox, peak, sums, and the bounds of the sums' exactness
-      uint32_t x0 = 0xFFFFFFFFu, x1 = 0, y0 = 0xFFFFFFFFu, y1 = 0;
-      int32_t z0 = INT32_MAX, z1 = INT32_MIN;
-      double sv = 0.0, svv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-      uint32_t vmax = 0, kmax = 0xFFFFFFFFu, zabs = 0, imax = 0;
-      int32_t zmax = 0;
-      auto visit = [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
+      continue;
+    }
+    // the first round's members in registers: lane + 64 j, j < 4
+    uint32_t m_idx[4], m_val[4];
+    int32_t m_z[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t k = lane + 64u * j;
+      m_idx[j] = k < n ? mem_index[s0 + k] : 0;
+      m_val[j] = k < n ? mem_value[s0 + k] : 0;
+      m_z[j] = k < n ? mem_frame[s0 + k] : 0;
+    }
+    // bbox, peak, sums, and the bounds of the sums' exactness
+    uint32_t x0 = 0xFFFFFFFFu, x1 = 0, y0 = 0xFFFFFFFFu, y1 = 0;
+    int32_t z0 = INT32_MAX, z1 = INT32_MIN;
+    double sv = 0.0, svv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    uint32_t vmax = 0, kmax = 0xFFFFFFFFu, zabs = 0, imax = 0;
+    int32_t zmax = 0;
+    auto visit = [&](uint32_t k, uint32_t idx, uint32_t v, int32_t z) {
+      const uint32_t y = row_of(a, idx);
+      const uint32_t x = idx - y * a.width;
+      x0 = min(x0, x);
+      x1 = max(x1, x + 1);
+      y0 = min(y0, y);
+      y1 = max(y1, y + 1);
+      z0 = min(z0, z);
+      z1 = max(z1, z + 1);
+      const double value = (double)v;
+      sv += value;
+      svv += value * value;
+      sx += value * (double)(2 * x + 1);
+      sy += value * (double)(2 * y + 1);
+      sz += value * (double)(2 * (int64_t)z + 1);
+      zabs = max(zabs, (uint32_t)(z < 0 ? -(int64_t)z : (int64_t)z));
+      if (v > vmax) { // lanes see members in order, so first per lane
+        vmax = v;
+        kmax = k;
+        imax = idx;
+        zmax = z;
+      }
+    };
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t k = lane + 64u * j;
+      if (k < n)
+        visit(k, m_idx[j], m_val[j], m_z[j]);
+    }
+    for (uint32_t k = lane + kTermRound; k < n; k += 64)
+      visit(k, mem_index[s0 + k], mem_value[s0 + k], mem_frame[s0 + k]);
+    x0 = wave_min_u32(x0);
+    x1 = wave_max_u32(x1);
+    y0 = wave_min_u32(y0);
+    y1 = wave_max_u32(y1);
+    z0 = wave_min_i32(z0);
+    z1 = wave_max_i32(z1);
+    zabs = wave_max_u32(zabs);
+    const uint32_t vtop = wave_max_u32(vmax);
+    const uint32_t ktop = wave_min_u32(vmax == vtop ? kmax : 0xFFFFFFFFu);
+    // Exactness of the reference's double sums: every term and partial
+    // sum below 2^52 (with margin) in magnitude.
+    const double bound = 4503599627370496.0; // 2^52
+    const double top = (double)vtop, cnt = (double)n;
+    const double reach =
+        (double)max(max(a.width, a.pixels / a.width), zabs) * 2.0 + 1.0;
+    const bool exact = cnt * top * top < bound && cnt * top * reach < bound;
+    double sum, weighted[3];
+    if (exact) {
+      sum = wave_sum_f64(sv);
+      const double sum_sq = wave_sum_f64(svv);
+      weighted[0] = wave_sum_f64(sx) * 0.5;
+      weighted[1] = wave_sum_f64(sy) * 0.5;
+      weighted[2] = wave_sum_f64(sz) * 0.5;
+      double peak[3] = {0.0, 0.0, 0.0};
+      if (vtop > 0) { // member ktop was lane (ktop mod 64)'s
+        const int owner = (int)(ktop & 63u);
+        const uint32_t idx =
+            (uint32_t)__builtin_amdgcn_readlane((int)imax, owner);
+        const int32_t z = __builtin_amdgcn_readlane(zmax, owner);
         const uint32_t y = row_of(a, idx);
-        const uint32_t x = idx - y * a.width;
-        x0 = min(x0, x);
-        x1 = max(x1, x + 1);
-        y0 = min(y0, y);
-        y1 = max(y1, y + 1);
-        z0 = min(z0, z);
-        z1 = max(z1, z + 1);
-        const double value = (double)v;
-        sv += value;
-        svv += value * value;
-        sx += value * (double)(2 * x + 1);
-        sy += value * (double)(2 * y + 1);
-        sz += value * (double)(2 * (int64_t)z + 1);
-        zabs = max(zabs, (uint32_t)(z < 0 ? -(int64_t)z : (int64_t)z));
-        if (v > vmax) { // lanes see members in order, so first per lane
-          vmax = v;
-          kmax = k;
-          imax = idx;
-          zmax = z;
-        }
-      };
-  #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const uint32_t k = lane + 64u * j;
-        if (k < n)
-          visit(k, m_idx[j], m_val[j], m_z[j]);
+        peak[0] = (double)(idx - y * a.width) + 0.5;
+        peak[1] = (double)y + 0.5;
+        peak[2] = (double)z + 0.5;
       }
-      for (uint32_t k = lane + kTermRound; k < n; k += 64)
-        visit(k, mem_index[s0 + k], mem_value[s0 + k], mem_frame[s0 + k]);
-      x0 = wave_min_u32(x0);
-      x1 = wave_max_u32(x1);
-      y0 = wave_min_u32(y0);
-      y1 = wave_max_u32(y1);
-      z0 = wave_min_i32(z0);
-      z1 = wave_max_i32(z1);
-      zabs = wave_max_u32(zabs);
-      const uint32_t vtop = wave_max_u32(vmax);
-      const uint32_t ktop = wave_min_u32(vmax == vtop ? kmax : 0xFFFFFFFFu);
-      // Exactness of the reference's double sums: every term and partial
-      // sum below 2^52 (with margin) in magnitude.
-      const double bound = 4503599627370496.0; // 2^52
-      const double top = (double)vtop, cnt = (double)n;
-      const double reach =
-          (double)max(max(a.width, a.pixels / a.width), zabs) * 2.0 + 1.0;
-      const bool exact = cnt * top * top < bound && cnt * top * reach < bound;
-      double sum, weighted[3];
-      if (exact) {
-        sum = wave_sum_f64(sv);
-        const double sum_sq = wave_sum_f64(svv);
-        weighted[0] = wave_sum_f64(sx) * 0.5;
-        weighted[1] = wave_sum_f64(sy) * 0.5;
-        weighted[2] = wave_sum_f64(sz) * 0.5;
-        double peak[3] = {0.0, 0.0, 0.0};
-        if (vtop > 0) { // member ktop was lane (ktop mod 64)'s
-          const int owner = (int)(ktop & 63u);
-          const uint32_t idx =
-              (uint32_t)__builtin_amdgcn_readlane((int)imax, owner);
-          const int32_t z = __builtin_amdgcn_readlane(zmax, owner);
-          const uint32_t y = row_of(a, idx);
-          peak[0] = (double)(idx - y * a.width) + 0.5;
-          peak[1] = (double)y + 0.5;
-          peak[2] = (double)z + 0.5;
+      if (lane == 0) {
+        out->sum = sum;
+        out->sum_sq = sum_sq;
+        for (int axis = 0; axis < 3; axis++) {
+          out->weighted[axis] = weighted[axis];
+          out->peak[axis] = peak[axis];
         }
-        if (lane == 0) {
-          out->sum = sum;
-          out->sum_sq = sum_sq;
-          for (int axis = 0; axis < 3; axis++) {
-            out->weighted[axis] = weighted[axis];
-            out->peak[axis] = peak[axis];
-          }
-          out->bbox[0] = (int32_t)x0;
-          out->bbox[1] = (int32_t)x1;
-          out->bbox[2] = (int32_t)y0;
-          out->bbox[3] = (int32_t)y1;
-          out->bbox[4] = z0;
-          out->bbox[5] = z1;
-          out->n = n;
-          out->kind = -1;
-        }
-      } else {
-        CompSums cs; // lane 0's
-        if (lane == 0) {
-          memset(&cs, 0, sizeof(cs));
-          sequential_sums(a, s0, n, mem_frame, mem_index, mem_value, cs);
-          cs.n = n;
-          cs.kind = -1;
-          *out = cs;
-        }
-        sum = readlane_f64(cs.sum, 0);
-        for (int axis = 0; axis < 3; axis++)
-          weighted[axis] = readlane_f64(cs.weighted[axis], 0);
+        out->bbox[0] = (int32_t)x0;
+        out->bbox[1] = (int32_t)x1;
+        out->bbox[2] = (int32_t)y0;
+        out->bbox[3] = (int32_t)y1;
+        out->bbox[4] = z0;
+        out->bbox[5] = z1;
+        out->n = n;
+        out->kind = -1;
       }
-      double acc = 0.0; // lane a < 3: axis a's variance sum, in member order
-      if (sum > 0.0) {
-        // the centroid as finish_spot forms it; then the terms
-        double centre[3];
-        for (int axis = 0; axis < 3; axis++)
-          centre[axis] = weighted[axis] / sum;
-        for (uint32_t k0 = 0; k0 < n; k0 += kTermRound) {
-  #pragma unroll
-          for (int j = 0; j < 4; j++) {
-            const uint32_t k = k0 + lane + 64u * j;
-            uint32_t idx = m_idx[j], v = m_val[j];
-            int32_t z = m_z[j];
-            if (k0 > 0) {
-              idx = k < n ? mem_index[s0 + k] : 0;
-              v = k < n ? mem_value[s0 + k] : 0;
-              z = k < n ? mem_frame[s0 + k] : 0;
-            }
-            const uint32_t y = row_of(a, idx);
-            const double value = (double)v;
-            const double position[3] = {(double)(idx - y * a.width) + 0.5,
-                                        (double)y + 0.5, (double)z + 0.5};
-            for (int axis = 0; axis < 3; axis++) {
-              const double delta = position[axis] - centre[axis];
-              mine[axis * kTermStride + lane + 64u * j] = value * delta * delta;
-            }
-          }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          if (lane < 3) {
-            // Sixteen terms at a time, two sets of eight: each set is read
-            // while the other is added. The round's slots past the last
-            // member hold +0.0 (a zero value times a finite square), and
-            // adding +0.0 to the sum, which is never -0.0, leaves it
-            // unchanged, so the last sixteen need no tail.
-            const uint32_t m = (min(kTermRound, n - k0) + 15u) & ~15u;
-            const double *row = mine + lane * kTermStride;
-            double p[8], q[8];
-  #pragma unroll
-            for (int t = 0; t < 8; t++)
-              p[t] = row[t];
-            for (uint32_t i = 0; i < m; i += 16) {
-  #pragma unroll
-              for (int t = 0; t < 8; t++)
-                q[t] = row[i + 8 + t];
-  #pragma unroll
-              for (int t = 0; t < 8; t++)
-                acc += p[t];
-              if (i + 16 < m) {
-  #pragma unroll
-                for (int t = 0; t < 8; t++)
-                  p[t] = row[i + 16 + t];
-              }
-  #pragma unroll
-              for (int t = 0; t < 8; t++)
-                acc += q[t];
-            }
-          }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
+    } else {
+      CompSums cs; // lane 0's
+      if (lane == 0) {
+        memset(&cs, 0, sizeof(cs));
+        sequential_sums(a, s0, n, mem_frame, mem_index, mem_value, cs);
+        cs.n = n;
+        cs.kind = -1;
+        *out = cs;
       }
-      if (lane < 3)
-        out->delta_sq[lane] = acc;
+      sum = readlane_f64(cs.sum, 0);
+      for (int axis = 0; axis < 3; axis++)
+        weighted[axis] = readlane_f64(cs.weighted[axis], 0);
     }
+    double acc = 0.0; // lane a < 3: axis a's variance sum, in member order
+    if (sum > 0.0) {
+      // the centroid as finish_spot forms it; then the terms
+      double centre[3];
+      for (int axis = 0; axis < 3; axis++)
+        centre[axis] = weighted[axis] / sum;
+      for (uint32_t k0 = 0; k0 < n; k0 += kTermRound) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint32_t k = k0 + lane + 64u * j;
+          uint32_t idx = m_idx[j], v = m_val[j];
+          int32_t z = m_z[j];
+          if (k0 > 0) {
+            idx = k < n ? mem_index[s0 + k] : 0;
+            v = k < n ? mem_value[s0 + k] : 0;
+            z = k < n ? mem_frame[s0 + k] : 0;
+          }
+          const uint32_t y = row_of(a, idx);
+          const double value = (double)v;
+          const double position[3] = {(double)(idx - y * a.width) + 0.5,
+                                      (double)y + 0.5, (double)z + 0.5};
+          for (int axis = 0; axis < 3; axis++) {
+            const double delta = position[axis] - centre[axis];
+            mine[axis * kTermStride + lane + 64u * j] = value * delta * delta;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < 3) {
+          // Sixteen terms at a time, two sets of eight: each set is read
+          // while the other is added. The round's slots past the last
+          // member hold +0.0 (a zero value times a finite square), and
+          // adding +0.0 to the sum, which is never -0.0, leaves it
+          // unchanged, so the last sixteen need no tail.
+          const uint32_t m = (min(kTermRound, n - k0) + 15u) & ~15u;
+          const double *row = mine + lane * kTermStride;
+          double p[8], q[8];
+#pragma unroll
+          for (int t = 0; t < 8; t++)
+            p[t] = row[t];
+          for (uint32_t i = 0; i < m; i += 16) {
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+              q[t] = row[i + 8 + t];
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+              acc += p[t];
+            if (i + 16 < m) {
+#pragma unroll
+              for (int t = 0; t < 8; t++)
+                p[t] = row[i + 16 + t];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+              acc += q[t];
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+    if (lane < 3)
+      out->delta_sq[lane] = acc;
   }
 }
 
