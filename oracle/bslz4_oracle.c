@@ -14,7 +14,10 @@
  *            block_size elements, each a 4-byte big-endian LZ4 length and an
  *            LZ4 block holding the block's bit planes; the last partial block
  *            rounded down to a multiple of 8 elements; the last size % 8
- *            elements stored raw;
+ *            elements stored raw. The bit-plane layout is parity unpinned
+ *            against bitshuffle itself: no implementation or fixture of it
+ *            is available here, and only a numpy construction of the
+ *            published layout and hand-made known answers check it;
  *   "lz4"    the HDF5 LZ4 filter's framing (src/decompress.cc:42-70).
  * The LZ4 block decoder follows the published block format
  * (lz4_Block_format.md, liblz4 1.9): sequences of a token, literals, a
