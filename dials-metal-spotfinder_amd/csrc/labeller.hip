@@ -190,98 +190,129 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return readlane_f64(x, 63);
 }
 
-// Nodes per tile of gather_nodes and number_runs, threads per tile (two
-// nodes a thread), and each wave's share of a tile.
+// Nodes per tile of gather_chunk and number_runs, threads per tile of
+// number_runs (two nodes a thread), and each wave's share of a tile.
 constexpr uint32_t kTile = 2048;
 constexpr uint32_t kTileThreads = 1024;
 constexpr uint32_t kTileWaves = kTileThreads / 64;
 constexpr uint32_t kWaveNodes = kTile / kTileWaves;
 
-// Nodes from the frontier and the chunk's lists, in (frame, index) order
-// (and the chunk's signal-pixel count into counts[0]), and which of them
-// begin a run: a horizontal run is consecutive indices on
-// one row of one frame, and every carried node is a run of its own. Bit
-// (g & 63) of head_bits[g >> 6] marks node g a run head; tile t (nodes
-// [t, t + 1) * kTile) counts its heads in head_count[t].
-__global__ __launch_bounds__(kTileThreads) void gather_nodes(
-    LabelArgs a, const ChunkDesc *frames, const int32_t *carry_frame,
-    const uint32_t *carry_index, const uint32_t *carry_value,
-    int32_t *node_frame, uint32_t *node_index, uint32_t *node_value,
-    uint32_t *node_slot, uint64_t *head_bits, uint32_t *head_count,
-    unsigned long long *counts) {
+// A chunk's own nodes (its signal pixels, without the carried frontier) at
+// chunk-local places j: frame k's from first[k] on, in (frame, index) order.
+// index, value and slot (k + 1) per node; bit (j & 63) of heads[j >> 6]
+// marks node j a run head; tile_heads[t] counts the heads of nodes
+// [t, t + 1) * kTile.
+struct ChunkNodes {
+  uint32_t *index, *value, *slot;
+  uint64_t *heads;
+  uint32_t *tile_heads;
+};
+
+// The chunk's nodes out of its frames' lists: node j of frame k is entry
+// j - first[k] of frame k's list (first[n_frames]: the total). A node
+// begins a run when it is its frame's first, the first column of a row, or
+// not the index after its list predecessor's. Blocks take tiles of kTile
+// nodes in turn (the grid may be an upper bound: the total is on the
+// device).
+__global__ __launch_bounds__(256) void gather_chunk(LabelArgs a,
+                                                    const ChunkDesc *frames,
+                                                    uint32_t n_frames,
+                                                    const uint32_t *first,
+                                                    ChunkNodes out) {
+  __shared__ int tile_frame; // the frame of the tile's first node
   __shared__ uint32_t listed;
-  if (blockIdx.x == 0 && threadIdx.x < 64) {
-    // the chunk's signal pixels (frames whose list fit), by one wave
-    uint32_t total = 0; // a chunk's lists hold fewer than 2^32 pixels
-    for (uint32_t k = threadIdx.x; k < a.n_frames; k += 64) {
-      const uint32_t c = *frames[k].count;
-      total += c <= frames[k].capacity ? c : 0;
-    }
-    total =
-        wave_reduce(total, 0u, [](uint32_t p, uint32_t q) { return p + q; });
-    if (threadIdx.x == 0)
-      counts[0] += total;
-  }
-  __shared__ int tile_frame; // the chunk frame of the tile's first chunk node
+  if (n_frames == 0)
+    return;
+  const uint32_t total = first[n_frames];
   const uint32_t lane = lane_id();
-  const uint32_t tile = blockIdx.x * kTile;
-  if (threadIdx.x == 0) {
-    listed = 0;
-    tile_frame = a.n_frames ? chunk_frame_of(frames, a.n_frames,
-                                             max(tile, a.n_carry))
-                            : 0;
-  }
-  __syncthreads();
-  uint32_t mine = 0;
-  int k = tile_frame; // stepped forward to each node's frame
-  for (uint32_t g = tile + threadIdx.x;
-       g < min(tile + kTile, a.n_nodes); g += blockDim.x) {
-    bool head = true;
-    if (g < a.n_carry) {
-      node_frame[g] = carry_frame[g];
-      node_index[g] = carry_index[g];
-      node_value[g] = carry_value[g];
-      node_slot[g] = 0;
-    } else {
-      while (k + 1 < (int)a.n_frames && frames[k + 1].offset <= g)
+  for (uint32_t t = blockIdx.x; t < (total + kTile - 1) / kTile;
+       t += gridDim.x) {
+    const uint32_t tile = t * kTile;
+    if (threadIdx.x == 0) {
+      listed = 0;
+      int lo = 0, hi = (int)n_frames - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= tile)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      tile_frame = lo;
+    }
+    __syncthreads();
+    int k = tile_frame; // stepped forward to each node's frame
+    uint32_t mine = 0;
+    const uint32_t end = min(tile + kTile, total);
+    for (uint32_t j = tile + threadIdx.x; j < end; j += blockDim.x) {
+      while (k + 1 < (int)n_frames && first[k + 1] <= j)
         k++;
       const sf_signal_pixel *list = frames[k].list;
-      const uint32_t at = g - frames[k].offset;
+      const uint32_t at = j - first[k];
       const sf_signal_pixel e = list[at];
-      node_frame[g] = frames[k].number;
-      node_index[g] = e.index;
-      node_value[g] = e.value;
-      node_slot[g] = (uint32_t)k + 1;
-      head = at == 0 || e.index - row_of(a, e.index) * a.width == 0 ||
-             list[at - 1].index != e.index - 1;
+      out.index[j] = e.index;
+      out.value[j] = e.value;
+      out.slot[j] = (uint32_t)k + 1;
+      const bool head = at == 0 || e.index - row_of(a, e.index) * a.width == 0 ||
+                        list[at - 1].index != e.index - 1;
+      const uint64_t mask = ballot(head);
+      if (lane == 0)
+        out.heads[j >> 6] = mask; // the wave's 64 nodes start at j
+      mine += head;
     }
-    const uint64_t mask = ballot(head);
-    if (lane == 0)
-      head_bits[g >> 6] = mask; // the wave's 64 nodes start at g
-    mine += head;
+    if (mine)
+      atomicAdd(&listed, mine);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      out.tile_heads[t] = listed;
+    __syncthreads();
   }
-  if (mine)
-    atomicAdd(&listed, mine);
-  __syncthreads();
-  if (threadIdx.x == 0)
-    head_count[blockIdx.x] = listed;
 }
 
-// Runs numbered in node order: run r is the r-th head. Tile t's first run
-// number is the heads of tiles 0 .. t - 1, which each block adds up itself;
-// each wave takes kWaveNodes consecutive nodes of its tile. Writes run_of per
-// node; per run its head node, first and last pixel index, slot and initial
-// parent -- itself, or for a carried node its carried root: carried nodes
-// are all heads and come first, so their run numbers are their node
-// numbers -- and a zero last_slot. The block of the last tile writes the run
-// count, here and to the host's mapped copy.
+// Runs numbered in node order, over the carried nodes g < n_carry (blocks
+// chunk_tiles on, kTile nodes a block: each a run of its own, numbered as
+// its node, whose parent is its carried root) and the chunk's own nodes
+// (block t: chunk tile t; node j is node g = n_carry + j, and run r is the
+// r-th head). A chunk tile's first run is n_carry plus the heads of the
+// chunk tiles before it, which each block adds up itself; each wave takes
+// kWaveNodes consecutive nodes of its tile. Writes run_of per node; per run
+// its head node, first and last pixel index, slot and initial parent (a
+// chunk run's is itself), and a zero last_slot. The last chunk tile's block
+// (the first block when there is none) writes the run count, here and to
+// the host's mapped copy; the first block adds the chunk's own nodes to the
+// signal-pixel count.
 __global__ __launch_bounds__(kTileThreads) void number_runs(
-    LabelArgs a, const uint64_t *head_bits, const uint32_t *head_count,
-    const uint32_t *node_index, const uint32_t *node_slot,
+    LabelArgs a, const uint32_t *cn_index, const uint32_t *cn_slot,
+    const uint64_t *cn_heads, const uint32_t *cn_tile_heads,
+    uint32_t chunk_tiles, const uint32_t *carry_index,
     const uint32_t *carry_root, uint32_t *run_of, uint32_t *run_head,
     uint32_t *run_idx, uint32_t *run_last, uint32_t *run_slot,
     uint32_t *parent, uint32_t *last_slot, uint32_t *n_runs,
-    uint32_t *host_runs) {
+    uint32_t *host_runs, unsigned long long *counts) {
+  const uint32_t n_chunk = a.n_nodes - a.n_carry;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counts[0] += n_chunk;
+    if (chunk_tiles == 0) {
+      *n_runs = a.n_carry;
+      *host_runs = a.n_carry; // host-mapped
+      __threadfence_system();
+    }
+  }
+  if (blockIdx.x >= chunk_tiles) {
+    const uint32_t base = (blockIdx.x - chunk_tiles) * kTile;
+    const uint32_t end = min(base + kTile, a.n_carry);
+    for (uint32_t g = base + threadIdx.x; g < end; g += blockDim.x) {
+      const uint32_t idx = carry_index[g];
+      run_of[g] = g;
+      run_head[g] = g;
+      run_idx[g] = idx;
+      run_last[g] = idx;
+      run_slot[g] = 0;
+      parent[g] = carry_root[g];
+      last_slot[g] = 0;
+    }
+    return;
+  }
   // part[w]: wave w's share of the earlier tiles' heads; part[kTileWaves +
   // w]: the heads among wave w's own nodes
   __shared__ uint32_t part[2 * kTileWaves];
@@ -289,64 +320,91 @@ __global__ __launch_bounds__(kTileThreads) void number_runs(
   const uint32_t tile = blockIdx.x * kTile;
   uint32_t before = 0;
   for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x)
-    before += head_count[i];
+    before += cn_tile_heads[i];
   before = wave_reduce(before, 0u,
                        [](uint32_t p, uint32_t q) { return p + q; });
   const uint32_t first = tile + kWaveNodes * wave;
   uint32_t own = 0;
-  if (lane < kWaveNodes / 64 && first + 64 * lane < a.n_nodes)
-    own = (uint32_t)__builtin_popcountll(head_bits[(first >> 6) + lane]);
+  if (lane < kWaveNodes / 64 && first + 64 * lane < n_chunk)
+    own = (uint32_t)__builtin_popcountll(cn_heads[(first >> 6) + lane]);
   own = wave_reduce(own, 0u, [](uint32_t p, uint32_t q) { return p + q; });
   if (lane == 0) {
     part[wave] = before;
     part[kTileWaves + wave] = own;
   }
   __syncthreads();
-  uint32_t tiles_before = 0, tile_runs = 0;
+  uint32_t tiles_before = a.n_carry, tile_runs = 0;
   for (uint32_t w = 0; w < kTileWaves; w++) {
     tiles_before += part[w];
     tile_runs += part[kTileWaves + w];
   }
-  uint32_t run = tiles_before; // heads before this wave's nodes
+  uint32_t run = tiles_before; // runs before this wave's nodes
   for (uint32_t w = 0; w < wave; w++)
     run += part[kTileWaves + w];
-  if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0) {
+  if (blockIdx.x + 1 == chunk_tiles && threadIdx.x == 0) {
     *n_runs = tiles_before + tile_runs;
     *host_runs = *n_runs; // host-mapped
     __threadfence_system();
   }
   const uint64_t upto = ~0ull >> (63 - lane);
-  for (uint32_t j = 0; j < kWaveNodes / 64; j++) {
-    const uint32_t g = first + 64 * j + lane;
-    if (first + 64 * j >= a.n_nodes)
+  for (uint32_t q = 0; q < kWaveNodes / 64; q++) {
+    const uint32_t j = first + 64 * q + lane;
+    if (first + 64 * q >= n_chunk)
       break;
-    const uint64_t mask = head_bits[(first >> 6) + j];
+    const uint64_t mask = cn_heads[(first >> 6) + q];
     // the next group's heads, for lane 63's tail test
-    const uint64_t next = first + 64 * (j + 1) < a.n_nodes
-                              ? head_bits[(first >> 6) + j + 1]
+    const uint64_t next = first + 64 * (q + 1) < n_chunk
+                              ? cn_heads[(first >> 6) + q + 1]
                               : 1ull;
-    if (g < a.n_nodes) {
+    if (j < n_chunk) {
+      const uint32_t g = a.n_carry + j;
       const uint32_t r = run + (uint32_t)__builtin_popcountll(mask & upto) - 1;
       run_of[g] = r;
       const bool head = (mask >> lane) & 1;
-      const bool tail = g + 1 == a.n_nodes ||
+      const bool tail = j + 1 == n_chunk ||
                         (lane < 63 ? (mask >> (lane + 1)) & 1 : next & 1);
       if (head || tail) {
-        const uint32_t idx = node_index[g];
+        const uint32_t idx = cn_index[j];
         if (head) {
+          const uint32_t slot = cn_slot[j];
           run_head[r] = g;
           run_idx[r] = idx;
-          run_slot[r] = node_slot[g];
-          parent[r] = g < a.n_carry ? carry_root[g] : r;
+          run_slot[r] = slot;
+          parent[r] = r;
           // the last frame (slot) this run alone reaches, raised to its
           // component's by resolve_runs
-          last_slot[r] = node_slot[g];
+          last_slot[r] = slot;
         }
         if (tail)
           run_last[r] = idx;
       }
     }
     run += (uint32_t)__builtin_popcountll(mask);
+  }
+}
+
+// A node's frame, index and value: carried nodes g < n_carry from the
+// frontier, the chunk's own from its chunk-local arrays (the frame by its
+// slot in the chunk table).
+struct NodeSrc {
+  const int32_t *carry_frame;
+  const uint32_t *carry_index, *carry_value;
+  const uint32_t *index, *value, *slot;
+  const ChunkDesc *frames;
+};
+
+__device__ __forceinline__ void node_at(const LabelArgs &a, const NodeSrc &s,
+                                        uint32_t g, int32_t &frame,
+                                        uint32_t &index, uint32_t &value) {
+  if (g < a.n_carry) {
+    frame = s.carry_frame[g];
+    index = s.carry_index[g];
+    value = s.carry_value[g];
+  } else {
+    const uint32_t j = g - a.n_carry;
+    index = s.index[j];
+    value = s.value[j];
+    frame = s.frames[s.slot[j] - 1].number;
   }
 }
 
@@ -915,9 +973,7 @@ __global__ void gather_members(LabelArgs a, const uint32_t *d_closed_runs,
                                const uint32_t *run_of, const uint32_t *pos_of,
                                const uint32_t *run_head,
                                const unsigned long long *rank,
-                               const int32_t *node_frame,
-                               const uint32_t *node_index,
-                               const uint32_t *node_value, int32_t *mem_frame,
+                               NodeSrc nodes, int32_t *mem_frame,
                                uint32_t *mem_index, uint32_t *mem_value) {
   const uint32_t closed = *d_closed_runs;
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
@@ -927,9 +983,7 @@ __global__ void gather_members(LabelArgs a, const uint32_t *d_closed_runs,
     if (i >= closed)
       continue;
     const uint32_t at = (uint32_t)rank[i] + (g - run_head[r]);
-    mem_frame[at] = node_frame[g];
-    mem_index[at] = node_index[g];
-    mem_value[at] = node_value[g];
+    node_at(a, nodes, g, mem_frame[at], mem_index[at], mem_value[at]);
   }
 }
 
@@ -1303,9 +1357,7 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
                              const unsigned long long *run_rank,
                              const uint32_t *comp_mem,
                              const unsigned long long *spot_rank,
-                             const int32_t *node_frame,
-                             const uint32_t *node_index,
-                             const uint32_t *node_value,
+                             NodeSrc nodes,
                              const unsigned long long *totals,
                              sf_spot *spots, unsigned long long spots_from,
                              sf_pixel *pixels,
@@ -1338,9 +1390,7 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
       continue;
     const uint32_t at = (uint32_t)q + (g - run_head[r]);
     sf_pixel px;
-    px.frame = node_frame[g];
-    px.index = node_index[g];
-    px.value = node_value[g];
+    node_at(a, nodes, g, px.frame, px.index, px.value);
     pixel_rows[(uint32_t)spot_rank[c] + (at - comp_mem[c])] = px;
   }
 }
@@ -1391,9 +1441,8 @@ __global__ __launch_bounds__(256) void close_chunk(
 __global__ void carry_out(LabelArgs a, const uint32_t *run_of,
                           const uint32_t *open_len, const uint32_t *open_at,
                           const uint32_t *root, const uint32_t *run_head,
-                          const uint32_t *n_runs, const int32_t *node_frame,
-                          const uint32_t *node_index,
-                          const uint32_t *node_value, int32_t *carry_frame,
+                          const uint32_t *n_runs, NodeSrc nodes,
+                          int32_t *carry_frame,
                           uint32_t *carry_index, uint32_t *carry_value,
                           uint32_t *carry_root, uint32_t *n_carry_out) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1407,55 +1456,52 @@ __global__ void carry_out(LabelArgs a, const uint32_t *run_of,
     if (!open_len[r])
       continue;
     const uint32_t at = open_at[r] + (g - run_head[r]);
-    carry_frame[at] = node_frame[g];
-    carry_index[at] = node_index[g];
-    carry_value[at] = node_value[g];
+    node_at(a, nodes, g, carry_frame[at], carry_index[at], carry_value[at]);
     carry_root[at] = open_at[root[r]];
   }
 }
 
-// The chunk's signal pixels (frames whose list fit), by one wave.
-__global__ __launch_bounds__(64) void add_signal_count(
-    const ChunkDesc *frames, uint32_t n_frames, unsigned long long *counts) {
-  uint32_t total = 0; // a chunk's lists hold fewer than 2^32 pixels
-  for (uint32_t k = threadIdx.x; k < n_frames; k += 64) {
-    const uint32_t c = *frames[k].count;
-    total += c <= frames[k].capacity ? c : 0;
-  }
-  total = wave_reduce(total, 0u, [](uint32_t p, uint32_t q) { return p + q; });
+// Where each frame's nodes start among a chunk's own nodes, from the counts
+// on the device: first[k] = the counts of frames before k, where a list
+// over its capacity counts 0; first[n_frames] their total. Each frame's
+// count as found also goes to host_counts (host-mapped). One block. This
+// is add_plan's, which copies nothing else: the plan's lists are gathered
+// into the chunk's node arrays (gather_chunk) on the snapshot stream.
+__global__ __launch_bounds__(256) void chunk_offsets(const ChunkDesc *frames,
+                                                     uint32_t n_frames,
+                                                     uint32_t *first,
+                                                     uint32_t *host_counts) {
+  __shared__ uint32_t part[4];
+  __shared__ uint32_t carry;
   if (threadIdx.x == 0)
-    counts[0] += total;
-}
-
-// A plan's lists as add_plan takes them: each frame's list (its count
-// entries, none if the list overflowed its capacity) copied to dst +
-// dst_offset on the plan's own stream, and the counts to device memory and
-// to host-mapped memory, so that the plan may be rebound and run again at
-// once while the labeller reads the copy later.
-struct SnapFrame {
-  const sf_signal_pixel *src;
-  const uint32_t *count;
-  uint32_t capacity, pad;
-  uint64_t dst_offset;
-};
-
-__global__ __launch_bounds__(256) void snapshot_lists(
-    const SnapFrame *frames, uint32_t n_frames, sf_signal_pixel *dst,
-    uint32_t *counts, uint32_t *host_counts) {
-  for (uint32_t k = blockIdx.y; k < n_frames; k += gridDim.y) {
-    const SnapFrame f = frames[k];
-    const uint32_t c = *f.count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      counts[k] = c;
-      host_counts[k] = c;
+    carry = 0;
+  __syncthreads();
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  for (uint32_t k0 = 0; k0 < n_frames; k0 += 256) {
+    const uint32_t k = k0 + threadIdx.x;
+    uint32_t c = 0;
+    if (k < n_frames) {
+      const uint32_t found = *frames[k].count;
+      host_counts[k] = found;
+      c = found <= frames[k].capacity ? found : 0u;
     }
-    const uint32_t take = c <= f.capacity ? c : 0u;
-    const uint4 *from = reinterpret_cast<const uint4 *>(f.src);
-    uint4 *to = reinterpret_cast<uint4 *>(dst + f.dst_offset);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < take;
-         i += gridDim.x * blockDim.x)
-      to[i] = from[i];
+    const uint32_t incl = wave_inclusive(c);
+    if (lane == 63)
+      part[wave] = incl;
+    __syncthreads();
+    uint32_t before = carry;
+    for (uint32_t w = 0; w < wave; w++)
+      before += part[w];
+    if (k < n_frames)
+      first[k] = before + incl - c;
+    __syncthreads();
+    if (threadIdx.x == 255)
+      carry = before + incl;
+    __syncthreads();
   }
+  if (threadIdx.x == 0)
+    first[n_frames] = carry;
+  __threadfence_system();
 }
 
 // ---------------------------------------------------------------------------
@@ -1772,11 +1818,12 @@ struct sf_labeller {
   sf_options options{};
   int device = 0;
   hipStream_t stream = nullptr;
-  // add_plan's chunks: the plan's lists are copied on the plan's stream to
-  // one of two snapshot sides (snapshot_lists), and a worker thread labels
-  // the chunk once the copy is done, so add_plan returns without waiting
-  // for the plan or the labelling. A chunk is handed over while the worker
-  // may still hold the one before it (jobs_given - jobs_done <= 2).
+  // add_plan's chunks: the plan's lists are gathered on the snapshot stream
+  // into one of two sides' node sets (chunk_offsets, gather_chunk), and a
+  // worker thread labels the chunk once the gather is done, so add_plan
+  // returns without waiting for the plan or the labelling. A chunk is
+  // handed over while the worker may still hold the one before it
+  // (jobs_given - jobs_done <= 2).
   struct Job {
     std::vector<sf::ChunkDesc> frames;
     int side = 0;
@@ -1789,10 +1836,40 @@ struct sf_labeller {
   Job job;
   int deferred_status = 0; // the first failure of a handed-over chunk
   std::string deferred_error;
-  sf::DevArray<sf_signal_pixel> snap_list[2];
-  sf::DevArray<sf::SnapFrame> snap_table[2];
-  sf::DevArray<uint32_t> snap_counts[2];
-  sf::SnapFrame *h_snap_table[2] = {nullptr, nullptr}; // pinned staging
+  // A chunk's own nodes (sf::ChunkNodes) and its frames' starts among them:
+  // two sets for add_plan's sides, gathered on the snapshot stream, and one
+  // for the chunks that run_chunk gathers itself.
+  struct NodeSet {
+    sf::DevArray<uint32_t> index, value, slot, tile_heads, first;
+    sf::DevArray<uint64_t> heads;
+    hipError_t ensure(size_t nodes, size_t frames) {
+      const size_t n = std::max<size_t>(nodes, 1);
+      hipError_t e;
+      if ((e = index.ensure(n)) != hipSuccess ||
+          (e = value.ensure(n)) != hipSuccess ||
+          (e = slot.ensure(n)) != hipSuccess ||
+          (e = heads.ensure(n / 64 + 1)) != hipSuccess ||
+          (e = tile_heads.ensure(n / sf::kTile + 1)) != hipSuccess)
+        return e;
+      return first.ensure(frames + 1);
+    }
+    sf::ChunkNodes view() const {
+      return {index.ptr, value.ptr, slot.ptr, heads.ptr, tile_heads.ptr};
+    }
+    void release() {
+      index.release();
+      value.release();
+      slot.release();
+      tile_heads.release();
+      first.release();
+      heads.release();
+    }
+  };
+  NodeSet snap_nodes[2], own_nodes;
+  uint32_t *h_first = nullptr; // pinned staging of own_nodes.first
+  size_t h_first_cap = 0;
+  sf::DevArray<sf::ChunkDesc> snap_table[2];
+  sf::ChunkDesc *h_snap_table[2] = {nullptr, nullptr}; // pinned staging
   uint32_t *h_snap_counts[2] = {nullptr, nullptr};     // host-mapped
   uint32_t *hd_snap_counts[2] = {nullptr, nullptr};
   size_t snap_frames_cap[2] = {0, 0};
@@ -1822,10 +1899,7 @@ struct sf_labeller {
   sf::ChunkDesc *h_chunk = nullptr; // pinned staging of the chunk table
   uint32_t h_chunk_cap = 0;
   // per node
-  sf::DevArray<int32_t> node_frame;
-  sf::DevArray<uint32_t> node_index, node_value, node_slot, run_of;
-  sf::DevArray<uint64_t> head_bits;  // run heads, a bit per node
-  sf::DevArray<uint32_t> head_count; // run heads per tile of nodes
+  sf::DevArray<uint32_t> run_of;
   // per run (at most one per node)
   sf::DevArray<uint32_t> run_head, run_idx, run_last, run_slot, parent, root,
       last_slot, pos_of;
@@ -1940,10 +2014,13 @@ hipError_t poll_stream(sf_labeller *l) {
 
 // Label one chunk whose frame lists are device-resident. `frames` holds
 // list/count/capacity/number for each frame; offsets are filled here from
-// the host counts.
+// the host counts. `gathered`: the chunk's own nodes, already gathered from
+// the lists (add_plan's side); otherwise they are gathered here, on
+// `producer` when given.
 int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
               const std::vector<uint32_t> &counts, bool closing,
-              hipStream_t producer) {
+              hipStream_t producer,
+              const sf_labeller::NodeSet *gathered = nullptr) {
   hipStream_t s = l->stream;
   // the frontier and the scratch of the previous chunk must be done with
   LAB_CHECK(poll_stream(l), "waiting for the previous chunk");
@@ -2012,22 +2089,30 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
                         ? counts[n_frames - 1]
                         : 0;
   if (n == 0) {
+    // nothing carried and every frame empty (or over its capacity): no
+    // signal pixel to count
     *l->h_scalars = 0;
-    if (n_frames) {
-      LAB_CHECK(l->chunk.ensure(n_frames), "chunk table");
-      LAB_CHECK(hipMemcpyAsync(l->chunk.ptr, l->h_chunk,
-                               n_frames * sizeof(sf::ChunkDesc),
-                               hipMemcpyHostToDevice, s),
-                "chunk table");
-      sf::add_signal_count<<<1, 64, 0, s>>>(l->chunk.ptr, n_frames,
-                                            l->d_counts);
-    }
     return SF_OK;
   }
-  LAB_CHECK(l->node_frame.ensure(n), "alloc");
-  LAB_CHECK(l->node_index.ensure(n), "alloc");
-  LAB_CHECK(l->node_value.ensure(n), "alloc");
-  LAB_CHECK(l->node_slot.ensure(n), "alloc");
+  const uint32_t n_chunk = n - l->n_carry;
+  const uint32_t chunk_tiles = (n_chunk + sf::kTile - 1) / sf::kTile;
+  const uint32_t carry_tiles = (l->n_carry + sf::kTile - 1) / sf::kTile;
+  if (!gathered) {
+    LAB_CHECK(l->own_nodes.ensure(n_chunk, n_frames), "alloc");
+    if (n_frames + 1 > l->h_first_cap) {
+      if (l->h_first)
+        (void)hipHostFree(l->h_first);
+      l->h_first = nullptr;
+      l->h_first_cap = 0;
+      LAB_CHECK(hipHostMalloc(&l->h_first, (n_frames + 1) * sizeof(uint32_t)),
+                "chunk staging");
+      l->h_first_cap = n_frames + 1;
+    }
+    for (uint32_t k = 0; k < n_frames; k++)
+      l->h_first[k] = frames[k].offset - l->n_carry;
+    l->h_first[n_frames] = n_chunk;
+  }
+  const sf_labeller::NodeSet &nodes = gathered ? *gathered : l->own_nodes;
   LAB_CHECK(l->run_of.ensure(n), "alloc");
   LAB_CHECK(l->run_head.ensure(n), "alloc");
   LAB_CHECK(l->run_idx.ensure(n), "alloc");
@@ -2050,9 +2135,6 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   LAB_CHECK(l->results.ensure(n), "alloc");
   LAB_CHECK(l->spot_pack.ensure(n), "alloc");
   LAB_CHECK(l->spot_rank.ensure(n), "alloc");
-  const uint32_t tiles = (n + sf::kTile - 1) / sf::kTile;
-  LAB_CHECK(l->head_bits.ensure(n / 64 + 1), "alloc");
-  LAB_CHECK(l->head_count.ensure(tiles), "alloc");
   const int in = l->carry_side, outside = 1 - l->carry_side;
   LAB_CHECK(l->carry_frame[outside].ensure(n), "alloc");
   LAB_CHECK(l->carry_index[outside].ensure(n), "alloc");
@@ -2072,30 +2154,40 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
                              n_frames * sizeof(sf::ChunkDesc),
                              hipMemcpyHostToDevice, s),
               "chunk table");
-  // The gather reads the producer's lists: run it on the producer's stream so
-  // that the producer cannot overwrite them first, and have this stream wait
-  // for it.
-  hipStream_t gs = producer ? producer : s;
-  if (producer) {
-    LAB_CHECK(hipEventRecord(l->done, s), "event");
-    LAB_CHECK(hipStreamWaitEvent(gs, l->done, 0), "event wait");
+  if (!gathered && n_chunk) {
+    LAB_CHECK(hipMemcpyAsync(l->own_nodes.first.ptr, l->h_first,
+                             (n_frames + 1) * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, s),
+              "chunk table");
+    // The gather reads the producer's lists: run it on the producer's
+    // stream so that the producer cannot overwrite them first, and have this
+    // stream wait for it.
+    hipStream_t gs = producer ? producer : s;
+    if (producer) {
+      LAB_CHECK(hipEventRecord(l->done, s), "event");
+      LAB_CHECK(hipStreamWaitEvent(gs, l->done, 0), "event wait");
+    }
+    sf::gather_chunk<<<std::min<uint32_t>(chunk_tiles, 4096), 256, 0, gs>>>(
+        a, l->chunk.ptr, n_frames, l->own_nodes.first.ptr,
+        l->own_nodes.view());
+    LAB_CHECK(hipGetLastError(), "gather");
+    if (producer) {
+      LAB_CHECK(hipEventRecord(l->done, gs), "event");
+      LAB_CHECK(hipStreamWaitEvent(s, l->done, 0), "event wait");
+    }
   }
   const uint32_t grid = grid_for(n);
-  sf::gather_nodes<<<tiles, sf::kTileThreads, 0, gs>>>(
-      a, l->chunk.ptr, l->carry_frame[in].ptr, l->carry_index[in].ptr,
-      l->carry_value[in].ptr, l->node_frame.ptr, l->node_index.ptr,
-      l->node_value.ptr, l->node_slot.ptr, l->head_bits.ptr,
-      l->head_count.ptr, l->d_counts);
-  LAB_CHECK(hipGetLastError(), "gather");
-  if (producer) {
-    LAB_CHECK(hipEventRecord(l->done, gs), "event");
-    LAB_CHECK(hipStreamWaitEvent(s, l->done, 0), "event wait");
-  }
-  sf::number_runs<<<tiles, sf::kTileThreads, 0, s>>>(
-      a, l->head_bits.ptr, l->head_count.ptr, l->node_index.ptr,
-      l->node_slot.ptr, l->carry_root[in].ptr, l->run_of.ptr,
-      l->run_head.ptr, l->run_idx.ptr, l->run_last.ptr, l->run_slot.ptr,
-      l->parent.ptr, l->last_slot.ptr, n_runs, l->hd_scalars + 1);
+  sf::number_runs<<<chunk_tiles + carry_tiles, sf::kTileThreads, 0, s>>>(
+      a, nodes.index.ptr, nodes.slot.ptr, nodes.heads.ptr,
+      nodes.tile_heads.ptr, chunk_tiles, l->carry_index[in].ptr,
+      l->carry_root[in].ptr, l->run_of.ptr, l->run_head.ptr, l->run_idx.ptr,
+      l->run_last.ptr, l->run_slot.ptr, l->parent.ptr, l->last_slot.ptr,
+      n_runs, l->hd_scalars + 1, l->d_counts);
+  // where the later kernels read each node's frame, index and value
+  const sf::NodeSrc node_src{l->carry_frame[in].ptr, l->carry_index[in].ptr,
+                             l->carry_value[in].ptr, nodes.index.ptr,
+                             nodes.value.ptr,        nodes.slot.ptr,
+                             l->chunk.ptr};
   // the host needs the run count for the sort; the kernels up to the keys
   // are queued before it waits
   LAB_CHECK(hipEventRecord(l->runs_ready, s), "event");
@@ -2230,8 +2322,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   uint32_t *mem_value = mem_index + n;
   sf::gather_members<<<grid, 256, 0, s>>>(
       a, d_closed_runs, l->run_of.ptr, l->pos_of.ptr, l->run_head.ptr,
-      l->run_rank.ptr, l->node_frame.ptr, l->node_index.ptr,
-      l->node_value.ptr, mem_frame, mem_index, mem_value);
+      l->run_rank.ptr, node_src, mem_frame, mem_index, mem_value);
   // the centroid pass: a wave per component for what reads the members,
   // then a lane per component for the rest; each loops over components on a
   // bounded grid
@@ -2254,8 +2345,8 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   sf::write_pixels<<<grid, 256, 0, s>>>(
       a, l->results.ptr, d_comps, d_closed_runs, l->run_of.ptr,
       l->pos_of.ptr, l->run_head.ptr, l->run_pack.ptr, l->run_rank.ptr,
-      l->comp_mem.ptr, l->spot_rank.ptr, l->node_frame.ptr,
-      l->node_index.ptr, l->node_value.ptr, l->d_totals, spot_seg->ptr,
+      l->comp_mem.ptr, l->spot_rank.ptr, node_src, l->d_totals,
+      spot_seg->ptr,
       spot_seg->base, pixel_seg->ptr, pixel_seg->base);
   sf::close_chunk<<<1, 256, 0, s>>>(l->tallies.ptr, finish_grid,
                                     d_comps, l->spot_rank.ptr, l->results.ptr,
@@ -2267,8 +2358,7 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
             "ranking the frontier");
   sf::carry_out<<<grid, 256, 0, s>>>(
       a, l->run_of.ptr, l->open_len.ptr, l->open_at.ptr, l->root.ptr,
-      l->run_head.ptr, n_runs, l->node_frame.ptr, l->node_index.ptr,
-      l->node_value.ptr,
+      l->run_head.ptr, n_runs, node_src,
       l->carry_frame[outside].ptr, l->carry_index[outside].ptr,
       l->carry_value[outside].ptr, l->carry_root[outside].ptr,
       l->hd_scalars);
@@ -2315,7 +2405,8 @@ int label_job(sf_labeller *l, sf_labeller::Job &job) {
   std::vector<uint32_t> counts(job.frames.size());
   for (size_t k = 0; k < counts.size(); k++)
     counts[k] = l->h_snap_counts[side][k];
-  const int status = run_chunk(l, job.frames, counts, false, nullptr);
+  const int status =
+      run_chunk(l, job.frames, counts, false, nullptr, &l->snap_nodes[side]);
   LAB_CHECK(hipEventRecord(l->snap_free[side], l->stream), "event");
   return status;
 }
@@ -2532,7 +2623,7 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
   stop_worker(l); // labels what was handed over first
   if (l->stream)
     (void)hipStreamSynchronize(l->stream);
-  if (l->snap_stream) // a snapshot may still be copying into a side
+  if (l->snap_stream) // a gather may still be writing a side
     (void)hipStreamSynchronize(l->snap_stream);
   for (int i = 0; i < 2; i++) {
     if (l->snap_ready[i]) {
@@ -2543,9 +2634,8 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
       (void)hipEventDestroy(l->snap_free[i]);
     if (l->snap_after[i])
       (void)hipEventDestroy(l->snap_after[i]);
-    l->snap_list[i].release();
+    l->snap_nodes[i].release();
     l->snap_table[i].release();
-    l->snap_counts[i].release();
     if (l->h_snap_table[i])
       (void)hipHostFree(l->h_snap_table[i]);
     if (l->h_snap_counts[i])
@@ -2557,13 +2647,10 @@ void sf_hip_labeller_destroy(sf_labeller *l) {
     l->carry_value[i].release();
     l->carry_root[i].release();
   }
-  l->node_frame.release();
-  l->node_index.release();
-  l->node_value.release();
-  l->node_slot.release();
+  l->own_nodes.release();
+  if (l->h_first)
+    (void)hipHostFree(l->h_first);
   l->run_of.release();
-  l->head_bits.release();
-  l->head_count.release();
   l->run_rows.release();
   l->run_head.release();
   l->run_idx.release();
@@ -2684,10 +2771,12 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
   uint64_t total = 0;
   for (uint32_t k = 0; k < n_frames; k++)
     total += caps[k];
-  LAB_CHECK(l->snap_list[side].ensure(std::max<uint64_t>(total, 1)),
-            "snapshot alloc");
+  if (total > 0xF0000000ull) {
+    g_label_error = "a plan whose lists hold more than 4 billion pixels";
+    return SF_BAD_ARGUMENT;
+  }
+  LAB_CHECK(l->snap_nodes[side].ensure(total, n_frames), "snapshot alloc");
   LAB_CHECK(l->snap_table[side].ensure(n_frames), "snapshot alloc");
-  LAB_CHECK(l->snap_counts[side].ensure(n_frames), "snapshot alloc");
   if (n_frames > l->snap_frames_cap[side]) {
     if (l->h_snap_table[side])
       (void)hipHostFree(l->h_snap_table[side]);
@@ -2697,7 +2786,7 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
     l->h_snap_counts[side] = nullptr;
     l->snap_frames_cap[side] = 0;
     LAB_CHECK(hipHostMalloc(&l->h_snap_table[side],
-                            n_frames * sizeof(sf::SnapFrame)),
+                            n_frames * sizeof(sf::ChunkDesc)),
               "snapshot staging");
     LAB_CHECK(hipHostMalloc(&l->h_snap_counts[side],
                             n_frames * sizeof(uint32_t), hipHostMallocMapped),
@@ -2708,41 +2797,45 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
               "snapshot counts");
     l->snap_frames_cap[side] = n_frames;
   }
-  uint64_t at = 0;
   std::vector<sf::ChunkDesc> frames(n_frames);
   for (uint32_t k = 0; k < n_frames; k++) {
-    sf::SnapFrame &f = l->h_snap_table[side][k];
-    f.src = lists[k];
+    sf::ChunkDesc &f = l->h_snap_table[side][k];
+    f = sf::ChunkDesc{};
+    f.list = lists[k];
     f.count = count_ptrs[k];
     f.capacity = caps[k];
-    f.pad = 0;
-    f.dst_offset = at;
-    frames[k].list = l->snap_list[side].ptr + at;
-    frames[k].count = l->snap_counts[side].ptr + k;
     frames[k].capacity = caps[k];
     frames[k].number = (int32_t)(first_frame + (int64_t)k);
     frames[k].linked =
         k > 0 || (l->started && first_frame == l->last_frame + 1);
-    at += caps[k];
   }
   // on the snapshot stream: after the plan's run that wrote the lists (what
-  // is enqueued on the plan's stream now) and after the side's last reader;
-  // the plan's next write of its lists waits for the snapshot, and nothing
-  // else the plan's stream does later waits for it
+  // is enqueued on the plan's stream now) and after the side's last reader,
+  // the chunk's nodes gathered from the lists into the side's node set; the
+  // plan's next write of its lists waits for the gather, and nothing else
+  // the plan's stream does later waits for it
   LAB_CHECK(hipEventRecord(l->snap_after[side], producer), "event");
   LAB_CHECK(hipStreamWaitEvent(l->snap_stream, l->snap_after[side], 0),
             "event wait");
   LAB_CHECK(hipStreamWaitEvent(l->snap_stream, l->snap_free[side], 0),
             "event wait");
   LAB_CHECK(hipMemcpyAsync(l->snap_table[side].ptr, l->h_snap_table[side],
-                           n_frames * sizeof(sf::SnapFrame),
+                           n_frames * sizeof(sf::ChunkDesc),
                            hipMemcpyHostToDevice, l->snap_stream),
             "snapshot table");
-  sf::snapshot_lists<<<dim3(32, std::min<uint32_t>(n_frames, 65535)), 256, 0,
-                       l->snap_stream>>>(l->snap_table[side].ptr, n_frames,
-                                         l->snap_list[side].ptr,
-                                         l->snap_counts[side].ptr,
-                                         l->hd_snap_counts[side]);
+  sf::chunk_offsets<<<1, 256, 0, l->snap_stream>>>(
+      l->snap_table[side].ptr, n_frames, l->snap_nodes[side].first.ptr,
+      l->hd_snap_counts[side]);
+  sf::LabelArgs ga{};
+  ga.width = (uint32_t)l->width;
+  ga.pixels = (uint32_t)(l->height * l->width);
+  ga.inv_width = 1.0 / (double)l->width;
+  const uint64_t tiles = (total + sf::kTile - 1) / sf::kTile;
+  sf::gather_chunk<<<(uint32_t)std::max<uint64_t>(
+                         1, std::min<uint64_t>(tiles, 4096)),
+                     256, 0, l->snap_stream>>>(
+      ga, l->snap_table[side].ptr, n_frames, l->snap_nodes[side].first.ptr,
+      l->snap_nodes[side].view());
   LAB_CHECK(hipGetLastError(), "snapshot");
   LAB_CHECK(hipEventRecord(l->snap_ready[side], l->snap_stream), "event");
   status = sf::plan_add_reader(plan, l->snap_stream);
