@@ -966,15 +966,18 @@ __device__ __forceinline__ void for_members(uint32_t s0, uint32_t n,
   }
 }
 
-// The closed members' frame, index and value in emission order, gathered
-// once, one thread per node: node g of run r is member
-// (first member of r's sorted place) + (g - r's head).
+// The closed members' frame, index and value in emission order, and each
+// member's component, gathered once, one thread per node: node g of run r,
+// at sorted place i, is member (i's first member) + (g - r's head) of
+// component (i's scan rank, high half) + i's start bit - 1.
 __global__ void gather_members(LabelArgs a, const uint32_t *d_closed_runs,
                                const uint32_t *run_of, const uint32_t *pos_of,
                                const uint32_t *run_head,
+                               const unsigned long long *pack,
                                const unsigned long long *rank,
                                NodeSrc nodes, int32_t *mem_frame,
-                               uint32_t *mem_index, uint32_t *mem_value) {
+                               uint32_t *mem_index, uint32_t *mem_value,
+                               uint32_t *mem_comp) {
   const uint32_t closed = *d_closed_runs;
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
        g += gridDim.x * blockDim.x) {
@@ -982,8 +985,10 @@ __global__ void gather_members(LabelArgs a, const uint32_t *d_closed_runs,
     const uint32_t i = pos_of[r];
     if (i >= closed)
       continue;
-    const uint32_t at = (uint32_t)rank[i] + (g - run_head[r]);
+    const unsigned long long q = rank[i];
+    const uint32_t at = (uint32_t)q + (g - run_head[r]);
     node_at(a, nodes, g, mem_frame[at], mem_index[at], mem_value[at]);
+    mem_comp[at] = (uint32_t)(q >> 32) + (uint32_t)(pack[i] >> 32) - 1;
   }
 }
 
@@ -1345,24 +1350,20 @@ __global__ __launch_bounds__(256) void finish_components(
   store_tallies(tally, tallies);
 }
 
-// Appends the kept spots and their pixels (a pixel a thread): node g of run r, at
-// sorted place i, belongs to component c = (i's scan rank, high half) + i's
-// start bit - 1, and is member (i's first member) + (g - r's head).
-__global__ void write_pixels(LabelArgs a, const SpotOut *res,
-                             const uint32_t *n_comps,
-                             const uint32_t *d_closed_runs,
-                             const uint32_t *run_of, const uint32_t *pos_of,
-                             const uint32_t *run_head,
-                             const unsigned long long *pack,
-                             const unsigned long long *run_rank,
+// Appends the kept spots and their pixels (a member a thread): member m of
+// component c is pixel row (c's first pixel row) + (m - c's first member).
+__global__ void write_pixels(const SpotOut *res, const uint32_t *n_comps,
+                             const uint32_t *n_members,
                              const uint32_t *comp_mem,
                              const unsigned long long *spot_rank,
-                             NodeSrc nodes,
+                             const int32_t *mem_frame,
+                             const uint32_t *mem_index,
+                             const uint32_t *mem_value,
+                             const uint32_t *mem_comp,
                              const unsigned long long *totals,
                              sf_spot *spots, unsigned long long spots_from,
                              sf_pixel *pixels,
                              unsigned long long pixels_from) {
-  const uint32_t closed = *d_closed_runs;
   // rows are numbered over the whole table; `spots` and `pixels` hold it
   // from row spots_from and pixels_from on
   const unsigned long long spot_base = totals[0], pixel_base = totals[1];
@@ -1378,20 +1379,17 @@ __global__ void write_pixels(LabelArgs a, const SpotOut *res,
     sp.first = pixel_base + (uint32_t)spot_rank[c];
     spot_rows[spot_rank[c] >> 32] = sp;
   }
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
-       g += gridDim.x * blockDim.x) {
-    const uint32_t r = run_of[g];
-    const uint32_t i = pos_of[r];
-    if (i >= closed)
-      continue;
-    const unsigned long long p = pack[i], q = run_rank[i];
-    const uint32_t c = (uint32_t)(q >> 32) + (uint32_t)(p >> 32) - 1;
+  const uint32_t members = *n_members;
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < members;
+       m += gridDim.x * blockDim.x) {
+    const uint32_t c = mem_comp[m];
     if (!res[c].accepted)
       continue;
-    const uint32_t at = (uint32_t)q + (g - run_head[r]);
     sf_pixel px;
-    node_at(a, nodes, g, px.frame, px.index, px.value);
-    pixel_rows[(uint32_t)spot_rank[c] + (at - comp_mem[c])] = px;
+    px.frame = mem_frame[m];
+    px.index = mem_index[m];
+    px.value = mem_value[m];
+    pixel_rows[(uint32_t)spot_rank[c] + (m - comp_mem[c])] = px;
   }
 }
 
@@ -2320,9 +2318,11 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
   int32_t *mem_frame = reinterpret_cast<int32_t *>(l->values.ptr);
   uint32_t *mem_index = reinterpret_cast<uint32_t *>(l->keys.ptr);
   uint32_t *mem_value = mem_index + n;
+  uint32_t *mem_comp = keys32_sorted; // the sort's output, dead now
   sf::gather_members<<<grid, 256, 0, s>>>(
       a, d_closed_runs, l->run_of.ptr, l->pos_of.ptr, l->run_head.ptr,
-      l->run_rank.ptr, node_src, mem_frame, mem_index, mem_value);
+      l->run_pack.ptr, l->run_rank.ptr, node_src, mem_frame, mem_index,
+      mem_value, mem_comp);
   // the centroid pass: a wave per component for what reads the members,
   // then a lane per component for the rest; each loops over components on a
   // bounded grid
@@ -2343,11 +2343,10 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
                                     rocprim::plus<unsigned long long>(), s),
             "ranking spots and their pixels");
   sf::write_pixels<<<grid, 256, 0, s>>>(
-      a, l->results.ptr, d_comps, d_closed_runs, l->run_of.ptr,
-      l->pos_of.ptr, l->run_head.ptr, l->run_pack.ptr, l->run_rank.ptr,
-      l->comp_mem.ptr, l->spot_rank.ptr, node_src, l->d_totals,
-      spot_seg->ptr,
-      spot_seg->base, pixel_seg->ptr, pixel_seg->base);
+      l->results.ptr, d_comps, d_closed_mem, l->comp_mem.ptr,
+      l->spot_rank.ptr, mem_frame, mem_index, mem_value, mem_comp,
+      l->d_totals, spot_seg->ptr, spot_seg->base, pixel_seg->ptr,
+      pixel_seg->base);
   sf::close_chunk<<<1, 256, 0, s>>>(l->tallies.ptr, finish_grid,
                                     d_comps, l->spot_rank.ptr, l->results.ptr,
                                     l->d_counts, l->d_totals, l->hd_totals);
