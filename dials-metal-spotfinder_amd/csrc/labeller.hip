@@ -1435,27 +1435,32 @@ __global__ __launch_bounds__(256) void close_chunk(
 
 // The open runs' pixels, in node order, become the next frontier; each
 // points at its component's first pixel -- the first pixel of its root run,
-// which is open with it. One thread per node.
-__global__ void carry_out(LabelArgs a, const uint32_t *run_of,
-                          const uint32_t *open_len, const uint32_t *open_at,
-                          const uint32_t *root, const uint32_t *run_head,
-                          const uint32_t *n_runs, NodeSrc nodes,
-                          int32_t *carry_frame,
+// which is open with it. One thread per run: the open runs (few) copy their
+// nodes, the rest return.
+__global__ void carry_out(LabelArgs a, const uint32_t *open_len,
+                          const uint32_t *open_at, const uint32_t *root,
+                          const uint32_t *run_head, const uint32_t *n_runs,
+                          NodeSrc nodes, int32_t *carry_frame,
                           uint32_t *carry_index, uint32_t *carry_value,
                           uint32_t *carry_root, uint32_t *n_carry_out) {
+  const uint32_t runs = *n_runs;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const uint32_t last = *n_runs - 1;
+    const uint32_t last = runs - 1;
     *n_carry_out = open_at[last] + open_len[last]; // host-mapped
     __threadfence_system();
   }
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.n_nodes;
-       g += gridDim.x * blockDim.x) {
-    const uint32_t r = run_of[g];
-    if (!open_len[r])
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < runs;
+       r += gridDim.x * blockDim.x) {
+    const uint32_t len = open_len[r];
+    if (!len)
       continue;
-    const uint32_t at = open_at[r] + (g - run_head[r]);
-    node_at(a, nodes, g, carry_frame[at], carry_index[at], carry_value[at]);
-    carry_root[at] = open_at[root[r]];
+    const uint32_t at = open_at[r], head = run_head[r];
+    const uint32_t to_root = open_at[root[r]];
+    for (uint32_t k = 0; k < len; k++) {
+      node_at(a, nodes, head + k, carry_frame[at + k], carry_index[at + k],
+              carry_value[at + k]);
+      carry_root[at + k] = to_root;
+    }
   }
 }
 
@@ -2355,12 +2360,11 @@ int run_chunk(sf_labeller *l, std::vector<sf::ChunkDesc> &frames,
                                     l->open_at.ptr, 0u, (size_t)runs,
                                     rocprim::plus<uint32_t>(), s),
             "ranking the frontier");
-  sf::carry_out<<<grid, 256, 0, s>>>(
-      a, l->run_of.ptr, l->open_len.ptr, l->open_at.ptr, l->root.ptr,
-      l->run_head.ptr, n_runs, node_src,
-      l->carry_frame[outside].ptr, l->carry_index[outside].ptr,
-      l->carry_value[outside].ptr, l->carry_root[outside].ptr,
-      l->hd_scalars);
+  sf::carry_out<<<rgrid, 256, 0, s>>>(
+      a, l->open_len.ptr, l->open_at.ptr, l->root.ptr, l->run_head.ptr,
+      n_runs, node_src, l->carry_frame[outside].ptr,
+      l->carry_index[outside].ptr, l->carry_value[outside].ptr,
+      l->carry_root[outside].ptr, l->hd_scalars);
   LAB_CHECK(hipGetLastError(), "finalise kernels");
   l->carry_side = outside;
   return SF_OK;
