@@ -938,9 +938,10 @@ def main() -> int:
     # erosion, Poisson pass and ordering) while it scouts the next chunk,
     # bound to the other: two captured graphs, replayed alternately, with
     # the next chunk's frame pointers bound before each. The finished chunk
-    # then goes to the labeller (its gather runs on that plan's stream, so
-    # that the plan's next step waits for it), whose kernels overlap the
-    # next step's. Frame numbers keep ascending as the series is replayed.
+    # then goes to the labeller (which gathers its lists on a stream of its
+    # own; the plan's next write of them waits for that), whose kernels
+    # overlap the next step's. Frame numbers keep ascending as the series
+    # is replayed.
     plans = [plan, sf.Plan(2, [(HEIGHT, WIDTH)] * chunk, band_rows=args.band)]
     next_frame = [0]  # the first frame number of the next chunk to label
 

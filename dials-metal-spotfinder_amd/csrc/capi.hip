@@ -36,6 +36,8 @@ hipError_t launch_sparse32_threshold(const FrameDesc *, int, uint32_t,
 hipError_t launch_sparse_threshold(const FrameDesc *, int, uint32_t,
                                    hipStream_t, hipEvent_t *);
 hipError_t launch_sparse_order(const FrameDesc *, int, uint32_t, hipStream_t);
+hipError_t launch_sparse_places(const FrameDesc *, int, uint32_t, hipStream_t);
+hipError_t launch_sparse_copy(const FrameDesc *, int, hipStream_t);
 hipError_t launch_sparse(const FrameDesc *, int, uint32_t, uint32_t,
                          hipStream_t);
 hipError_t launch_sparse32_pipeline(const FrameDesc *, int, uint32_t,
@@ -171,6 +173,9 @@ struct sf_plan {
   // ordering to sf_hip_plan_order (order_pending until then); whether the
   // captured step was captured so
   bool defer_order = false, order_pending = false, pipe_deferred = false;
+  // a reader of a pending ordering took its places (plan_take_places): only
+  // the copy into the lists is left, which sf_hip_plan_order does
+  bool copy_pending = false;
   // a reader of the lists on another stream (the labeller's snapshot): the
   // plan's next write of its lists waits for it
   hipEvent_t read_done = nullptr;
@@ -627,6 +632,7 @@ int sf_hip_plan_run(sf_plan *p, const void *const *frames) {
     return up;
   p->scouted = false;
   p->order_pending = false; // a run orders its own lists
+  p->copy_pending = false;
   const int waited = wait_reader(p, p->stream);
   if (waited != SF_OK)
     return waited;
@@ -677,6 +683,7 @@ int sf_hip_plan_replay(sf_plan *p) {
   if (!p || !p->exec)
     return bad("sf_hip_plan_replay: nothing captured");
   p->order_pending = false; // a replayed run orders its own lists
+  p->copy_pending = false;
   const int waited = wait_reader(p, p->stream);
   if (waited != SF_OK)
     return waited;
@@ -801,9 +808,12 @@ int pipeline_after(sf_plan *cur, sf_plan *next, hipStream_t s) {
   if (cur) {
     cur->scouted = false;
     cur->order_pending = cur->defer_order;
+    cur->copy_pending = false;
   }
-  if (next)
+  if (next) {
     next->scouted = true;
+    next->copy_pending = false; // the scout rewrote what the copy reads
+  }
   return SF_OK;
 }
 
@@ -835,6 +845,11 @@ int sf_hip_plan_pipeline(sf_plan *cur, sf_plan *next) {
   if (status != SF_OK)
     return status;
   if ((status = pipeline_wait_other(next, s)) != SF_OK)
+    return status;
+  // the next plan's clearing and scout rewrite the tables that a reader
+  // which took its last ordering's places may still use (plan_take_places);
+  // a reader of its lists waits only for their next write, by the ordering
+  if (next && next->copy_pending && (status = wait_reader(next, s)) != SF_OK)
     return status;
   const bool order = !(cur && cur->defer_order);
   if (order && (status = wait_reader(cur, s)) != SF_OK)
@@ -887,6 +902,8 @@ int sf_hip_plan_replay_pipeline(sf_plan *cur, sf_plan *next) {
                "the other ordering setting; capture it again");
   if ((status = pipeline_wait_other(next, s)) != SF_OK)
     return status;
+  if (next && next->copy_pending && (status = wait_reader(next, s)) != SF_OK)
+    return status;
   if (!owner->pipe_deferred && (status = wait_reader(cur, s)) != SF_OK)
     return status;
   SF_CHECK(hipGraphLaunch(owner->pipe_exec, s), "hipGraphLaunch");
@@ -905,18 +922,23 @@ int sf_hip_plan_set_deferred_order(sf_plan *p, int deferred) {
 int sf_hip_plan_order(sf_plan *p) {
   if (!p)
     return bad("sf_hip_plan_order: no plan");
-  if (!p->order_pending)
+  if (!p->order_pending && !p->copy_pending)
     return bad("sf_hip_plan_order: no pipelined step left this plan's "
                "ordering pending");
   // after the step (enqueued on this plan's stream), and after the last
-  // reader of the lists it rewrites
+  // reader of the lists it rewrites (or of the places it copies by)
   const int waited = wait_reader(p, p->stream);
   if (waited != SF_OK)
     return waited;
-  SF_CHECK(sf::launch_sparse_order(p->d_desc, p->n, p->max_cell_rows,
-                                   p->stream),
-           "launching the ordering");
+  if (p->order_pending)
+    SF_CHECK(sf::launch_sparse_order(p->d_desc, p->n, p->max_cell_rows,
+                                     p->stream),
+             "launching the ordering");
+  else
+    SF_CHECK(sf::launch_sparse_copy(p->d_desc, p->n, p->stream),
+             "launching the copy into the lists");
   p->order_pending = false;
+  p->copy_pending = false;
   return SF_OK;
 }
 
@@ -1194,6 +1216,36 @@ int plan_lists(sf_plan *p, std::vector<const sf_signal_pixel *> &lists,
     capacity[f] = p->capacity[f];
   }
   *stream = p->stream;
+  return SF_OK;
+}
+
+// For a reader of a plan whose last pipelined step left its ordering
+// pending: the ordering's places (row prefixes, each survivor's place in
+// the list, each frame's count) enqueued on `reader`, and per frame where
+// the survivors sit (staging) and their places (dstof), so that the reader
+// takes the survivors in list order without the lists being written; they
+// are written only if sf_hip_plan_order is called. `taken` is false, and
+// nothing is enqueued, when no ordering is pending (the lists are ordered).
+int plan_take_places(sf_plan *p, hipStream_t reader,
+                     std::vector<const sf_signal_pixel *> &staging,
+                     std::vector<const uint32_t *> &places, bool &taken) {
+  taken = false;
+  if (!p)
+    return SF_BAD_ARGUMENT;
+  if (!p->order_pending)
+    return SF_OK;
+  SF_CHECK(sf::launch_sparse_places(p->d_desc, p->n, p->max_cell_rows,
+                                    reader),
+           "launching the places");
+  staging.resize(p->n);
+  places.resize(p->n);
+  for (int f = 0; f < p->n; f++) {
+    staging[f] = p->desc[f].staging;
+    places[f] = p->desc[f].dstof;
+  }
+  p->order_pending = false;
+  p->copy_pending = true;
+  taken = true;
   return SF_OK;
 }
 

@@ -1157,6 +1157,38 @@ hipError_t launch_sparse_order(const FrameDesc *frames, int n_frames,
   return hipGetLastError();
 }
 
+// The ordering in two parts, for a reader that takes the survivors from
+// staging by their places (dstof) itself: the places (row prefix and tags;
+// each frame's count to out_count too), then, should the lists be wanted
+// later, the copy into them.
+namespace {
+__global__ void found_kernel(const FrameDesc *frames) {
+  const FrameDesc &fd = frames[blockIdx.x];
+  if (threadIdx.x == 0)
+    *fd.out_count = *fd.counter;
+}
+} // namespace
+
+hipError_t launch_sparse_places(const FrameDesc *frames, int n_frames,
+                                uint32_t max_cell_rows, hipStream_t stream) {
+  if (n_frames <= 0)
+    return hipSuccess;
+  rowscan_kernel<<<n_frames, 1024, 0, stream>>>(frames);
+  if (max_cell_rows)
+    tag_kernel<<<dim3((max_cell_rows + 3) / 4, n_frames), 256, 0, stream>>>(
+        frames);
+  found_kernel<<<n_frames, 64, 0, stream>>>(frames);
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse_copy(const FrameDesc *frames, int n_frames,
+                              hipStream_t stream) {
+  if (n_frames <= 0)
+    return hipSuccess;
+  place_kernel<<<dim3(64, n_frames), 256, 0, stream>>>(frames);
+  return hipGetLastError();
+}
+
 // Blocks of a mixed launch: on each XCD its eighth of both kinds.
 uint32_t mixed_grid(uint32_t scout_blocks, uint32_t sparse_blocks) {
   return 8 * ((scout_blocks + 7) / 8 + (sparse_blocks + 7) / 8);

@@ -46,7 +46,9 @@ namespace sf {
 
 
 struct ChunkDesc {
-  const sf_signal_pixel *list; // device list of this frame
+  const sf_signal_pixel *list; // device list of this frame (or its
+                               // survivors in staging order, with `place`)
+  const uint32_t *place;       // null, or each survivor's place in the list
   const uint32_t *count;       // device count (may exceed capacity)
   uint32_t capacity;
   uint32_t offset;             // first node of this frame
@@ -258,6 +260,84 @@ __global__ __launch_bounds__(256) void gather_chunk(LabelArgs a,
       const uint64_t mask = ballot(head);
       if (lane == 0)
         out.heads[j >> 6] = mask; // the wave's 64 nodes start at j
+      mine += head;
+    }
+    if (mine)
+      atomicAdd(&listed, mine);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      out.tile_heads[t] = listed;
+    __syncthreads();
+  }
+}
+
+// The chunk's nodes from survivors in staging order with their places in
+// the list (frame k's survivor e is node first[k] + place[e]); a list over
+// its capacity counts 0. The run heads then come from heads_chunk. Grid
+// (blocks, frames in turn).
+__global__ __launch_bounds__(256) void scatter_chunk(const ChunkDesc *frames,
+                                                     uint32_t n_frames,
+                                                     const uint32_t *first,
+                                                     ChunkNodes out) {
+  for (uint32_t k = blockIdx.y; k < n_frames; k += gridDim.y) {
+    const ChunkDesc f = frames[k];
+    const uint32_t c = *f.count;
+    if (c > f.capacity)
+      continue;
+    const uint32_t base = first[k];
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < c;
+         e += gridDim.x * blockDim.x) {
+      const uint32_t j = base + f.place[e];
+      const sf_signal_pixel sp = f.list[e];
+      out.index[j] = sp.index;
+      out.value[j] = sp.value;
+      out.slot[j] = k + 1;
+    }
+  }
+}
+
+// The run heads of a chunk's nodes already in place (scatter_chunk), as
+// gather_chunk marks them: its frame's first, the first column of a row, or
+// not the index after the node before it. Blocks take tiles in turn.
+__global__ __launch_bounds__(256) void heads_chunk(LabelArgs a,
+                                                   uint32_t n_frames,
+                                                   const uint32_t *first,
+                                                   ChunkNodes out) {
+  __shared__ int tile_frame;
+  __shared__ uint32_t listed;
+  if (n_frames == 0)
+    return;
+  const uint32_t total = first[n_frames];
+  const uint32_t lane = lane_id();
+  for (uint32_t t = blockIdx.x; t < (total + kTile - 1) / kTile;
+       t += gridDim.x) {
+    const uint32_t tile = t * kTile;
+    if (threadIdx.x == 0) {
+      listed = 0;
+      int lo = 0, hi = (int)n_frames - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= tile)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      tile_frame = lo;
+    }
+    __syncthreads();
+    int k = tile_frame;
+    uint32_t mine = 0;
+    const uint32_t end = min(tile + kTile, total);
+    for (uint32_t j = tile + threadIdx.x; j < end; j += blockDim.x) {
+      while (k + 1 < (int)n_frames && first[k + 1] <= j)
+        k++;
+      const uint32_t idx = out.index[j];
+      const bool head = j == first[k] ||
+                        idx - row_of(a, idx) * a.width == 0 ||
+                        out.index[j - 1] != idx - 1;
+      const uint64_t mask = ballot(head);
+      if (lane == 0)
+        out.heads[j >> 6] = mask;
       mine += head;
     }
     if (mine)
@@ -2379,6 +2459,9 @@ int plan_lists(sf_plan *plan, std::vector<const sf_signal_pixel *> &lists,
                std::vector<const uint32_t *> &counts,
                std::vector<uint32_t> &capacity, hipStream_t *stream);
 int plan_add_reader(sf_plan *plan, hipStream_t reader);
+int plan_take_places(sf_plan *plan, hipStream_t reader,
+                     std::vector<const sf_signal_pixel *> &staging,
+                     std::vector<const uint32_t *> &places, bool &taken);
 } // namespace sf
 
 namespace {
@@ -2800,11 +2883,31 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
               "snapshot counts");
     l->snap_frames_cap[side] = n_frames;
   }
+  // on the snapshot stream: after the plan's run that wrote the lists (what
+  // is enqueued on the plan's stream now) and after the side's last reader,
+  // the chunk's nodes gathered from the lists into the side's node set; the
+  // plan's next write of its lists waits for the gather, and nothing else
+  // the plan's stream does later waits for it. A plan whose pipelined step
+  // left its ordering pending has no lists yet: its ordering's places are
+  // enqueued here instead, and the nodes are taken from its staging by
+  // them (the lists are written only if the plan is ordered later).
+  LAB_CHECK(hipEventRecord(l->snap_after[side], producer), "event");
+  LAB_CHECK(hipStreamWaitEvent(l->snap_stream, l->snap_after[side], 0),
+            "event wait");
+  LAB_CHECK(hipStreamWaitEvent(l->snap_stream, l->snap_free[side], 0),
+            "event wait");
+  std::vector<const sf_signal_pixel *> staging;
+  std::vector<const uint32_t *> places;
+  bool placed = false;
+  status = sf::plan_take_places(plan, l->snap_stream, staging, places, placed);
+  if (status != SF_OK)
+    return status;
   std::vector<sf::ChunkDesc> frames(n_frames);
   for (uint32_t k = 0; k < n_frames; k++) {
     sf::ChunkDesc &f = l->h_snap_table[side][k];
     f = sf::ChunkDesc{};
-    f.list = lists[k];
+    f.list = placed ? staging[k] : lists[k];
+    f.place = placed ? places[k] : nullptr;
     f.count = count_ptrs[k];
     f.capacity = caps[k];
     frames[k].capacity = caps[k];
@@ -2812,16 +2915,6 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
     frames[k].linked =
         k > 0 || (l->started && first_frame == l->last_frame + 1);
   }
-  // on the snapshot stream: after the plan's run that wrote the lists (what
-  // is enqueued on the plan's stream now) and after the side's last reader,
-  // the chunk's nodes gathered from the lists into the side's node set; the
-  // plan's next write of its lists waits for the gather, and nothing else
-  // the plan's stream does later waits for it
-  LAB_CHECK(hipEventRecord(l->snap_after[side], producer), "event");
-  LAB_CHECK(hipStreamWaitEvent(l->snap_stream, l->snap_after[side], 0),
-            "event wait");
-  LAB_CHECK(hipStreamWaitEvent(l->snap_stream, l->snap_free[side], 0),
-            "event wait");
   LAB_CHECK(hipMemcpyAsync(l->snap_table[side].ptr, l->h_snap_table[side],
                            n_frames * sizeof(sf::ChunkDesc),
                            hipMemcpyHostToDevice, l->snap_stream),
@@ -2834,11 +2927,21 @@ int sf_hip_labeller_add_plan(sf_labeller *l, sf_plan *plan,
   ga.pixels = (uint32_t)(l->height * l->width);
   ga.inv_width = 1.0 / (double)l->width;
   const uint64_t tiles = (total + sf::kTile - 1) / sf::kTile;
-  sf::gather_chunk<<<(uint32_t)std::max<uint64_t>(
-                         1, std::min<uint64_t>(tiles, 4096)),
-                     256, 0, l->snap_stream>>>(
-      ga, l->snap_table[side].ptr, n_frames, l->snap_nodes[side].first.ptr,
-      l->snap_nodes[side].view());
+  const uint32_t tile_grid =
+      (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, 4096));
+  if (placed) {
+    sf::scatter_chunk<<<dim3(64, std::min<uint32_t>(n_frames, 65535)), 256,
+                        0, l->snap_stream>>>(
+        l->snap_table[side].ptr, n_frames, l->snap_nodes[side].first.ptr,
+        l->snap_nodes[side].view());
+    sf::heads_chunk<<<tile_grid, 256, 0, l->snap_stream>>>(
+        ga, n_frames, l->snap_nodes[side].first.ptr,
+        l->snap_nodes[side].view());
+  } else {
+    sf::gather_chunk<<<tile_grid, 256, 0, l->snap_stream>>>(
+        ga, l->snap_table[side].ptr, n_frames, l->snap_nodes[side].first.ptr,
+        l->snap_nodes[side].view());
+  }
   LAB_CHECK(hipGetLastError(), "snapshot");
   LAB_CHECK(hipEventRecord(l->snap_ready[side], l->snap_stream), "event");
   status = sf::plan_add_reader(plan, l->snap_stream);

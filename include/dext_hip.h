@@ -246,7 +246,12 @@ int sf_hip_plan_replay_pipeline(sf_plan *current, sf_plan *next);
  * sf_hip_plan_order the plan's lists and counts are not yet those of the
  * step, and the plan cannot be the `next` of a step (SF_BAD_ARGUMENT);
  * sf_hip_plan_order with no ordering pending, or a setting change while one
- * is, is SF_BAD_ARGUMENT. A run or replay orders its own lists. */
+ * is, is SF_BAD_ARGUMENT. A run or replay orders its own lists.
+ * sf_hip_labeller_add_plan takes a plan whose ordering is pending: it
+ * enqueues the ordering's places on its own stream and takes the survivors
+ * in list order by them; the lists are then written only by a later
+ * sf_hip_plan_order (before the plan is scouted again), and the next step
+ * that scouts the plan waits for that reader. */
 int sf_hip_plan_set_deferred_order(sf_plan *plan, int deferred);
 int sf_hip_plan_order(sf_plan *plan);
 
@@ -270,11 +275,14 @@ int sf_hip_labeller_add(sf_labeller *labeller, int64_t frame,
 /* Add every frame of a plan's last run, as frames first_frame,
  * first_frame + 1, ... straight from the device lists (no host copy). A
  * frame whose list overflowed (-2) is added empty. Returns without waiting
- * for the plan: the lists are copied on the plan's stream (so the plan may be
- * rebound and run again at once) and a worker thread labels them after the
- * copy. A failure of that labelling is returned by the next call on the
- * labeller (add, add_plan, flush, finish, wait or a query). Holds twice the
- * plan's list capacity in device memory for the copies. */
+ * for the plan: the survivors are gathered into the labeller's node arrays
+ * on a stream of its own, after the plan's work enqueued so far (the plan's
+ * next write of its lists waits for the gather, so the plan may be rebound
+ * and run again at once), and a worker thread labels them after the gather.
+ * A plan whose ordering is pending is taken by the ordering's places (see
+ * sf_hip_plan_set_deferred_order). A failure of that labelling is returned
+ * by the next call on the labeller (add, add_plan, flush, finish, wait or a
+ * query). Holds two chunks' nodes in device memory for the gathers. */
 int sf_hip_labeller_add_plan(sf_labeller *labeller, sf_plan *plan,
                              int64_t first_frame);
 /* Add n frames whose ordered lists are already in device memory, as frame

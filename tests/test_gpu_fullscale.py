@@ -13,6 +13,9 @@ frames generated by the same value models as the measured workloads:
   an 8-frame chunk, through the device labeller: every list against the
   oracle, the Spot/Pixel/Counts tables against the oracle Labeller and the
   reference's own Labeller;
+* the measured steps themselves (pipelined, captured, the ordering left to
+  the labeller) over three 60-frame chunks: tables against a labeller fed
+  the same chunks' ordered lists;
 * the four serving series at their real shapes and pixel types, together
   in one server, against the oracle.
 
@@ -191,6 +194,79 @@ def test_full_size_chunk_through_the_device_labeller(sf):
             assert ref.add(1000 + k, expected[k]) == 0
         ref.finish()
         assert same_tables(got, ref.results())
+
+
+def test_full_size_steps_leaving_the_ordering_to_the_labeller(sf):
+    """The measured path as bench.py runs it: two plans in turn, captured
+    pipelined steps that leave each chunk's ordering pending, and the
+    labeller taking each chunk's survivors by the ordering's places (no
+    list is written). Over three 60-frame chunks of 4362 x 4148 uint16
+    frames and an 8-frame one, its tables equal, row for row, those of a
+    labeller fed the same chunks' ordered lists by plain runs (the path the
+    test above holds to the oracle); a chunk ordered after the hand-over
+    gives the same lists as a plain run."""
+    size, n_chunks = 60, 3
+    n = size * n_chunks + 8
+    buf = sf.DeviceBuffer(2 * H * W * n)
+    spots = fr.diffraction_spots(91, H, W, n, 1500.0, peak=20000.0,
+                                 min_peak=30.0)
+    sf.synth_frames(buf, 2, H, W, n, 91, 0.3, True, 0.0005, spots)
+    ptr = [buf.ptr + 2 * H * W * f for f in range(n)]
+    chunks = [ptr[c * size:(c + 1) * size] for c in range(n_chunks)]
+    # the reference: plain runs, ordered lists, add_plan
+    ref = sf.Labeller(H, W)
+    plain = sf.Plan(2, [(H, W)] * size)
+    lists = []
+    for c, frames in enumerate(chunks):
+        plain.run(frames)
+        plain.wait()
+        counts = plain.counts()
+        if c == 1:
+            lists = [plain.output(k, int(counts[k])) for k in range(size)]
+        ref.add_plan(plain, size * c)
+        ref.wait()
+    tail = sf.Plan(2, [(H, W)] * 8)
+    tail.run(ptr[size * n_chunks:])
+    ref.add_plan(tail, size * n_chunks)
+    ref.finish()
+    want = ref.results()
+    ref.close()
+    # the measured path
+    plans = [sf.Plan(2, [(H, W)] * size) for _ in range(2)]
+    for p in plans:
+        p.set_deferred_order(True)
+    sf.capture_pipeline(plans[0], plans[1])
+    sf.capture_pipeline(plans[1], plans[0])
+    lab = sf.Labeller(H, W)
+    plans[0].bind(chunks[0])
+    sf.pipeline(None, plans[0])
+    for c in range(n_chunks):
+        cur, nxt = plans[c % 2], plans[(c + 1) % 2]
+        if c + 1 < n_chunks:
+            nxt.bind(chunks[c + 1])
+            sf.replay_pipeline(cur, nxt)
+        else:
+            sf.pipeline(cur, None)
+        lab.add_plan(cur, size * c)
+        if c == 1:
+            cur.order()
+            cur.wait()
+            counts = cur.counts()
+            for k in range(size):
+                got = cur.output(k, int(counts[k]))
+                assert got.tobytes() == lists[k].tobytes(), (
+                    k, first_difference(got, lists[k]))
+    tail.run(ptr[size * n_chunks:])
+    lab.add_plan(tail, size * n_chunks)
+    lab.finish()
+    got = lab.results()
+    lab.close()
+    for p in plans + [plain, tail]:
+        p.close()
+    buf.close()
+    assert got[2] == want[2], (got[2], want[2])
+    assert same_tables(got, want)
+    assert want[2]["accepted"] > 10000
 
 
 def test_serving_series_at_real_sizes(sf):

@@ -251,6 +251,57 @@ def test_refusals(sf):
     buf.close()
 
 
+def test_labeller_takes_a_pending_ordering(sf):
+    """Two plans in turn with deferred ordering, each chunk handed to the
+    labeller with its ordering still pending: the labeller enqueues the
+    ordering's places itself and takes the survivors from staging by them,
+    so its tables equal the oracle Labeller's although no list is written;
+    a plan ordered after the hand-over (only the copy is left) still gives
+    the oracle's lists, at 16 and at 32 bits."""
+    for pixel_bytes, (h, w) in ((2, (256, 384)), (4, (300, 517))):
+        shapes = [(h, w)] * 3
+        n = 6
+        if pixel_bytes == 2:
+            chunks = [chunk_frames(sf, shapes, 500 + 10 * c, density=60000.0)
+                      for c in range(n)]
+        else:
+            chunks = [chunk_frames32(sf, shapes, 500 + 10 * c)
+                      for c in range(n)]
+        p = [sf.Plan(pixel_bytes, shapes) for _ in range(2)]
+        for q in p:
+            q.set_deferred_order(True)
+        p[0].bind(chunks[0][1])
+        sf.pipeline(None, p[0])
+        lab = sf.Labeller(h, w)
+        oracle = cpu.Labeller(h, w)
+        for c in range(n):
+            cur, nxt = p[c % 2], p[(c + 1) % 2]
+            if c + 1 < n:
+                nxt.bind(chunks[c + 1][1])
+                sf.pipeline(cur, nxt)
+            else:
+                sf.pipeline(cur, None)
+            lab.add_plan(cur, 3 * c)
+            for k, host in enumerate(chunks[c][2]):
+                assert oracle.add(3 * c + k, cpu.dext(host)[1]) == 0
+            if c % 3 == 1:
+                cur.order()
+                check(cur, chunks[c][2], ("ordered after", pixel_bytes, c))
+        lab.finish()
+        oracle.finish()
+        got, want = lab.results(), oracle.results()
+        assert got[2] == want[2], (pixel_bytes, got[2], want[2])
+        assert got[0].tobytes() == want[0].tobytes()
+        assert got[1].tobytes() == want[1].tobytes()
+        assert want[2]["accepted"] > 5
+        lab.close()
+        for q in p:
+            q.close()
+        for bufs, _, _ in chunks:
+            for b in bufs:
+                b.close()
+
+
 def test_deferred_ordering_three_plans_and_the_labeller(sf):
     """Three plans in turn with deferred ordering: each captured step
     (P[c] -> P[c+1]) leaves P[c]'s ordering to order(), which runs while the
