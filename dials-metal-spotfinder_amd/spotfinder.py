@@ -603,8 +603,9 @@ class Labeller:
     def add_plan(self, plan: "Plan", first_frame: int) -> None:
         """Add every frame of the plan's last run as consecutive frames from
         first_frame, straight from the device lists. Returns at once: the
-        lists are copied on the plan's stream and labelled by a worker
-        thread; a failure there is raised by the next call."""
+        lists are gathered on the labeller's stream (by the ordering's
+        places, where the plan's ordering is pending) and labelled by a
+        worker thread; a failure there is raised by the next call."""
         self._status(lib().sf_hip_labeller_add_plan(
             self.handle, plan.handle, int(first_frame)), "add_plan")
 
